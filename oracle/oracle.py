@@ -1,0 +1,292 @@
+"""ctypes wrapper of the CPU restatement (``liboracle.so``).
+
+TEST INFRASTRUCTURE: this module is the oracle.  Only ``tests/``,
+``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg may import
+it, and only as the checker (or the timed CPU baseline).  The product package
+``taichi_elements_amd`` never imports it.
+
+Particle arrays are AoS numpy arrays: x (N, d), v (N, d), C (N, d, d),
+F (N, d, d), Jp (N,), material (N,) int32.  The grid is a dense node window
+``[grid_lo, grid_lo + grid_n)``; ``grid_m`` has shape grid_n reversed
+(z, y, x) and ``grid_v`` the same plus a trailing axis of length d.
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+MAX_POST = 16
+POST_BBOX, POST_PLANE, POST_SPHERE = 0, 1, 2
+WATER, ELASTIC, SNOW, SAND, STATIONARY = 0, 1, 2, 3, 4
+
+
+class PostOp(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("surface", ctypes.c_int32),
+                ("point", ctypes.c_double * 3), ("normal", ctypes.c_double * 3),
+                ("radius", ctypes.c_double), ("friction", ctypes.c_double)]
+
+
+class Cfg(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int32), ("res", ctypes.c_int32 * 3),
+                ("padding", ctypes.c_int32), ("n_post", ctypes.c_int32),
+                ("dx", ctypes.c_double), ("inv_dx", ctypes.c_double),
+                ("p_vol", ctypes.c_double), ("p_mass", ctypes.c_double),
+                ("mu_0", ctypes.c_double), ("lambda_0", ctypes.c_double),
+                ("alpha", ctypes.c_double), ("water_density", ctypes.c_double),
+                ("gravity", ctypes.c_double * 3),
+                ("grid_lo", ctypes.c_int32 * 3), ("grid_n", ctypes.c_int32 * 3),
+                ("post", PostOp * MAX_POST)]
+
+
+def build():
+    """Compile liboracle.so with the Makefile beside this file."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE, "liboracle.so"], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise ImportError(f"{_LIB_PATH} is missing: run __graft_entry__.build()")
+        _lib = ctypes.CDLL(_LIB_PATH)
+    return _lib
+
+
+def reference_constants(res, size=1, dt_scale=1, E_scale=1):
+    """Host-side constants of MPMSolver.__init__ (mpm_solver.py:82-87, 200-210),
+    computed with Python floats exactly as the reference does."""
+    dim = len(res)
+    dx = size / res[0]
+    inv_dx = 1.0 / dx
+    default_dt = 2e-2 * dx / size * dt_scale
+    p_vol = dx ** dim
+    p_rho = 1000
+    p_mass = p_vol * p_rho
+    E, nu = 1e6 * size * E_scale, 0.2
+    mu_0, lambda_0 = E / (2 * (1 + nu)), E * nu / ((1 + nu) * (1 - 2 * nu))
+    sin_phi = math.sin(math.radians(45))
+    alpha = math.sqrt(2 / 3) * 2 * sin_phi / (3 - sin_phi)
+    return dict(dim=dim, dx=dx, inv_dx=inv_dx, default_dt=default_dt, p_vol=p_vol,
+                p_mass=p_mass, E=E, nu=nu, mu_0=mu_0, lambda_0=lambda_0, alpha=alpha)
+
+
+def make_cfg(res, size=1, padding=3, gravity=None, water_density=1.0, post=None,
+             grid_lo=None, grid_n=None, E_scale=1, dt_scale=1):
+    """Build a Cfg.  ``post`` is a list of dicts: {'kind': 'bbox'} |
+    {'kind': 'plane', 'point', 'normal', 'surface', 'friction'} |
+    {'kind': 'sphere', 'center', 'radius', 'surface'}; default: bbox only
+    (mpm_solver.py:300-302)."""
+    k = reference_constants(res, size, dt_scale, E_scale)
+    dim = k["dim"]
+    c = Cfg()
+    c.dim = dim
+    for a in range(3):
+        c.res[a] = res[a] if a < dim else 1
+    c.padding = padding
+    for name in ("dx", "inv_dx", "p_vol", "p_mass", "mu_0", "lambda_0", "alpha"):
+        setattr(c, name, k[name])
+    c.water_density = water_density
+    if gravity is None:
+        gravity = (0, -9.8, 0) if dim == 3 else (0, -9.8)
+    for a in range(3):
+        c.gravity[a] = gravity[a] if a < len(gravity) else 0.0
+    if grid_lo is None:
+        grid_lo = [-8] * dim
+    if grid_n is None:
+        grid_n = [res[a] + 16 for a in range(dim)]
+    for a in range(3):
+        c.grid_lo[a] = grid_lo[a] if a < dim else 0
+        c.grid_n[a] = grid_n[a] if a < dim else 1
+    if post is None:
+        post = [{"kind": "bbox"}]
+    c.n_post = len(post)
+    for i, op in enumerate(post):
+        p = c.post[i]
+        if op["kind"] == "bbox":
+            p.kind = POST_BBOX
+        elif op["kind"] == "plane":
+            p.kind = POST_PLANE
+            n = list(op["normal"])
+            s = 1.0 / math.sqrt(sum(v * v for v in n))  # mpm_solver.py:654-655
+            for a in range(dim):
+                p.point[a] = op["point"][a]
+                p.normal[a] = s * n[a]
+            p.surface = op.get("surface", 0)
+            p.friction = op.get("friction", 0.0)
+        elif op["kind"] == "sphere":
+            p.kind = POST_SPHERE
+            for a in range(dim):
+                p.point[a] = op["center"][a]
+            p.radius = op["radius"]
+            p.surface = op.get("surface", 0)
+        else:
+            raise ValueError(op)
+    return c
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+def _types(dtype):
+    if np.dtype(dtype) == np.float32:
+        return np.float32, ctypes.c_float, "f32"
+    return np.float64, ctypes.c_double, "f64"
+
+
+def svd(A, dtype=np.float32):
+    npt, ct, suf = _types(dtype)
+    A = np.ascontiguousarray(A, dtype=npt)
+    d = A.shape[0]
+    U = np.zeros((d, d), npt)
+    V = np.zeros((d, d), npt)
+    s = np.zeros(d, npt)
+    fn = getattr(lib(), "oracle_svd_" + suf)
+    fn(ctypes.c_int(d), _ptr(A, ct), _ptr(U, ct), _ptr(s, ct), _ptr(V, ct))
+    return U, s, V
+
+
+def sand_projection(sig, Jp, mu_0, lambda_0, alpha, dtype=np.float32):
+    npt, ct, suf = _types(dtype)
+    s = np.ascontiguousarray(sig, dtype=npt).copy()
+    j = np.array([Jp], dtype=npt)
+    fn = getattr(lib(), "oracle_sand_projection_" + suf)
+    fn(ctypes.c_int(len(s)), _ptr(s, ct), _ptr(j, ct), ctypes.c_double(mu_0),
+       ctypes.c_double(lambda_0), ctypes.c_double(alpha))
+    return s, float(j[0])
+
+
+def constitutive(cfg, material, dt, F, C, Jp, dtype=np.float32):
+    npt, ct, suf = _types(dtype)
+    d = cfg.dim
+    F = np.ascontiguousarray(F, dtype=npt).copy()
+    C = np.ascontiguousarray(C, dtype=npt)
+    j = np.array([Jp], dtype=npt)
+    st = np.zeros((d, d), npt)
+    fn = getattr(lib(), "oracle_constitutive_" + suf)
+    fn(ctypes.byref(cfg), ctypes.c_int(material), ct(dt), _ptr(F, ct), _ptr(C, ct),
+       _ptr(j, ct), _ptr(st, ct))
+    return F, float(j[0]), st
+
+
+def grid_shape(cfg):
+    return tuple(cfg.grid_n[a] for a in reversed(range(cfg.dim)))
+
+
+def _state(state, npt):
+    out = {}
+    for k in ("x", "v", "C", "F", "Jp"):
+        out[k] = np.ascontiguousarray(state[k], dtype=npt).copy()
+    out["material"] = np.ascontiguousarray(state["material"], dtype=np.int32)
+    return out
+
+
+def p2g(cfg, state, dt, dtype=np.float32, nthreads=1):
+    """Returns (grid_m, grid_momentum, new_state) after p2g (grid not normalised)."""
+    npt, ct, suf = _types(dtype)
+    s = _state(state, npt)
+    n = len(s["Jp"])
+    gm = np.zeros(grid_shape(cfg), npt)
+    gv = np.zeros(grid_shape(cfg) + (cfg.dim,), npt)
+    fn = getattr(lib(), "oracle_p2g_" + suf)
+    fn(ctypes.byref(cfg), ctypes.c_int64(n), _ptr(s["x"], ct), _ptr(s["v"], ct),
+       _ptr(s["C"], ct), _ptr(s["F"], ct), _ptr(s["Jp"], ct),
+       _ptr(s["material"], ctypes.c_int32), ct(dt), _ptr(gm, ct), _ptr(gv, ct),
+       ctypes.c_int(nthreads))
+    return gm, gv, s
+
+
+def grid_op(cfg, grid_m, grid_v, dt, t=0.0, dtype=np.float32, nthreads=1):
+    npt, ct, suf = _types(dtype)
+    gm = np.ascontiguousarray(grid_m, dtype=npt).copy()
+    gv = np.ascontiguousarray(grid_v, dtype=npt).copy()
+    fn = getattr(lib(), "oracle_grid_op_" + suf)
+    fn(ctypes.byref(cfg), ct(dt), ctypes.c_double(t), _ptr(gm, ct), _ptr(gv, ct),
+       ctypes.c_int(nthreads))
+    return gm, gv
+
+
+def g2p(cfg, state, grid_v, dt, dtype=np.float32, nthreads=1):
+    npt, ct, suf = _types(dtype)
+    s = _state(state, npt)
+    n = len(s["Jp"])
+    gv = np.ascontiguousarray(grid_v, dtype=npt)
+    mv = np.zeros(1, npt)
+    fn = getattr(lib(), "oracle_g2p_" + suf)
+    fn(ctypes.byref(cfg), ctypes.c_int64(n), _ptr(s["x"], ct), _ptr(s["v"], ct),
+       _ptr(s["C"], ct), _ptr(s["material"], ctypes.c_int32), ct(dt), _ptr(gv, ct),
+       _ptr(mv, ct), ctypes.c_int(nthreads))
+    return s, float(mv[0])
+
+
+class Substepper:
+    """Runs whole substeps in place on one state (grid buffers reused)."""
+
+    def __init__(self, cfg, state, dtype=np.float32, nthreads=1):
+        self.npt, self.ct, suf = _types(dtype)
+        self.cfg = cfg
+        self.s = _state(state, self.npt)
+        self.n = len(self.s["Jp"])
+        self.gm = np.zeros(grid_shape(cfg), self.npt)
+        self.gv = np.zeros(grid_shape(cfg) + (cfg.dim,), self.npt)
+        self.mv = np.zeros(1, self.npt)
+        self.nthreads = nthreads
+        self.fn = getattr(lib(), "oracle_substep_" + suf)
+
+    def substep(self, dt, t=0.0):
+        s, ct = self.s, self.ct
+        self.fn(ctypes.byref(self.cfg), ctypes.c_int64(self.n), _ptr(s["x"], ct),
+                _ptr(s["v"], ct), _ptr(s["C"], ct), _ptr(s["F"], ct), _ptr(s["Jp"], ct),
+                _ptr(s["material"], ctypes.c_int32), ct(dt), ctypes.c_double(t),
+                _ptr(self.gm, ct), _ptr(self.gv, ct), _ptr(self.mv, ct),
+                ctypes.c_int(self.nthreads))
+        return float(self.mv[0])
+
+
+def substep(cfg, state, dt, t=0.0, dtype=np.float32, nthreads=1):
+    """One full substep; returns (new_state, grid_m, grid_v, max_v)."""
+    st = Substepper(cfg, state, dtype, nthreads)
+    mv = st.substep(dt, t)
+    return st.s, st.gm, st.gv, mv
+
+
+def block_keys(x, inv_dx, block_size, blo, bn):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    n, dim = x.shape
+    keys = np.zeros(n, np.int32)
+    blo = np.ascontiguousarray(blo, dtype=np.int32)
+    bn = np.ascontiguousarray(bn, dtype=np.int32)
+    lib().oracle_block_keys(ctypes.c_int(dim), ctypes.c_int64(n), _ptr(x, ctypes.c_float),
+                            ctypes.c_float(inv_dx), ctypes.c_int(block_size),
+                            _ptr(blo, ctypes.c_int32), _ptr(bn, ctypes.c_int32),
+                            _ptr(keys, ctypes.c_int32))
+    return keys
+
+
+def sort_by_key(keys, n_keys):
+    keys = np.ascontiguousarray(keys, dtype=np.int32)
+    perm = np.zeros(len(keys), np.int32)
+    lib().oracle_sort_by_key(ctypes.c_int64(len(keys)), _ptr(keys, ctypes.c_int32),
+                             ctypes.c_int32(n_keys), _ptr(perm, ctypes.c_int32))
+    return perm
+
+
+def rest_state(x, v, material):
+    """Initial particle state as seed_particle writes it (mpm_solver.py:823-835):
+    F = I, C = 0, Jp = 0 for sand and 1 otherwise."""
+    x = np.asarray(x, np.float32)
+    n, d = x.shape
+    material = np.asarray(material, np.int32)
+    return dict(x=x.copy(), v=np.asarray(v, np.float32).copy(),
+                C=np.zeros((n, d, d), np.float32),
+                F=np.tile(np.eye(d, dtype=np.float32), (n, 1, 1)),
+                Jp=np.where(material == SAND, 0.0, 1.0).astype(np.float32),
+                material=material.copy())
