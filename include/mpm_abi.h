@@ -1,0 +1,171 @@
+/*
+ * C ABI of the MI355X MLS-MPM substep engine (libmpm_hip.so).
+ *
+ * This is the drop-in boundary for taichi_elements' MPMSolver
+ * (engine/mpm_solver.py).  The reference has no FFI of its own: its host
+ * Python calls @ti.kernel methods that Taichi JIT-compiles.  Each entry point
+ * below replaces the reference function cited beside it; a Python host
+ * (taichi_elements_amd/mpm_solver.py, via ctypes) keeps the reference's
+ * method signatures on top of it.
+ *
+ * Conventions
+ *  - Plain C types only; host pointers are borrowed for the duration of the
+ *    call and copied.  Particle arrays at this boundary are row-major AoS in
+ *    insertion order: x/v (n, dim) float32, C/F (n, dim, dim) float32,
+ *    Jp (n,) float32, material/color (n,) int32 -- the layout
+ *    MPMSolver.particle_info() returns (mpm_solver.py:1172-1191).
+ *  - Every function returns 0 on success or a negative MPM_E* code; the
+ *    message of the last failure on the calling thread is mpm_last_error().
+ *  - A handle owns its device buffers and one HIP stream; it is not
+ *    thread-safe, but any single thread may drive it (the reference is driven
+ *    from a worker thread by its Blender add-on, blender/operators.py:403).
+ *  - mpm_substeps() only enqueues work; device faults and particles that
+ *    leave the simulated window are reported by the next call that
+ *    synchronises (mpm_read_max_velocity, mpm_particle_info, mpm_get_state,
+ *    mpm_sync).
+ */
+#ifndef MPM_ABI_H
+#define MPM_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPM_OK 0
+#define MPM_EINVAL (-1)   /* bad argument (reference: assert / ValueError) */
+#define MPM_EHIP (-2)     /* HIP runtime error or device fault */
+#define MPM_EDOMAIN (-3)  /* particles left the simulated window */
+#define MPM_ENOMEM (-4)
+
+/* Materials (mpm_solver.py:16-20) and surfaces (mpm_solver.py:32-36). */
+#define MPM_WATER 0
+#define MPM_ELASTIC 1
+#define MPM_SNOW 2
+#define MPM_SAND 3
+#define MPM_STATIONARY 4
+#define MPM_STICKY 0
+#define MPM_SLIP 1
+#define MPM_SEPARATE 2
+
+/* Fields for mpm_get_state / mpm_set_state. */
+#define MPM_FIELD_X 0
+#define MPM_FIELD_V 1
+#define MPM_FIELD_C 2
+#define MPM_FIELD_F 3
+#define MPM_FIELD_JP 4
+#define MPM_FIELD_MATERIAL 5
+#define MPM_FIELD_COLOR 6
+
+typedef struct mpm_solver mpm_solver;
+
+/* Solver constants.  The host computes them exactly as
+ * MPMSolver.__init__ does with Python floats (mpm_solver.py:82-87,
+ * 200-210) and the library rounds them to float32 for the kernels, as
+ * Taichi does with kernel literals. */
+typedef struct {
+  int32_t dim;              /* 2 or 3 = len(res) (mpm_solver.py:67)      */
+  int32_t res[3];           /* cells per axis; res[2] ignored in 2D      */
+  int32_t padding;          /* wall thickness in cells (mpm_solver.py:52) */
+  int32_t device;           /* HIP device ordinal                         */
+  int32_t reorder_interval; /* substeps between physical particle
+                               reorders (0: default, <0: never)          */
+  int32_t chunk;            /* particles per transfer work item (0: default) */
+  double dx, inv_dx, p_vol, p_mass;
+  double mu_0, lambda_0, alpha; /* Lame parameters, sand friction coefficient */
+  double water_density;     /* mpm_solver.py:61, 571-573                  */
+} mpm_config;
+
+/* MPMSolver.__init__ (mpm_solver.py:44-310) minus the Python-side state:
+ * allocates the solver on cfg->device with gravity (0,-9.8[,0]) and an
+ * empty grid post-process list (the Python host adds the bounding box
+ * first, as mpm_solver.py:300-302 does). */
+int mpm_create(const mpm_config* cfg, mpm_solver** out);
+int mpm_destroy(mpm_solver* s);
+/* Message of the last failing call on this thread ("" if none). */
+const char* mpm_last_error(void);
+
+/* MPMSolver.set_gravity (mpm_solver.py:316-319); g has dim entries. */
+int mpm_set_gravity(mpm_solver* s, const double* g);
+
+/* grid_postprocess list (mpm_solver.py:300, 642, 687-692), applied in the
+ * order added after gravity.  unbounded must be 0. */
+int mpm_add_bounding_box(mpm_solver* s, int32_t unbounded);
+/* add_surface_collider (mpm_solver.py:647-687): normal is normalised here
+ * as the reference does on the host; sticky with friction != 0 is
+ * MPM_EINVAL (the reference raises ValueError). */
+int mpm_add_surface_collider(mpm_solver* s, const double* point, const double* normal,
+                             int32_t surface, double friction);
+/* add_sphere_collider (mpm_solver.py:618-642). */
+int mpm_add_sphere_collider(mpm_solver* s, const double* center, double radius,
+                            int32_t surface);
+/* clear_grid_postprocess (mpm_solver.py:644-645): removes every entry,
+ * the bounding box included. */
+int mpm_clear_grid_postprocess(mpm_solver* s);
+
+/* Appends n particles: seed_particle (mpm_solver.py:823-838) as called by
+ * seed_from_external_array / recover_from_external_array
+ * (mpm_solver.py:1081-1126): F = I, C = 0, Jp = 0 for sand else 1.
+ * v may be NULL (zero velocity); color may be NULL (0xFFFFFF). */
+int mpm_add_particles(mpm_solver* s, int64_t n, const float* x, const float* v,
+                      const int32_t* material, const int32_t* color);
+/* n_particles[None] (mpm_solver.py:81). */
+int mpm_num_particles(mpm_solver* s, int64_t* n);
+
+/* n iterations of the substep body of step() (mpm_solver.py:788-799):
+ * build_pid, p2g, grid_normalization_and_gravity, grid_postprocess, g2p,
+ * compute_max_velocity -- all with the same dt; t0 is the time handed to
+ * the post-process of the first substep.  Asynchronous. */
+int mpm_substeps(mpm_solver* s, int32_t n, float dt, double t0);
+/* Per-substep max over particles of max_i |v_i| (compute_max_velocity,
+ * mpm_solver.py:726-735) for the substeps of the last mpm_substeps call;
+ * out has room for that call's n entries.  Synchronises. */
+int mpm_read_max_velocity(mpm_solver* s, int32_t n, float* out);
+
+/* particle_info (mpm_solver.py:1172-1191): position (n,dim), velocity
+ * (n,dim), material (n,), color (n,) in insertion order.  Any pointer may
+ * be NULL to skip that field.  Synchronises. */
+int mpm_particle_info(mpm_solver* s, float* x, float* v, int32_t* material, int32_t* color);
+
+/* Full particle state access in insertion order (no reference counterpart:
+ * the reference exposes its Taichi fields x, v, C, F, Jp, material directly,
+ * mpm_solver.py:101-136).  Used by parity tests, checkpoints and restarts. */
+int mpm_get_state(mpm_solver* s, int32_t field, void* host);
+int mpm_set_state(mpm_solver* s, int32_t field, const void* host);
+
+/* Grid after the last grid operator (no reference counterpart: the
+ * reference's grid_v / grid_m fields, mpm_solver.py:162-186).  Copies the
+ * node window [lo, lo+n) (n[a] nodes along axis a, x fastest) into
+ * m (prod n) and v (prod n * dim); nodes outside the active blocks read 0. */
+int mpm_read_grid(mpm_solver* s, const int32_t* lo, const int32_t* n, float* m, float* v);
+
+/* Binning of the last substep (build_pid, mpm_solver.py:344-361): keys[i] is
+ * the block key of the particle in physical slot i, perm the stable
+ * order of slots by key, slot_to_insertion[i] the insertion index of the
+ * particle in slot i.  Arrays hold mpm_num_particles entries. */
+int mpm_read_binning(mpm_solver* s, int32_t* keys, int32_t* perm, int32_t* slot_to_insertion);
+
+/* Debug switches: bit 0 = grid operator writes raw p2g output (mass and
+ * momentum, no normalisation / gravity / post-process). */
+int mpm_set_debug(mpm_solver* s, int32_t flags);
+
+/* Waits for the solver's stream; reports device faults and domain escapes. */
+int mpm_sync(mpm_solver* s);
+
+/* Kernel timing with HIP events on the solver's stream.  While enabled each
+ * kernel launch of mpm_substeps is bracketed by events; mpm_kernel_times
+ * returns, for every kernel name (newline-separated into names), the number
+ * of launches and the summed milliseconds since the last reset. */
+int mpm_set_timing(mpm_solver* s, int32_t enabled);
+int mpm_kernel_times(mpm_solver* s, char* names, int32_t names_cap, int32_t* counts,
+                     double* ms, int32_t cap, int32_t* n_kernels);
+
+/* Bookkeeping of the last substep: number of valid particles, transfer
+ * work items (chunks), active grid blocks, and particle blocks. */
+int mpm_last_counts(mpm_solver* s, int64_t* out4);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,451 @@
+// Device-side building blocks of the MLS-MPM substep for gfx950:
+// quadratic B-spline stencil set-up, 2x2 / 3x3 SVD, the four constitutive
+// models of p2g and the grid post-process operators.  Every function cites
+// the reference lines of engine/mpm_solver.py that it computes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mpm {
+
+enum Material : int { kWater = 0, kElastic = 1, kSnow = 2, kSand = 3, kStationary = 4 };
+enum PostKind : int { kPostBBox = 0, kPostPlane = 1, kPostSphere = 2 };
+enum Surface : int { kSticky = 0, kSlip = 1, kSeparate = 2 };
+
+constexpr int kMaxPost = 16;
+constexpr int kBlock = 4;  // grid block edge (nodes); 4^3 in 3D, 4^2 in 2D
+constexpr int kTile = kBlock + 2;  // nodes touched by the particles of one block
+
+template <int D>
+struct Geo;
+template <>
+struct Geo<3> {
+  static constexpr int kNodesPerBlock = 64;
+  static constexpr int kTileNodes = 216;
+  static constexpr int kStencil = 27;
+};
+template <>
+struct Geo<2> {
+  static constexpr int kNodesPerBlock = 16;
+  static constexpr int kTileNodes = 36;
+  static constexpr int kStencil = 9;
+};
+
+struct PostOp {
+  int kind, surface;
+  float point[3];
+  float normal[3];
+  float radius, friction;
+};
+
+// Constants of one substep, passed by value to every kernel.  Host-side
+// Python floats of MPMSolver.__init__ (mpm_solver.py:82-87, 200-210) are
+// rounded to f32 here, as the reference's kernels see them.
+struct Params {
+  float dx, inv_dx, dt, p_mass, mu_0, lambda_0, alpha, water_density;
+  float stress_scale;   // (-dt * p_vol) * 4 * inv_dx^2, mpm_solver.py:569
+  float c4;             // 4 * inv_dx, mpm_solver.py:721
+  float sand_coef;      // (d * lambda_0 + 2 mu_0) / (2 mu_0), mpm_solver.py:336-337
+  float gravity[3];
+  int res[3];
+  int padding;
+  int blo[3];           // block window origin (in blocks)
+  int bn[3];            // block window extent (in blocks)
+  int n_post;
+  PostOp post[kMaxPost];
+};
+
+// base = floor(x * inv_dx - 0.5), fx = x * inv_dx - base (mpm_solver.py:497,
+// 503).  Contraction is disabled so the binning pass and both transfers see
+// bit-identical bases.
+__device__ __forceinline__ void base_fx(float xa, float inv_dx, int& base, float& fx) {
+#pragma clang fp contract(off)
+  float t = xa * inv_dx;
+  float u = t - 0.5f;
+  base = (int)floorf(u);
+  fx = t - (float)base;
+}
+
+// Quadratic B-spline weights, mpm_solver.py:505.
+__device__ __forceinline__ void bspline(float f, float w[3]) {
+  w[0] = 0.5f * (1.5f - f) * (1.5f - f);
+  w[1] = 0.75f - (f - 1.0f) * (f - 1.0f);
+  w[2] = 0.5f * (f - 0.5f) * (f - 0.5f);
+}
+
+__device__ __forceinline__ int floor_div4(int b) { return b >= 0 ? (b >> 2) : -((-b + 3) >> 2); }
+
+// --------------------------------------------------------------------------
+// SVD F = U diag(s) V^T, U, V in SO(d), s decreasing, s[d-1] signed.
+// Replaces ti.svd (mpm_solver.py:525).  3x3: Jacobi eigen-analysis of F^T F
+// with the approximate Givens angles of McAdams et al. 2011 (UW-Madison
+// TR1690), 5 cyclic sweeps, a column sort and a Givens QR of F V.
+// --------------------------------------------------------------------------
+__device__ __forceinline__ void approx_givens(float app, float aqq, float apq, float& c, float& s) {
+  const float gamma = 5.82842712474619f;  // 3 + 2 sqrt 2
+  const float cstar = 0.9238795325112867f, sstar = 0.3826834323650898f;  // cos, sin pi/8
+  float ch = 2.0f * (app - aqq), sh = apq;
+  bool ok = gamma * sh * sh < ch * ch;
+  float w = 1.0f / sqrtf(ch * ch + sh * sh);
+  ch = ok ? w * ch : cstar;
+  sh = ok ? w * sh : sstar;
+  c = ch * ch - sh * sh;
+  s = 2.0f * ch * sh;
+}
+
+template <int P, int Q>
+__device__ __forceinline__ void jacobi_step(float S[3][3], float V[3][3]) {
+  constexpr int R = 3 - P - Q;
+  float c, s;
+  approx_givens(S[P][P], S[Q][Q], S[P][Q], c, s);
+  float spp = S[P][P], sqq = S[Q][Q], spq = S[P][Q];
+  float npp = c * c * spp + 2.0f * c * s * spq + s * s * sqq;
+  float nqq = s * s * spp - 2.0f * c * s * spq + c * c * sqq;
+  float npq = (c * c - s * s) * spq + c * s * (sqq - spp);
+  float srp = S[R][P], srq = S[R][Q];
+  S[P][P] = npp;
+  S[Q][Q] = nqq;
+  S[P][Q] = S[Q][P] = npq;
+  S[R][P] = S[P][R] = c * srp + s * srq;
+  S[R][Q] = S[Q][R] = -s * srp + c * srq;
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    float vp = V[i][P], vq = V[i][Q];
+    V[i][P] = c * vp + s * vq;
+    V[i][Q] = -s * vp + c * vq;
+  }
+}
+
+template <int I, int J>
+__device__ __forceinline__ void sort_cols(float B[3][3], float V[3][3]) {
+  float ni = B[0][I] * B[0][I] + B[1][I] * B[1][I] + B[2][I] * B[2][I];
+  float nj = B[0][J] * B[0][J] + B[1][J] * B[1][J] + B[2][J] * B[2][J];
+  if (ni < nj) {
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      float tb = B[r][I];
+      B[r][I] = B[r][J];
+      B[r][J] = -tb;
+      float tv = V[r][I];
+      V[r][I] = V[r][J];
+      V[r][J] = -tv;
+    }
+  }
+}
+
+template <int P, int Q, int D>
+__device__ __forceinline__ void givens_qr(float B[3][3], float U[3][3]) {
+  float a = B[P][P], b = B[Q][P];
+  float r2 = a * a + b * b;
+  float c = 1.0f, s = 0.0f;
+  if (r2 > 0.0f) {
+    float ir = 1.0f / sqrtf(r2);
+    c = a * ir;
+    s = b * ir;
+  }
+#pragma unroll
+  for (int k = 0; k < D; k++) {
+    float bp = B[P][k], bq = B[Q][k];
+    B[P][k] = c * bp + s * bq;
+    B[Q][k] = -s * bp + c * bq;
+    float up = U[k][P], uq = U[k][Q];
+    U[k][P] = c * up + s * uq;
+    U[k][Q] = -s * up + c * uq;
+  }
+}
+
+__device__ __forceinline__ void svd3(const float A[3][3], float U[3][3], float sig[3], float V[3][3]) {
+  float S[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      S[i][j] = A[0][i] * A[0][j] + A[1][i] * A[1][j] + A[2][i] * A[2][j];
+      V[i][j] = i == j ? 1.0f : 0.0f;
+      U[i][j] = i == j ? 1.0f : 0.0f;
+    }
+#pragma unroll
+  for (int sweep = 0; sweep < 5; sweep++) {
+    jacobi_step<0, 1>(S, V);
+    jacobi_step<1, 2>(S, V);
+    jacobi_step<0, 2>(S, V);
+  }
+  float B[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) B[i][j] = A[i][0] * V[0][j] + A[i][1] * V[1][j] + A[i][2] * V[2][j];
+  sort_cols<0, 1>(B, V);
+  sort_cols<0, 2>(B, V);
+  sort_cols<1, 2>(B, V);
+  givens_qr<0, 1, 3>(B, U);
+  givens_qr<0, 2, 3>(B, U);
+  givens_qr<1, 2, 3>(B, U);
+  sig[0] = B[0][0];
+  sig[1] = B[1][1];
+  sig[2] = B[2][2];
+}
+
+// 2x2: one exact Jacobi rotation of F^T F, column sort, one Givens QR.
+__device__ __forceinline__ void svd2(const float A[3][3], float U[3][3], float sig[3], float V[3][3]) {
+  float s00 = A[0][0] * A[0][0] + A[1][0] * A[1][0];
+  float s11 = A[0][1] * A[0][1] + A[1][1] * A[1][1];
+  float s01 = A[0][0] * A[0][1] + A[1][0] * A[1][1];
+  float c = 1.0f, s = 0.0f;
+  if (s01 != 0.0f) {
+    float tau = (s00 - s11) / (2.0f * s01);
+    float t = (tau >= 0.0f ? 1.0f : -1.0f) / (fabsf(tau) + sqrtf(1.0f + tau * tau));
+    c = 1.0f / sqrtf(1.0f + t * t);
+    s = t * c;
+  }
+  V[0][0] = c;
+  V[1][0] = s;
+  V[0][1] = -s;
+  V[1][1] = c;
+  float B[3][3];
+  B[0][0] = A[0][0] * c + A[0][1] * s;
+  B[1][0] = A[1][0] * c + A[1][1] * s;
+  B[0][1] = -A[0][0] * s + A[0][1] * c;
+  B[1][1] = -A[1][0] * s + A[1][1] * c;
+  float n0 = B[0][0] * B[0][0] + B[1][0] * B[1][0];
+  float n1 = B[0][1] * B[0][1] + B[1][1] * B[1][1];
+  if (n0 < n1) {
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+      float tb = B[r][0];
+      B[r][0] = B[r][1];
+      B[r][1] = -tb;
+      float tv = V[r][0];
+      V[r][0] = V[r][1];
+      V[r][1] = -tv;
+    }
+  }
+  U[0][0] = 1.0f;
+  U[0][1] = 0.0f;
+  U[1][0] = 0.0f;
+  U[1][1] = 1.0f;
+  givens_qr<0, 1, 2>(B, U);
+  sig[0] = B[0][0];
+  sig[1] = B[1][1];
+}
+
+// Drucker-Prager return map on the singular values, mpm_solver.py:321-342.
+template <int D>
+__device__ __forceinline__ void sand_projection(float sig[3], float& Jp, const Params& P) {
+  float eps[3], tr = 0.0f;
+#pragma unroll
+  for (int i = 0; i < D; i++) {
+    eps[i] = logf(fmaxf(fabsf(sig[i]), 1e-4f));
+    tr += eps[i];
+  }
+  tr += Jp;
+  float hat[3], n2 = 0.0f;
+#pragma unroll
+  for (int i = 0; i < D; i++) {
+    hat[i] = eps[i] - tr / (float)D;
+    n2 += hat[i] * hat[i];
+  }
+  float hat_norm = sqrtf(n2) + 1e-20f;
+  if (tr >= 0.0f) {
+    Jp = tr;
+#pragma unroll
+    for (int i = 0; i < D; i++) sig[i] = 1.0f;
+  } else {
+    Jp = 0.0f;
+    float dg = hat_norm + P.sand_coef * tr * P.alpha;
+    float k = fmaxf(dg, 0.0f) / hat_norm;
+#pragma unroll
+    for (int i = 0; i < D; i++) sig[i] = expf(eps[i] - k * hat[i]);
+  }
+}
+
+// Per-particle F / Jp update and Kirchhoff stress, mpm_solver.py:507-568.
+// F is updated in place; stress receives tau (before the p2g scale).
+template <int D>
+__device__ __forceinline__ void constitutive(int mat, const Params& P, float F[3][3],
+                                             const float C[3][3], float& Jp, float stress[3][3]) {
+  float G[3][3];
+  if (mat == kWater) {  // mpm_solver.py:508-511
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+      for (int j = 0; j < D; j++) F[i][j] = i == j ? 1.0f : 0.0f;
+    F[0][0] = Jp;
+  }
+#pragma unroll
+  for (int i = 0; i < D; i++)  // (I + dt C) F, mpm_solver.py:513
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+      float s = 0.0f;
+#pragma unroll
+      for (int k = 0; k < D; k++) s += ((i == k ? 1.0f : 0.0f) + P.dt * C[i][k]) * F[k][j];
+      G[i][j] = s;
+    }
+#pragma unroll
+  for (int i = 0; i < D; i++)
+#pragma unroll
+    for (int j = 0; j < D; j++) F[i][j] = G[i][j];
+
+  float h = 1.0f;  // mpm_solver.py:515-524
+  if (mat != kWater) h = expf(10.0f * (1.0f - Jp));
+  if (mat == kElastic) h = 0.3f;
+  float mu = P.mu_0 * h, la = P.lambda_0 * h;
+  if (mat == kWater) mu = 0.0f;
+
+  float U[3][3], V[3][3], sig[3];
+  if (D == 3)
+    svd3(F, U, sig, V);
+  else
+    svd2(F, U, sig, V);
+
+  float J = 1.0f;
+  if (mat != kSand) {  // mpm_solver.py:527-536
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+      float ns = sig[i];
+      if (mat == kSnow) ns = fminf(fmaxf(sig[i], 1.0f - 2.5e-2f), 1.0f + 4.5e-3f);
+      Jp *= sig[i] / ns;
+      sig[i] = ns;
+      J *= ns;
+    }
+  }
+  if (mat == kWater) {  // mpm_solver.py:537-542
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+      for (int j = 0; j < D; j++) F[i][j] = i == j ? 1.0f : 0.0f;
+    F[0][0] = J;
+    Jp = J;
+  }
+  if (mat == kSand) sand_projection<D>(sig, Jp, P);  // mpm_solver.py:554
+  if (mat == kSnow || mat == kSand) {                // F = U sig V^T, mpm_solver.py:545, 555
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+      for (int j = 0; j < D; j++) {
+        float s = 0.0f;
+#pragma unroll
+        for (int k = 0; k < D; k++) s += U[i][k] * sig[k] * V[j][k];
+        F[i][j] = s;
+      }
+  }
+  if (mat != kSand) {  // tau = 2 mu (F - U V^T) F^T + la J (J - 1) I, mpm_solver.py:549-551
+    float Dm[3][3];
+    const float two_mu = 2.0f * mu;
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+      for (int j = 0; j < D; j++) {
+        float r = 0.0f;
+#pragma unroll
+        for (int k = 0; k < D; k++) r += U[i][k] * V[j][k];
+        Dm[i][j] = two_mu * (F[i][j] - r);
+      }
+    const float iso = la * J * (J - 1.0f);
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+      for (int j = 0; j < D; j++) {
+        float s = 0.0f;
+#pragma unroll
+        for (int k = 0; k < D; k++) s += Dm[i][k] * F[j][k];
+        stress[i][j] = s + (i == j ? iso : 0.0f);
+      }
+  } else {  // tau = U diag(2 mu ln s / s + la sum ln s / s) V^T F^T, mpm_solver.py:556-566
+    float lsum = 0.0f, center[3];
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+      float l = logf(sig[i]);
+      lsum += l;
+      center[i] = 2.0f * P.mu_0 * l * (1.0f / sig[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < D; i++) center[i] += P.lambda_0 * lsum * (1.0f / sig[i]);
+    float T[3][3];
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+      for (int j = 0; j < D; j++) {
+        float s = 0.0f;
+#pragma unroll
+        for (int k = 0; k < D; k++) s += U[i][k] * center[k] * V[j][k];
+        T[i][j] = s;
+      }
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+      for (int j = 0; j < D; j++) {
+        float s = 0.0f;
+#pragma unroll
+        for (int k = 0; k < D; k++) s += T[i][k] * F[j][k];
+        stress[i][j] = s;
+      }
+  }
+}
+
+// grid_postprocess operators on one node velocity (mpm_solver.py:600-687).
+template <int D>
+__device__ __forceinline__ void apply_post(const Params& P, const int I[3], float v[3]) {
+  for (int k = 0; k < P.n_post; k++) {
+    const PostOp& op = P.post[k];
+    if (op.kind == kPostBBox) {  // mpm_solver.py:613-616
+#pragma unroll
+      for (int a = 0; a < D; a++) {
+        if (I[a] < P.padding && v[a] < 0.0f) v[a] = 0.0f;
+        if (I[a] >= P.res[a] - P.padding && v[a] > 0.0f) v[a] = 0.0f;
+      }
+    } else if (op.kind == kPostPlane) {  // mpm_solver.py:660-685
+      float dot = 0.0f;
+#pragma unroll
+      for (int a = 0; a < D; a++) dot += ((float)I[a] * P.dx - op.point[a]) * op.normal[a];
+      if (dot < 0.0f) {
+        if (op.surface == kSticky) {
+#pragma unroll
+          for (int a = 0; a < D; a++) v[a] = 0.0f;
+        } else {
+          float nc = 0.0f;
+#pragma unroll
+          for (int a = 0; a < D; a++) nc += op.normal[a] * v[a];
+          float proj = op.surface == kSlip ? nc : fminf(nc, 0.0f);
+          float vn2 = 0.0f;
+#pragma unroll
+          for (int a = 0; a < D; a++) {
+            v[a] = v[a] - op.normal[a] * proj;
+            vn2 += v[a] * v[a];
+          }
+          float vn = sqrtf(vn2);
+          if (nc < 0.0f && vn > 1e-30f) {
+            float mag = fmaxf(0.0f, vn + nc * op.friction);
+            float inv = 1.0f / vn;
+#pragma unroll
+            for (int a = 0; a < D; a++) v[a] = v[a] * inv * mag;
+          }
+        }
+      }
+    } else if (op.kind == kPostSphere) {  // mpm_solver.py:621-640
+      float off[3], n2 = 0.0f;
+#pragma unroll
+      for (int a = 0; a < D; a++) {
+        off[a] = (float)I[a] * P.dx - op.point[a];
+        n2 += off[a] * off[a];
+      }
+      if (n2 < op.radius * op.radius) {
+        if (op.surface == kSticky) {
+#pragma unroll
+          for (int a = 0; a < D; a++) v[a] = 0.0f;
+        } else {
+          float inv = 1.0f / (sqrtf(n2) + 1e-5f);
+          float nc = 0.0f;
+#pragma unroll
+          for (int a = 0; a < D; a++) nc += off[a] * inv * v[a];
+          float proj = op.surface == kSlip ? nc : fminf(nc, 0.0f);
+#pragma unroll
+          for (int a = 0; a < D; a++) v[a] = v[a] - off[a] * inv * proj;
+        }
+      }
+    }
+  }
+}
+
+}  // namespace mpm

@@ -1,0 +1,92 @@
+// Launch interface of the substep kernels (implemented in mpm_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mpm_device.h"
+
+namespace mpm {
+
+// Float particle components, each a [cap] array inside one allocation:
+// x (D), v (D), C (D*D), F (D*D), Jp (1).  Int components: material, slot
+// insertion index.
+template <int D>
+struct Layout {
+  static constexpr int X = 0, V = D, C = 2 * D, F = 2 * D + D * D, JP = 2 * D + 2 * D * D;
+  static constexpr int NF = 2 * D + 2 * D * D + 1;
+  static constexpr int MAT = 0, ORIG = 1, NI = 2;
+};
+
+constexpr int kThreads = 256;
+constexpr int kScanItems = 8;                       // per thread in the block scans
+constexpr int kScanTile = kThreads * kScanItems;    // 2048 blocks per workgroup
+constexpr int kRadixItems = 16;
+constexpr int kRadixTile = kThreads * kRadixItems;  // 4096 keys per workgroup
+constexpr int kRadixBins = 256;
+
+// Totals written on the device by the block scan (read by persistent kernels).
+enum Total : int { kTotValid = 0, kTotChunks = 1, kTotActive = 2, kTotPBlocks = 3, kTotCount = 4 };
+
+struct SubstepBuffers {
+  float* pf;  // particle float components
+  int* pi;    // particle int components
+  int64_t cap;
+  int n;
+  int* keys;        // [n] block key per slot
+  int* keys_alt;    // radix ping-pong
+  int* perm;        // [n] sorted slot order (radix output values)
+  int* perm_alt;
+  int* count;       // [total_blocks] particles per block (zero on entry)
+  int* count_next;  // [total_blocks] zeroed for the next substep
+  int* pstart;      // [total_blocks] first sorted position of the block
+  int* pchunk_begin;
+  int* pchunk_cnt;
+  int4* chunks;     // (block, begin, end, 0)
+  int* active;      // active grid blocks
+  int4* partials;   // block-scan partial sums
+  int* totals;      // kTot*
+  int* hist;        // radix histograms [256 * ntiles]
+  int* hist_partials;
+  float* tiles;     // p2g per-chunk tiles
+  float* grid;      // dense block-major grid [total_blocks][D+1][nodes]
+  int* err;         // error bits
+  unsigned* maxv;   // per-substep max |v| (float bits)
+  int total_blocks;
+  int key_bits;
+  int chunk;
+  int persistent_groups;
+};
+
+class Timer;
+
+template <int D>
+void launch_substep(const Params& P, SubstepBuffers& B, int substep_index, int raw_grid,
+                    hipStream_t st, Timer* timer);
+
+template <int D>
+void launch_reorder(SubstepBuffers& B, float* pf_alt, int* pi_alt, hipStream_t st, Timer* timer);
+
+template <int D>
+void launch_append(SubstepBuffers& B, const float* x, const float* v, const int* mat, int k,
+                   int n_old, hipStream_t st);
+
+void launch_gather_f(const float* src, int64_t cap, int ncomp, const int* orig, int n,
+                     float* dst, hipStream_t st);
+void launch_gather_i(const int* src, int64_t cap, int ncomp, const int* orig, int n, int* dst,
+                     hipStream_t st);
+void launch_scatter_f(const float* src, int64_t cap, int ncomp, const int* orig, int n,
+                      float* dst, hipStream_t st);
+void launch_scatter_i(const int* src, int64_t cap, int ncomp, const int* orig, int n, int* dst,
+                      hipStream_t st);
+void launch_fill_u32(unsigned* p, unsigned v, int64_t n, hipStream_t st);
+
+// Event-bracketed kernel timing (enabled on demand).
+class Timer {
+ public:
+  virtual ~Timer() {}
+  virtual void begin(const char* name, hipStream_t st) = 0;
+  virtual void end(hipStream_t st) = 0;
+};
+
+}  // namespace mpm

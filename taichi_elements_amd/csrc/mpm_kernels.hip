@@ -1,0 +1,813 @@
+// gfx950 kernels of the MLS-MPM substep.
+//
+// One substep (mpm_solver.py:789-799) is:
+//   k_keys            block key of every particle + per-block counts   (build_pid, mpm:344-361)
+//   k_bscan_*         scan of the dense block window: particle offsets,
+//                     transfer work items (chunks), active grid blocks  (replaces the pointer
+//                                                                        SNodes and deactivate_all)
+//   k_radix_*         stable LSD radix sort of slots by block key
+//   k_p2g             per chunk: constitutive update + scatter into an
+//                     LDS tile of (4+2)^d nodes, tile written to HBM     (p2g, mpm:487-584)
+//   k_grid_op         per active node: sum of the overlapping tiles,
+//                     normalisation, gravity, post-process              (mpm:586-692)
+//   k_g2p             per chunk: LDS tile of grid velocity, gather,
+//                     advection, max |v|                                (g2p, mpm:694-735)
+#include "mpm_kernels.h"
+
+namespace mpm {
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int4 add4(int4 a, int4 b) {
+  return make_int4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+}
+
+__device__ __forceinline__ int4 wave_inclusive_scan4(int4 v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    int4 t;
+    t.x = __shfl_up(v.x, off);
+    t.y = __shfl_up(v.y, off);
+    t.z = __shfl_up(v.z, off);
+    t.w = __shfl_up(v.w, off);
+    if (lane >= off) v = add4(v, t);
+  }
+  return v;
+}
+
+// Exclusive scan of one int4 per thread over a kThreads workgroup.
+__device__ __forceinline__ int4 block_exscan4(int4 v, int4* total, int4* wsum) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int4 inc = wave_inclusive_scan4(v);
+  if (lane == 63) wsum[wave] = inc;
+  __syncthreads();
+  int4 pre = make_int4(0, 0, 0, 0), tot = make_int4(0, 0, 0, 0);
+#pragma unroll
+  for (int w = 0; w < kThreads / 64; w++) {
+    if (w < wave) pre = add4(pre, wsum[w]);
+    tot = add4(tot, wsum[w]);
+  }
+  __syncthreads();
+  *total = tot;
+  return make_int4(pre.x + inc.x - v.x, pre.y + inc.y - v.y, pre.z + inc.z - v.z,
+                   pre.w + inc.w - v.w);
+}
+
+template <int D>
+__device__ __forceinline__ void decode_local(int bid, const Params& P, int lb[3]) {
+  lb[0] = bid % P.bn[0];
+  int r = bid / P.bn[0];
+  if (D == 3) {
+    lb[1] = r % P.bn[1];
+    lb[2] = r / P.bn[1];
+  } else {
+    lb[1] = r;
+    lb[2] = 0;
+  }
+}
+
+template <int D>
+__device__ __forceinline__ int encode_local(const int lb[3], const Params& P) {
+  return D == 3 ? (lb[2] * P.bn[1] + lb[1]) * P.bn[0] + lb[0] : lb[1] * P.bn[0] + lb[0];
+}
+
+// ---------------------------------------------------------------------------
+// binning: block keys and per-block counts (build_pid, mpm_solver.py:357-361)
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_keys(Params P, const float* __restrict__ pf,
+                                                   int64_t cap, int n, int total_blocks,
+                                                   int* __restrict__ keys,
+                                                   int* __restrict__ count, int* __restrict__ err) {
+  using L = Layout<D>;
+  const int lane = threadIdx.x & 63;
+  for (int i0 = blockIdx.x * kThreads; i0 < n; i0 += gridDim.x * kThreads) {
+    const int i = i0 + threadIdx.x;
+    bool valid = false;
+    int key = total_blocks;
+    if (i < n) {
+      int lin = 0;
+      valid = true;
+#pragma unroll
+      for (int a = D - 1; a >= 0; a--) {
+        int base;
+        float fx;
+        base_fx(pf[(int64_t)(L::X + a) * cap + i], P.inv_dx, base, fx);
+        const int lb = (base >> 2) - P.blo[a];
+        // the block and its +1 neighbour must lie inside the window
+        if (lb < 0 || lb >= P.bn[a] - 1) valid = false;
+        lin = lin * P.bn[a] + lb;
+      }
+      key = valid ? lin : total_blocks;
+      keys[i] = key;
+      if (!valid) atomicOr(err, 1);
+    }
+    // wave-aggregated counting: sorted input gives 1-3 distinct keys per wave
+    unsigned long long pending = __ballot(valid);
+    while (pending) {
+      const int leader = __ffsll(pending) - 1;
+      const int lk = __shfl(key, leader);
+      const unsigned long long same = __ballot(valid && key == lk);
+      if (lane == leader) atomicAdd(&count[lk], (int)__popcll(same));
+      pending &= ~same;
+    }
+  }
+}
+
+// Per-block values of the dense window scan: (particles, chunks, active grid
+// block, non-empty particle block).  A grid block is active when one of the
+// particle blocks b - {0,1}^d holds particles (their stencils reach it).
+template <int D>
+__device__ __forceinline__ int4 block_values(int bid, const int* __restrict__ count,
+                                             const Params& P, int chunk) {
+  const int c = count[bid];
+  int lb[3];
+  decode_local<D>(bid, P, lb);
+  bool act = false;
+#pragma unroll
+  for (int dz = 0; dz < (D == 3 ? 2 : 1); dz++)
+#pragma unroll
+    for (int dy = 0; dy < 2; dy++)
+#pragma unroll
+      for (int dx = 0; dx < 2; dx++) {
+        int q[3] = {lb[0] - dx, lb[1] - dy, lb[2] - dz};
+        if (q[0] < 0 || q[1] < 0 || q[2] < 0) continue;
+        act |= count[encode_local<D>(q, P)] > 0;
+      }
+  return make_int4(c, (c + chunk - 1) / chunk, act ? 1 : 0, c > 0 ? 1 : 0);
+}
+
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_bscan_reduce(Params P, const int* __restrict__ count,
+                                                           int total_blocks, int chunk,
+                                                           int4* __restrict__ partials) {
+  __shared__ int4 wsum[kThreads / 64];
+  int4 acc = make_int4(0, 0, 0, 0);
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) {
+    const int bid = blockIdx.x * kScanTile + k * kThreads + threadIdx.x;
+    if (bid < total_blocks) acc = add4(acc, block_values<D>(bid, count, P, chunk));
+  }
+  int4 tot;
+  block_exscan4(acc, &tot, wsum);
+  if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kThreads) void k_bscan_top(int4* __restrict__ partials, int nparts,
+                                                        int* __restrict__ totals) {
+  __shared__ int4 wsum[kThreads / 64];
+  int4 carry = make_int4(0, 0, 0, 0);
+  for (int base = 0; base < nparts; base += kThreads) {
+    const int i = base + threadIdx.x;
+    int4 v = i < nparts ? partials[i] : make_int4(0, 0, 0, 0);
+    int4 tot;
+    int4 ex = block_exscan4(v, &tot, wsum);
+    if (i < nparts) partials[i] = add4(ex, carry);
+    carry = add4(carry, tot);
+  }
+  if (threadIdx.x == 0) {
+    totals[kTotValid] = carry.x;
+    totals[kTotChunks] = carry.y;
+    totals[kTotActive] = carry.z;
+    totals[kTotPBlocks] = carry.w;
+  }
+}
+
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_bscan_down(
+    Params P, const int* __restrict__ count, int* __restrict__ count_next, int total_blocks,
+    int chunk, const int4* __restrict__ partials, int* __restrict__ pstart,
+    int* __restrict__ pchunk_begin, int* __restrict__ pchunk_cnt, int4* __restrict__ chunks,
+    int* __restrict__ active) {
+  __shared__ int4 wsum[kThreads / 64];
+  const int first = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  int4 vals[kScanItems];
+  int4 acc = make_int4(0, 0, 0, 0);
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) {
+    const int bid = first + k;
+    vals[k] = bid < total_blocks ? block_values<D>(bid, count, P, chunk) : make_int4(0, 0, 0, 0);
+    acc = add4(acc, vals[k]);
+  }
+  int4 tot;
+  int4 pre = add4(block_exscan4(acc, &tot, wsum), partials[blockIdx.x]);
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) {
+    const int bid = first + k;
+    if (bid >= total_blocks) break;
+    const int4 v = vals[k];
+    pstart[bid] = pre.x;
+    pchunk_begin[bid] = pre.y;
+    pchunk_cnt[bid] = v.y;
+    for (int c = 0; c < v.y; c++) {
+      const int b0 = pre.x + c * chunk;
+      const int b1 = min(b0 + chunk, pre.x + v.x);
+      chunks[pre.y + c] = make_int4(bid, b0, b1, 0);
+    }
+    if (v.z) active[pre.z] = bid;
+    count_next[bid] = 0;
+    pre = add4(pre, v);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// flat exclusive scan of ints (radix histograms)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_scan_reduce(const int* __restrict__ in, int n,
+                                                          int4* __restrict__ partials) {
+  __shared__ int4 wsum[kThreads / 64];
+  int acc = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) {
+    const int i = blockIdx.x * kScanTile + k * kThreads + threadIdx.x;
+    if (i < n) acc += in[i];
+  }
+  int4 tot;
+  block_exscan4(make_int4(acc, 0, 0, 0), &tot, wsum);
+  if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kThreads) void k_scan_down(int* __restrict__ data, int n,
+                                                        const int4* __restrict__ partials) {
+  __shared__ int4 wsum[kThreads / 64];
+  const int first = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  int vals[kScanItems];
+  int acc = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) {
+    vals[k] = first + k < n ? data[first + k] : 0;
+    acc += vals[k];
+  }
+  int4 tot;
+  int pre = block_exscan4(make_int4(acc, 0, 0, 0), &tot, wsum).x + partials[blockIdx.x].x;
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) {
+    if (first + k < n) data[first + k] = pre;
+    pre += vals[k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// stable LSD radix sort, 8-bit digits, 4096 keys per workgroup
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_radix_hist(const int* __restrict__ keys, int n,
+                                                         int shift, int ntiles,
+                                                         int* __restrict__ hist) {
+  __shared__ int cnt[kRadixBins];
+  const int lane = threadIdx.x & 63;
+  const int tile = blockIdx.x;
+  cnt[threadIdx.x] = 0;
+  __syncthreads();
+#pragma unroll 4
+  for (int r = 0; r < kRadixItems; r++) {
+    const int i = tile * kRadixTile + r * kThreads + threadIdx.x;
+    const bool valid = i < n;
+    const int d = valid ? (keys[i] >> shift) & (kRadixBins - 1) : 0;
+    unsigned long long pending = __ballot(valid);
+    while (pending) {
+      const int leader = __ffsll(pending) - 1;
+      const int ld = __shfl(d, leader);
+      const unsigned long long same = __ballot(valid && d == ld);
+      if (lane == leader) atomicAdd(&cnt[ld], (int)__popcll(same));
+      pending &= ~same;
+    }
+  }
+  __syncthreads();
+  hist[threadIdx.x * ntiles + tile] = cnt[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kThreads) void k_radix_scatter(
+    const int* __restrict__ keys_in, const int* __restrict__ vals_in, int* __restrict__ keys_out,
+    int* __restrict__ vals_out, int n, int shift, int ntiles, const int* __restrict__ hist_ex) {
+  __shared__ int wcnt[kThreads / 64][kRadixBins];
+  __shared__ int run[kRadixBins];
+  const int tile = blockIdx.x;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  run[threadIdx.x] = hist_ex[threadIdx.x * ntiles + tile];
+  for (int r = 0; r < kRadixItems; r++) {
+    const int i = tile * kRadixTile + r * kThreads + threadIdx.x;
+    const bool valid = i < n;
+    const int key = valid ? keys_in[i] : 0;
+    const int val = valid ? (vals_in ? vals_in[i] : i) : 0;
+    const int d = (key >> shift) & (kRadixBins - 1);
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; w++) wcnt[w][threadIdx.x] = 0;
+    __syncthreads();
+    unsigned long long peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      const bool bit = (d >> b) & 1;
+      const unsigned long long bal = __ballot(bit);
+      peers &= bit ? bal : ~bal;
+    }
+    const int rank = (int)__popcll(peers & lt);
+    if (valid && rank == 0) wcnt[wave][d] = (int)__popcll(peers);
+    __syncthreads();
+    {
+      int x = run[threadIdx.x];
+#pragma unroll
+      for (int w = 0; w < kThreads / 64; w++) {
+        const int c = wcnt[w][threadIdx.x];
+        wcnt[w][threadIdx.x] = x;
+        x += c;
+      }
+      run[threadIdx.x] = x;
+    }
+    __syncthreads();
+    if (valid) {
+      const int pos = wcnt[wave][d] + rank;
+      keys_out[pos] = key;
+      vals_out[pos] = val;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// P2G (mpm_solver.py:487-584)
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_p2g(Params P, float* __restrict__ pf,
+                                                  const int* __restrict__ pi, int64_t cap,
+                                                  const int* __restrict__ perm,
+                                                  const int4* __restrict__ chunks,
+                                                  const int* __restrict__ totals,
+                                                  float* __restrict__ tiles) {
+  using L = Layout<D>;
+  constexpr int TN = Geo<D>::kTileNodes, NC = D + 1;
+  __shared__ float tile[NC * TN];
+  const int nchunks = totals[kTotChunks];
+  for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    const int4 ck = chunks[c];
+    int b[3];
+    decode_local<D>(ck.x, P, b);
+#pragma unroll
+    for (int a = 0; a < D; a++) b[a] = (b[a] + P.blo[a]) * kBlock;  // first node of the block
+    for (int i = threadIdx.x; i < NC * TN; i += kThreads) tile[i] = 0.0f;
+    __syncthreads();
+    for (int j = ck.y + threadIdx.x; j < ck.z; j += kThreads) {
+      const int p = perm[j];
+      const int mat = pi[(int64_t)L::MAT * cap + p];
+      float xp[3], vp[3], C[3][3], F[3][3], stress[3][3];
+#pragma unroll
+      for (int a = 0; a < D; a++) {
+        xp[a] = pf[(int64_t)(L::X + a) * cap + p];
+        vp[a] = pf[(int64_t)(L::V + a) * cap + p];
+      }
+#pragma unroll
+      for (int a = 0; a < D; a++)
+#pragma unroll
+        for (int e = 0; e < D; e++) {
+          C[a][e] = pf[(int64_t)(L::C + a * D + e) * cap + p];
+          F[a][e] = pf[(int64_t)(L::F + a * D + e) * cap + p];
+        }
+      float Jp = pf[(int64_t)L::JP * cap + p];
+      int base[3], l[3];
+      float fx[3], w[3][3];
+      bool inside = true;
+#pragma unroll
+      for (int a = 0; a < D; a++) {
+        base_fx(xp[a], P.inv_dx, base[a], fx[a]);
+        bspline(fx[a], w[a]);
+        l[a] = base[a] - b[a];
+        inside &= (l[a] >= 0) & (l[a] < kBlock);
+      }
+      constitutive<D>(mat, P, F, C, Jp, stress);
+#pragma unroll
+      for (int a = 0; a < D; a++)
+#pragma unroll
+        for (int e = 0; e < D; e++) pf[(int64_t)(L::F + a * D + e) * cap + p] = F[a][e];
+      pf[(int64_t)L::JP * cap + p] = Jp;
+      if (!inside) continue;  // unreachable: keys and base come from the same x
+
+      float mass = P.p_mass;  // mpm_solver.py:571-573
+      if (mat == kWater) mass *= P.water_density;
+      float A[3][3], mv[3];
+#pragma unroll
+      for (int a = 0; a < D; a++) {
+        mv[a] = mass * vp[a];
+#pragma unroll
+        for (int e = 0; e < D; e++) A[a][e] = P.stress_scale * stress[a][e] + mass * C[a][e];
+      }
+#pragma unroll
+      for (int oz = 0; oz < (D == 3 ? 3 : 1); oz++)
+#pragma unroll
+        for (int oy = 0; oy < 3; oy++)
+#pragma unroll
+          for (int ox = 0; ox < 3; ox++) {
+            const int o[3] = {ox, oy, oz};
+            float weight = 1.0f, dpos[3];
+#pragma unroll
+            for (int a = 0; a < D; a++) {
+              weight *= w[a][o[a]];
+              dpos[a] = ((float)o[a] - fx[a]) * P.dx;
+            }
+            const int node = D == 3 ? (l[0] + ox) + kTile * ((l[1] + oy) + kTile * (l[2] + oz))
+                                    : (l[0] + ox) + kTile * (l[1] + oy);
+#pragma unroll
+            for (int a = 0; a < D; a++) {
+              float ad = 0.0f;
+#pragma unroll
+              for (int e = 0; e < D; e++) ad += A[a][e] * dpos[e];
+              atomicAdd(&tile[a * TN + node], weight * (mv[a] + ad));
+            }
+            atomicAdd(&tile[D * TN + node], weight * mass);
+          }
+    }
+    __syncthreads();
+    float* out = tiles + (int64_t)c * NC * TN;
+    for (int i = threadIdx.x; i < NC * TN; i += kThreads) out[i] = tile[i];
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// grid operator (mpm_solver.py:586-692)
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_grid_op(Params P, const int* __restrict__ active,
+                                                      const int* __restrict__ totals,
+                                                      const int* __restrict__ pchunk_begin,
+                                                      const int* __restrict__ pchunk_cnt,
+                                                      const float* __restrict__ tiles,
+                                                      float* __restrict__ grid, int raw) {
+  constexpr int NPB = Geo<D>::kNodesPerBlock, TN = Geo<D>::kTileNodes, NC = D + 1;
+  const int64_t total = (int64_t)totals[kTotActive] * NPB;
+  for (int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * kThreads) {
+    const int blk = (int)(t / NPB), node = (int)(t % NPB);
+    const int bid = active[blk];
+    int lb[3];
+    decode_local<D>(bid, P, lb);
+    const int l[3] = {node & 3, (node >> 2) & 3, node >> 4};
+    float m = 0.0f, p[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int dz = 0; dz < (D == 3 ? 2 : 1); dz++)
+#pragma unroll
+      for (int dy = 0; dy < 2; dy++)
+#pragma unroll
+        for (int dx = 0; dx < 2; dx++) {
+          const int d[3] = {dx, dy, dz};
+          int q[3], tl[3];
+          bool ok = true;
+#pragma unroll
+          for (int a = 0; a < D; a++) {
+            tl[a] = l[a] + kBlock * d[a];
+            q[a] = lb[a] - d[a];
+            ok &= (tl[a] < kTile) & (q[a] >= 0);
+          }
+          if (!ok) continue;
+          const int nb = encode_local<D>(q, P);
+          const int c0 = pchunk_begin[nb], cn = pchunk_cnt[nb];
+          const int tn = D == 3 ? tl[0] + kTile * (tl[1] + kTile * tl[2]) : tl[0] + kTile * tl[1];
+          for (int k = c0; k < c0 + cn; k++) {
+            const float* tp = tiles + (int64_t)k * NC * TN + tn;
+#pragma unroll
+            for (int a = 0; a < D; a++) p[a] += tp[a * TN];
+            m += tp[D * TN];
+          }
+        }
+    if (!raw) {
+      if (m > 0.0f) {  // mpm_solver.py:591-593
+        const float inv = 1.0f / m;
+#pragma unroll
+        for (int a = 0; a < D; a++) p[a] = inv * p[a];
+#pragma unroll
+        for (int a = 0; a < D; a++) p[a] += P.dt * P.gravity[a];
+      }
+      int I[3];
+#pragma unroll
+      for (int a = 0; a < D; a++) I[a] = (lb[a] + P.blo[a]) * kBlock + l[a];
+      apply_post<D>(P, I, p);
+    }
+    float* g = grid + (int64_t)bid * NC * NPB + node;
+#pragma unroll
+    for (int a = 0; a < D; a++) g[a * NPB] = p[a];
+    g[D * NPB] = m;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// G2P (mpm_solver.py:694-735)
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_g2p(Params P, float* __restrict__ pf,
+                                                  const int* __restrict__ pi, int64_t cap,
+                                                  const int* __restrict__ perm,
+                                                  const int4* __restrict__ chunks,
+                                                  const int* __restrict__ totals,
+                                                  const float* __restrict__ grid,
+                                                  unsigned* __restrict__ maxv) {
+  using L = Layout<D>;
+  constexpr int NPB = Geo<D>::kNodesPerBlock, TN = Geo<D>::kTileNodes, NC = D + 1;
+  __shared__ float tile[D * TN];
+  __shared__ float red[kThreads / 64];
+  float vmax = 0.0f;
+  const int nchunks = totals[kTotChunks];
+  for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    const int4 ck = chunks[c];
+    int lb[3];
+    decode_local<D>(ck.x, P, lb);
+    for (int i = threadIdx.x; i < TN; i += kThreads) {
+      const int tl[3] = {i % kTile, (i / kTile) % kTile, i / (kTile * kTile)};
+      int q[3], ln[3];
+#pragma unroll
+      for (int a = 0; a < D; a++) {
+        const int hi = tl[a] >= kBlock;
+        q[a] = lb[a] + hi;
+        ln[a] = tl[a] - kBlock * hi;
+      }
+      const int gb = encode_local<D>(q, P);
+      const int node = D == 3 ? ln[0] + 4 * (ln[1] + 4 * ln[2]) : ln[0] + 4 * ln[1];
+      const float* g = grid + (int64_t)gb * NC * NPB + node;
+#pragma unroll
+      for (int a = 0; a < D; a++) tile[a * TN + i] = g[a * NPB];
+    }
+    __syncthreads();
+    int b[3];
+#pragma unroll
+    for (int a = 0; a < D; a++) b[a] = (lb[a] + P.blo[a]) * kBlock;
+    for (int j = ck.y + threadIdx.x; j < ck.z; j += kThreads) {
+      const int p = perm[j];
+      const int mat = pi[(int64_t)L::MAT * cap + p];
+      float xp[3], fx[3], w[3][3];
+      int l[3];
+      bool inside = true;
+#pragma unroll
+      for (int a = 0; a < D; a++) {
+        xp[a] = pf[(int64_t)(L::X + a) * cap + p];
+        int base;
+        base_fx(xp[a], P.inv_dx, base, fx[a]);
+        bspline(fx[a], w[a]);
+        l[a] = base - b[a];
+        inside &= (l[a] >= 0) & (l[a] < kBlock);
+      }
+      float nv[3] = {0.0f, 0.0f, 0.0f}, nC[3][3] = {{0.0f}};
+      if (inside) {
+#pragma unroll
+        for (int oz = 0; oz < (D == 3 ? 3 : 1); oz++)
+#pragma unroll
+          for (int oy = 0; oy < 3; oy++)
+#pragma unroll
+            for (int ox = 0; ox < 3; ox++) {
+              const int o[3] = {ox, oy, oz};
+              float weight = 1.0f, dpos[3];
+#pragma unroll
+              for (int a = 0; a < D; a++) {
+                weight *= w[a][o[a]];
+                dpos[a] = (float)o[a] - fx[a];
+              }
+              const int node = D == 3 ? (l[0] + ox) + kTile * ((l[1] + oy) + kTile * (l[2] + oz))
+                                      : (l[0] + ox) + kTile * (l[1] + oy);
+              const float cw = P.c4 * weight;
+#pragma unroll
+              for (int a = 0; a < D; a++) {
+                const float gv = tile[a * TN + node];
+                nv[a] += weight * gv;
+#pragma unroll
+                for (int e = 0; e < D; e++) nC[a][e] += cw * (gv * dpos[e]);
+              }
+            }
+      }
+      if (mat != kStationary) {  // mpm_solver.py:722-724
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+          pf[(int64_t)(L::V + a) * cap + p] = nv[a];
+          pf[(int64_t)(L::X + a) * cap + p] = xp[a] + P.dt * nv[a];
+          vmax = fmaxf(vmax, fabsf(nv[a]));
+#pragma unroll
+          for (int e = 0; e < D; e++) pf[(int64_t)(L::C + a * D + e) * cap + p] = nC[a][e];
+        }
+      } else {
+#pragma unroll
+        for (int a = 0; a < D; a++) vmax = fmaxf(vmax, fabsf(pf[(int64_t)(L::V + a) * cap + p]));
+      }
+    }
+    __syncthreads();
+  }
+  // compute_max_velocity (mpm_solver.py:726-735): non-negative floats order
+  // like their bit patterns.
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = vmax;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float m = red[0];
+#pragma unroll
+    for (int w = 1; w < kThreads / 64; w++) m = fmaxf(m, red[w]);
+    atomicMax(maxv, __float_as_uint(m));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// particle bookkeeping kernels
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_reorder(const float* __restrict__ pf,
+                                                      float* __restrict__ pf2,
+                                                      const int* __restrict__ pi,
+                                                      int* __restrict__ pi2,
+                                                      const int* __restrict__ perm, int n,
+                                                      int64_t cap) {
+  using L = Layout<D>;
+  for (int j = blockIdx.x * kThreads + threadIdx.x; j < n; j += gridDim.x * kThreads) {
+    const int p = perm[j];
+#pragma unroll
+    for (int c = 0; c < L::NF; c++) pf2[(int64_t)c * cap + j] = pf[(int64_t)c * cap + p];
+#pragma unroll
+    for (int c = 0; c < L::NI; c++) pi2[(int64_t)c * cap + j] = pi[(int64_t)c * cap + p];
+  }
+}
+
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_append(const float* __restrict__ x,
+                                                     const float* __restrict__ v,
+                                                     const int* __restrict__ mat, int k,
+                                                     int n_old, float* __restrict__ pf,
+                                                     int* __restrict__ pi, int64_t cap) {
+  using L = Layout<D>;
+  const int t = blockIdx.x * kThreads + threadIdx.x;
+  if (t >= k) return;
+  const int s = n_old + t;
+  const int m = mat[t];
+#pragma unroll
+  for (int a = 0; a < D; a++) {
+    pf[(int64_t)(L::X + a) * cap + s] = x[(int64_t)t * D + a];
+    pf[(int64_t)(L::V + a) * cap + s] = v ? v[(int64_t)t * D + a] : 0.0f;
+#pragma unroll
+    for (int e = 0; e < D; e++) {
+      pf[(int64_t)(L::C + a * D + e) * cap + s] = 0.0f;
+      pf[(int64_t)(L::F + a * D + e) * cap + s] = a == e ? 1.0f : 0.0f;
+    }
+  }
+  pf[(int64_t)L::JP * cap + s] = m == kSand ? 0.0f : 1.0f;  // mpm_solver.py:831-835
+  pi[(int64_t)L::MAT * cap + s] = m;
+  pi[(int64_t)L::ORIG * cap + s] = s;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_gather(const T* __restrict__ src, int64_t cap,
+                                                     int ncomp, const int* __restrict__ orig,
+                                                     int n, T* __restrict__ dst) {
+  for (int j = blockIdx.x * kThreads + threadIdx.x; j < n; j += gridDim.x * kThreads) {
+    const int o = orig[j];
+    for (int c = 0; c < ncomp; c++) dst[(int64_t)o * ncomp + c] = src[(int64_t)c * cap + j];
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_scatter(const T* __restrict__ src, int64_t cap,
+                                                      int ncomp, const int* __restrict__ orig,
+                                                      int n, T* __restrict__ dst) {
+  for (int j = blockIdx.x * kThreads + threadIdx.x; j < n; j += gridDim.x * kThreads) {
+    const int o = orig[j];
+    for (int c = 0; c < ncomp; c++) dst[(int64_t)c * cap + j] = src[(int64_t)o * ncomp + c];
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_fill_u32(unsigned* __restrict__ p, unsigned v,
+                                                       int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kThreads)
+    p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline int div_up(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+static inline int clamp_groups(int64_t g, int cap) {
+  return (int)std::max<int64_t>(1, std::min<int64_t>(g, cap));
+}
+
+struct TimerScope {
+  Timer* t;
+  hipStream_t st;
+  TimerScope(Timer* t_, const char* name, hipStream_t s) : t(t_), st(s) {
+    if (t) t->begin(name, st);
+  }
+  ~TimerScope() {
+    if (t) t->end(st);
+  }
+};
+
+template <int D>
+void launch_substep(const Params& P, SubstepBuffers& B, int s_index, int raw, hipStream_t st,
+                    Timer* tm) {
+  using L = Layout<D>;
+  const int n = B.n;
+  const int G = B.persistent_groups;
+  if (n == 0) return;
+  {
+    TimerScope ts(tm, "k_keys", st);
+    k_keys<D><<<clamp_groups(div_up(n, kThreads), 8192), kThreads, 0, st>>>(
+        P, B.pf, B.cap, n, B.total_blocks, B.keys, B.count, B.err);
+  }
+  const int nparts = div_up(B.total_blocks, kScanTile);
+  {
+    TimerScope ts(tm, "k_bscan", st);
+    k_bscan_reduce<D><<<nparts, kThreads, 0, st>>>(P, B.count, B.total_blocks, B.chunk, B.partials);
+    k_bscan_top<<<1, kThreads, 0, st>>>(B.partials, nparts, B.totals);
+    k_bscan_down<D><<<nparts, kThreads, 0, st>>>(P, B.count, B.count_next, B.total_blocks, B.chunk,
+                                                 B.partials, B.pstart, B.pchunk_begin,
+                                                 B.pchunk_cnt, B.chunks, B.active);
+  }
+  {
+    TimerScope ts(tm, "k_radix", st);
+    const int ntiles = div_up(n, kRadixTile);
+    const int hn = kRadixBins * ntiles;
+    const int hparts = div_up(hn, kScanTile);
+    const int passes = (B.key_bits + 7) / 8;
+    const int* kin = B.keys;
+    const int* vin = nullptr;
+    int* kbuf[2] = {B.keys_alt, B.keys_alt + B.cap};
+    int* vbuf[2] = {B.perm, B.perm_alt};
+    for (int pass = 0; pass < passes; pass++) {
+      k_radix_hist<<<ntiles, kThreads, 0, st>>>(kin, n, pass * 8, ntiles, B.hist);
+      k_scan_reduce<<<hparts, kThreads, 0, st>>>(B.hist, hn, B.partials + nparts);
+      k_bscan_top<<<1, kThreads, 0, st>>>(B.partials + nparts, hparts, B.totals + kTotCount);
+      k_scan_down<<<hparts, kThreads, 0, st>>>(B.hist, hn, B.partials + nparts);
+      k_radix_scatter<<<ntiles, kThreads, 0, st>>>(kin, vin, kbuf[pass & 1], vbuf[pass & 1], n,
+                                                    pass * 8, ntiles, B.hist);
+      kin = kbuf[pass & 1];
+      vin = vbuf[pass & 1];
+    }
+    if (vin != B.perm) std::swap(B.perm, B.perm_alt);
+  }
+  const int chunk_bound = std::min(B.total_blocks, n) + div_up(n, B.chunk);
+  {
+    TimerScope ts(tm, "k_p2g", st);
+    k_p2g<D><<<clamp_groups(chunk_bound, G), kThreads, 0, st>>>(P, B.pf, B.pi, B.cap, B.perm,
+                                                               B.chunks, B.totals, B.tiles);
+  }
+  {
+    TimerScope ts(tm, "k_grid_op", st);
+    const int64_t node_bound = (int64_t)B.total_blocks * Geo<D>::kNodesPerBlock;
+    k_grid_op<D><<<clamp_groups(div_up(node_bound, kThreads), 4 * G), kThreads, 0, st>>>(
+        P, B.active, B.totals, B.pchunk_begin, B.pchunk_cnt, B.tiles, B.grid, raw);
+  }
+  {
+    TimerScope ts(tm, "k_g2p", st);
+    k_g2p<D><<<clamp_groups(chunk_bound, G), kThreads, 0, st>>>(
+        P, B.pf, B.pi, B.cap, B.perm, B.chunks, B.totals, B.grid, B.maxv + s_index);
+  }
+  std::swap(B.count, B.count_next);
+  (void)L::NF;
+}
+
+template <int D>
+void launch_reorder(SubstepBuffers& B, float* pf_alt, int* pi_alt, hipStream_t st, Timer* tm) {
+  TimerScope ts(tm, "k_reorder", st);
+  k_reorder<D><<<clamp_groups(div_up(B.n, kThreads), 8192), kThreads, 0, st>>>(
+      B.pf, pf_alt, B.pi, pi_alt, B.perm, B.n, B.cap);
+}
+
+template <int D>
+void launch_append(SubstepBuffers& B, const float* x, const float* v, const int* mat, int k,
+                   int n_old, hipStream_t st) {
+  if (k <= 0) return;
+  k_append<D><<<div_up(k, kThreads), kThreads, 0, st>>>(x, v, mat, k, n_old, B.pf, B.pi, B.cap);
+}
+
+void launch_gather_f(const float* src, int64_t cap, int ncomp, const int* orig, int n,
+                     float* dst, hipStream_t st) {
+  if (n > 0)
+    k_gather<float><<<clamp_groups(div_up(n, kThreads), 8192), kThreads, 0, st>>>(src, cap, ncomp,
+                                                                                 orig, n, dst);
+}
+void launch_gather_i(const int* src, int64_t cap, int ncomp, const int* orig, int n, int* dst,
+                     hipStream_t st) {
+  if (n > 0)
+    k_gather<int><<<clamp_groups(div_up(n, kThreads), 8192), kThreads, 0, st>>>(src, cap, ncomp,
+                                                                             orig, n, dst);
+}
+void launch_scatter_f(const float* src, int64_t cap, int ncomp, const int* orig, int n,
+                      float* dst, hipStream_t st) {
+  if (n > 0)
+    k_scatter<float><<<clamp_groups(div_up(n, kThreads), 8192), kThreads, 0, st>>>(src, cap, ncomp,
+                                                                                  orig, n, dst);
+}
+void launch_scatter_i(const int* src, int64_t cap, int ncomp, const int* orig, int n, int* dst,
+                      hipStream_t st) {
+  if (n > 0)
+    k_scatter<int><<<clamp_groups(div_up(n, kThreads), 8192), kThreads, 0, st>>>(src, cap, ncomp,
+                                                                              orig, n, dst);
+}
+void launch_fill_u32(unsigned* p, unsigned v, int64_t n, hipStream_t st) {
+  if (n > 0)
+    k_fill_u32<<<clamp_groups(div_up(n, kThreads), 4096), kThreads, 0, st>>>(p, v, n);
+}
+
+template void launch_substep<2>(const Params&, SubstepBuffers&, int, int, hipStream_t, Timer*);
+template void launch_substep<3>(const Params&, SubstepBuffers&, int, int, hipStream_t, Timer*);
+template void launch_reorder<2>(SubstepBuffers&, float*, int*, hipStream_t, Timer*);
+template void launch_reorder<3>(SubstepBuffers&, float*, int*, hipStream_t, Timer*);
+template void launch_append<2>(SubstepBuffers&, const float*, const float*, const int*, int, int,
+                               hipStream_t);
+template void launch_append<3>(SubstepBuffers&, const float*, const float*, const int*, int, int,
+                               hipStream_t);
+
+}  // namespace mpm

@@ -1,0 +1,757 @@
+// Host runtime and C ABI of the MLS-MPM substep engine (include/mpm_abi.h).
+//
+// A solver handle owns, on one device and one HIP stream:
+//  - particle state as structure-of-arrays in "slots" (float components
+//    x, v, C, F, Jp; int components material, insertion index), kept
+//    approximately sorted by grid block by a periodic physical reorder;
+//  - the binning / scan / tile / grid buffers of the substep;
+//  - the grid post-process list (bounding box and colliders).
+// Colors never take part in a substep and stay on the host in insertion
+// order.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/mpm_abi.h"
+#include "mpm_kernels.h"
+
+using namespace mpm;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(MPM_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));          \
+  } while (0)
+
+struct EventTimer : public Timer {
+  struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> pool;
+  std::map<std::string, std::pair<int, double>> acc;
+  std::vector<std::string> order;
+
+  hipEvent_t get() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    hipEventCreate(&e);
+    return e;
+  }
+  void begin(const char* name, hipStream_t st) override {
+    Pending p{name, get(), get()};
+    hipEventRecord(p.a, st);
+    pending.push_back(p);
+  }
+  void end(hipStream_t st) override { hipEventRecord(pending.back().b, st); }
+  // Called after a stream synchronisation.
+  void collect() {
+    for (auto& p : pending) {
+      float ms = 0.0f;
+      hipEventElapsedTime(&ms, p.a, p.b);
+      if (!acc.count(p.name)) order.push_back(p.name);
+      auto& e = acc[p.name];
+      e.first += 1;
+      e.second += ms;
+      pool.push_back(p.a);
+      pool.push_back(p.b);
+    }
+    pending.clear();
+  }
+  ~EventTimer() override {
+    for (auto& p : pending) {
+      hipEventDestroy(p.a);
+      hipEventDestroy(p.b);
+    }
+    for (auto e : pool) hipEventDestroy(e);
+  }
+};
+
+}  // namespace
+
+struct mpm_solver {
+  mpm_config cfg{};
+  int D = 3;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Params P{};  // dt filled per call
+  double gravity[3] = {0, 0, 0};
+  std::vector<PostOp> post;
+
+  // block window
+  int blo[3] = {0, 0, 0}, bn[3] = {1, 1, 1};
+  int total_blocks = 0;
+  int key_bits = 0;
+
+  // particles
+  int64_t n = 0, cap = 0;
+  float* pf = nullptr;
+  float* pf_alt = nullptr;
+  int* pi = nullptr;
+  int* pi_alt = nullptr;
+  std::vector<int32_t> color;  // insertion order, host side
+
+  // substep buffers sized by particle capacity
+  int* keys = nullptr;
+  int* keys_alt = nullptr;  // 2 * cap
+  int* perm = nullptr;
+  int* perm_alt = nullptr;
+  int* hist = nullptr;
+  float* tiles = nullptr;
+  int4* chunks = nullptr;
+  int64_t chunk_cap = 0;
+  // buffers sized by the block window
+  int* count = nullptr;
+  int* count_next = nullptr;
+  int* pstart = nullptr;
+  int* pchunk_begin = nullptr;
+  int* pchunk_cnt = nullptr;
+  int* active = nullptr;
+  int4* partials = nullptr;
+  float* grid = nullptr;
+  int* totals = nullptr;
+  int* err = nullptr;
+  unsigned* maxv = nullptr;
+  int maxv_cap = 0;
+  int last_batch = 0;
+
+  bool have_perm = false;  // perm of the previous substep matches the slots
+  int since_reorder = 0;
+  int reorder_interval = 16;
+  int chunk = 512;
+  int persistent_groups = 2048;
+  int debug = 0;
+  bool timing = false;
+  EventTimer timer;
+
+  int nf() const { return D == 3 ? Layout<3>::NF : Layout<2>::NF; }
+};
+
+namespace {
+
+template <typename T>
+int dalloc(T** p, int64_t count) {
+  *p = nullptr;
+  if (count <= 0) count = 1;
+  hipError_t e = hipMalloc((void**)p, sizeof(T) * (size_t)count);
+  if (e != hipSuccess)
+    return fail(MPM_ENOMEM, std::string("hipMalloc of ") + std::to_string(sizeof(T) * count) +
+                                " bytes failed: " + hipGetErrorString(e));
+  return MPM_OK;
+}
+
+template <typename T>
+void dfree(T*& p) {
+  if (p) hipFree((void*)p);
+  p = nullptr;
+}
+
+#define TRY(expr)           \
+  do {                      \
+    int r_ = (expr);        \
+    if (r_ != MPM_OK) return r_; \
+  } while (0)
+
+int check_device_errors(mpm_solver* s) {
+  HIP_TRY(hipSetDevice(s->device));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (s->timing) s->timer.collect();
+  int e = 0;
+  HIP_TRY(hipMemcpy(&e, s->err, sizeof(int), hipMemcpyDeviceToHost));
+  if (e & 1) {
+    int z = 0;
+    HIP_TRY(hipMemcpy(s->err, &z, sizeof(int), hipMemcpyHostToDevice));
+    return fail(MPM_EDOMAIN,
+                "particles left the simulated window (positions outside the grid around the "
+                "domain); they were frozen");
+  }
+  return MPM_OK;
+}
+
+void fill_params(mpm_solver* s, float dt) {
+  Params& P = s->P;
+  const mpm_config& c = s->cfg;
+  P.dx = (float)c.dx;
+  P.inv_dx = (float)c.inv_dx;
+  P.dt = dt;
+  P.p_mass = (float)c.p_mass;
+  P.mu_0 = (float)c.mu_0;
+  P.lambda_0 = (float)c.lambda_0;
+  P.alpha = (float)c.alpha;
+  P.water_density = (float)c.water_density;
+  // (-dt * p_vol * 4 * inv_dx**2) evaluated left to right in f32 (mpm:569)
+  P.stress_scale = ((-dt * (float)c.p_vol) * 4.0f) * (float)(c.inv_dx * c.inv_dx);
+  P.c4 = (float)(4 * c.inv_dx);
+  P.sand_coef = (float)((s->D * c.lambda_0 + 2 * c.mu_0) / (2 * c.mu_0));
+  for (int a = 0; a < 3; a++) {
+    P.gravity[a] = a < s->D ? (float)s->gravity[a] : 0.0f;
+    P.res[a] = a < s->D ? c.res[a] : 1;
+    P.blo[a] = s->blo[a];
+    P.bn[a] = s->bn[a];
+  }
+  P.padding = c.padding;
+  P.n_post = (int)s->post.size();
+  for (size_t k = 0; k < s->post.size(); k++) P.post[k] = s->post[k];
+}
+
+int free_particle_buffers(mpm_solver* s) {
+  dfree(s->pf);
+  dfree(s->pf_alt);
+  dfree(s->pi);
+  dfree(s->pi_alt);
+  dfree(s->keys);
+  dfree(s->keys_alt);
+  dfree(s->perm);
+  dfree(s->perm_alt);
+  dfree(s->hist);
+  dfree(s->tiles);
+  dfree(s->chunks);
+  return MPM_OK;
+}
+
+int grow(mpm_solver* s, int64_t need) {
+  if (need <= s->cap) return MPM_OK;
+  if (need > (int64_t)INT32_MAX / 2) return fail(MPM_EINVAL, "too many particles (max 2^30)");
+  int64_t ncap = std::max<int64_t>(need, std::max<int64_t>(1024, s->cap * 3 / 2));
+  const int nf = s->nf();
+  float *pf = nullptr, *pf_alt = nullptr;
+  int *pi = nullptr, *pi_alt = nullptr;
+  TRY(dalloc(&pf, (int64_t)nf * ncap));
+  TRY(dalloc(&pf_alt, (int64_t)nf * ncap));
+  TRY(dalloc(&pi, 2 * ncap));
+  TRY(dalloc(&pi_alt, 2 * ncap));
+  if (s->n > 0) {
+    HIP_TRY(hipMemcpy2DAsync(pf, ncap * sizeof(float), s->pf, s->cap * sizeof(float),
+                             s->n * sizeof(float), nf, hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(hipMemcpy2DAsync(pi, ncap * sizeof(int), s->pi, s->cap * sizeof(int),
+                             s->n * sizeof(int), 2, hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+  }
+  free_particle_buffers(s);
+  s->pf = pf;
+  s->pf_alt = pf_alt;
+  s->pi = pi;
+  s->pi_alt = pi_alt;
+  s->cap = ncap;
+  TRY(dalloc(&s->keys, ncap));
+  TRY(dalloc(&s->keys_alt, 2 * ncap));
+  TRY(dalloc(&s->perm, ncap));
+  TRY(dalloc(&s->perm_alt, ncap));
+  const int64_t ntiles = (ncap + kRadixTile - 1) / kRadixTile;
+  TRY(dalloc(&s->hist, kRadixBins * ntiles));
+  s->chunk_cap = std::min<int64_t>(s->total_blocks, ncap) + (ncap + s->chunk - 1) / s->chunk;
+  const int TN = s->D == 3 ? Geo<3>::kTileNodes : Geo<2>::kTileNodes;
+  TRY(dalloc(&s->tiles, s->chunk_cap * (s->D + 1) * TN));
+  TRY(dalloc(&s->chunks, s->chunk_cap));
+  // partials: block-window scan parts + radix-histogram scan parts
+  const int64_t nparts = (s->total_blocks + kScanTile - 1) / kScanTile;
+  const int64_t hparts = (kRadixBins * ntiles + kScanTile - 1) / kScanTile;
+  dfree(s->partials);
+  TRY(dalloc(&s->partials, nparts + hparts + 1));
+  s->have_perm = false;
+  return MPM_OK;
+}
+
+void fill_buffers(mpm_solver* s, SubstepBuffers& B) {
+  B.pf = s->pf;
+  B.pi = s->pi;
+  B.cap = s->cap;
+  B.n = (int)s->n;
+  B.keys = s->keys;
+  B.keys_alt = s->keys_alt;
+  B.perm = s->perm;
+  B.perm_alt = s->perm_alt;
+  B.count = s->count;
+  B.count_next = s->count_next;
+  B.pstart = s->pstart;
+  B.pchunk_begin = s->pchunk_begin;
+  B.pchunk_cnt = s->pchunk_cnt;
+  B.chunks = s->chunks;
+  B.active = s->active;
+  B.partials = s->partials;
+  B.totals = s->totals;
+  B.hist = s->hist;
+  B.tiles = s->tiles;
+  B.grid = s->grid;
+  B.err = s->err;
+  B.maxv = s->maxv;
+  B.total_blocks = s->total_blocks;
+  B.key_bits = s->key_bits;
+  B.chunk = s->chunk;
+  B.persistent_groups = s->persistent_groups;
+}
+
+void take_buffers(mpm_solver* s, const SubstepBuffers& B) {
+  s->perm = B.perm;
+  s->perm_alt = B.perm_alt;
+  s->count = B.count;
+  s->count_next = B.count_next;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mpm_last_error(void) { return g_last_error.c_str(); }
+
+int mpm_create(const mpm_config* cfg, mpm_solver** out) {
+  if (!cfg || !out) return fail(MPM_EINVAL, "null argument");
+  *out = nullptr;
+  if (cfg->dim != 2 && cfg->dim != 3)
+    return fail(MPM_EINVAL, "MPM solver supports only 2D and 3D simulations.");
+  for (int a = 0; a < cfg->dim; a++)
+    if (cfg->res[a] <= 0 || cfg->res[a] > 65536) return fail(MPM_EINVAL, "bad resolution");
+  if (!(cfg->dx > 0) || !(cfg->inv_dx > 0)) return fail(MPM_EINVAL, "bad dx");
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  if (cfg->device < 0 || cfg->device >= ndev)
+    return fail(MPM_EINVAL, "device " + std::to_string(cfg->device) + " not present (" +
+                                std::to_string(ndev) + " visible)");
+  mpm_solver* s = new mpm_solver();
+  s->cfg = *cfg;
+  s->D = cfg->dim;
+  s->device = cfg->device;
+  if (cfg->reorder_interval > 0) s->reorder_interval = cfg->reorder_interval;
+  if (cfg->reorder_interval < 0) s->reorder_interval = 0;
+  if (cfg->chunk > 0) s->chunk = cfg->chunk;
+  s->gravity[1] = -9.8;  // mpm_solver.py:283, 296
+  // Block window: 2 blocks below the domain, enough above it for the wall
+  // padding plus a margin; a particle block must also have its +1 neighbour
+  // inside the window.
+  for (int a = 0; a < 3; a++) {
+    if (a < s->D) {
+      s->blo[a] = -2;
+      s->bn[a] = (cfg->res[a] + 3) / 4 + 5;
+    } else {
+      s->blo[a] = 0;
+      s->bn[a] = 1;
+    }
+  }
+  int64_t tb = (int64_t)s->bn[0] * s->bn[1] * s->bn[2];
+  if (tb >= (1 << 28)) {
+    delete s;
+    return fail(MPM_EINVAL, "grid too large for the dense block window");
+  }
+  s->total_blocks = (int)tb;
+  s->key_bits = 1;
+  while ((1ll << s->key_bits) <= tb) s->key_bits++;  // key == total_blocks marks escapes
+  int r = MPM_OK;
+  if (hipSetDevice(s->device) != hipSuccess || hipStreamCreate(&s->stream) != hipSuccess) {
+    delete s;
+    return fail(MPM_EHIP, "cannot create a HIP stream");
+  }
+  const int NPB = s->D == 3 ? Geo<3>::kNodesPerBlock : Geo<2>::kNodesPerBlock;
+  if ((r = dalloc(&s->count, tb)) || (r = dalloc(&s->count_next, tb)) ||
+      (r = dalloc(&s->pstart, tb)) || (r = dalloc(&s->pchunk_begin, tb)) ||
+      (r = dalloc(&s->pchunk_cnt, tb)) || (r = dalloc(&s->active, tb)) ||
+      (r = dalloc(&s->grid, tb * (s->D + 1) * NPB)) || (r = dalloc(&s->totals, 16)) ||
+      (r = dalloc(&s->err, 1)) || (r = dalloc(&s->maxv, 1024))) {
+    mpm_destroy(s);
+    return r;
+  }
+  s->maxv_cap = 1024;
+  if (hipMemsetAsync(s->count, 0, sizeof(int) * tb, s->stream) != hipSuccess ||
+      hipMemsetAsync(s->count_next, 0, sizeof(int) * tb, s->stream) != hipSuccess ||
+      hipMemsetAsync(s->grid, 0, sizeof(float) * tb * (s->D + 1) * NPB, s->stream) != hipSuccess ||
+      hipMemsetAsync(s->totals, 0, sizeof(int) * 16, s->stream) != hipSuccess ||
+      hipMemsetAsync(s->err, 0, sizeof(int), s->stream) != hipSuccess ||
+      hipStreamSynchronize(s->stream) != hipSuccess) {
+    mpm_destroy(s);
+    return fail(MPM_EHIP, "initialising solver buffers failed");
+  }
+  *out = s;
+  return MPM_OK;
+}
+
+int mpm_destroy(mpm_solver* s) {
+  if (!s) return MPM_OK;
+  hipSetDevice(s->device);
+  if (s->stream) hipStreamSynchronize(s->stream);
+  free_particle_buffers(s);
+  dfree(s->count);
+  dfree(s->count_next);
+  dfree(s->pstart);
+  dfree(s->pchunk_begin);
+  dfree(s->pchunk_cnt);
+  dfree(s->active);
+  dfree(s->partials);
+  dfree(s->grid);
+  dfree(s->totals);
+  dfree(s->err);
+  dfree(s->maxv);
+  if (s->stream) hipStreamDestroy(s->stream);
+  delete s;
+  return MPM_OK;
+}
+
+int mpm_set_gravity(mpm_solver* s, const double* g) {
+  if (!s || !g) return fail(MPM_EINVAL, "null argument");
+  for (int a = 0; a < s->D; a++) s->gravity[a] = g[a];
+  return MPM_OK;
+}
+
+int mpm_add_bounding_box(mpm_solver* s, int32_t unbounded) {
+  if (!s) return fail(MPM_EINVAL, "null argument");
+  if (unbounded) return fail(MPM_EINVAL, "unbounded simulations are not supported");
+  if (s->post.size() >= (size_t)kMaxPost) return fail(MPM_EINVAL, "too many grid post-processes");
+  PostOp op{};
+  op.kind = kPostBBox;
+  s->post.push_back(op);
+  return MPM_OK;
+}
+
+int mpm_add_surface_collider(mpm_solver* s, const double* point, const double* normal,
+                             int32_t surface, double friction) {
+  if (!s || !point || !normal) return fail(MPM_EINVAL, "null argument");
+  if (surface < 0 || surface > 2) return fail(MPM_EINVAL, "unknown surface type");
+  if (surface == MPM_STICKY && friction != 0)
+    return fail(MPM_EINVAL, "friction must be 0 on sticky surfaces.");
+  if (s->post.size() >= (size_t)kMaxPost) return fail(MPM_EINVAL, "too many grid post-processes");
+  double n2 = 0;
+  for (int a = 0; a < s->D; a++) n2 += normal[a] * normal[a];
+  if (!(n2 > 0)) return fail(MPM_EINVAL, "collider normal must be non-zero");
+  const double scale = 1.0 / std::sqrt(n2);  // mpm_solver.py:654-655
+  PostOp op{};
+  op.kind = kPostPlane;
+  op.surface = surface;
+  op.friction = (float)friction;
+  for (int a = 0; a < s->D; a++) {
+    op.point[a] = (float)point[a];
+    op.normal[a] = (float)(scale * normal[a]);
+  }
+  s->post.push_back(op);
+  return MPM_OK;
+}
+
+int mpm_add_sphere_collider(mpm_solver* s, const double* center, double radius, int32_t surface) {
+  if (!s || !center) return fail(MPM_EINVAL, "null argument");
+  if (surface < 0 || surface > 2) return fail(MPM_EINVAL, "unknown surface type");
+  if (s->post.size() >= (size_t)kMaxPost) return fail(MPM_EINVAL, "too many grid post-processes");
+  PostOp op{};
+  op.kind = kPostSphere;
+  op.surface = surface;
+  op.radius = (float)radius;
+  for (int a = 0; a < s->D; a++) op.point[a] = (float)center[a];
+  s->post.push_back(op);
+  return MPM_OK;
+}
+
+int mpm_clear_grid_postprocess(mpm_solver* s) {
+  if (!s) return fail(MPM_EINVAL, "null argument");
+  s->post.clear();
+  return MPM_OK;
+}
+
+int mpm_add_particles(mpm_solver* s, int64_t k, const float* x, const float* v,
+                      const int32_t* material, const int32_t* color) {
+  if (!s) return fail(MPM_EINVAL, "null argument");
+  if (k < 0) return fail(MPM_EINVAL, "negative particle count");
+  if (k == 0) return MPM_OK;
+  if (!x || !material) return fail(MPM_EINVAL, "x and material are required");
+  for (int64_t i = 0; i < k; i++)
+    if (material[i] < 0 || material[i] > MPM_STATIONARY)
+      return fail(MPM_EINVAL, "unknown material id " + std::to_string(material[i]));
+  HIP_TRY(hipSetDevice(s->device));
+  TRY(grow(s, s->n + k));
+  const int D = s->D;
+  float* dx = nullptr;
+  float* dv = nullptr;
+  int* dm = nullptr;
+  TRY(dalloc(&dx, k * D));
+  if (v) TRY(dalloc(&dv, k * D));
+  TRY(dalloc(&dm, k));
+  HIP_TRY(hipMemcpyAsync(dx, x, sizeof(float) * k * D, hipMemcpyHostToDevice, s->stream));
+  if (v) HIP_TRY(hipMemcpyAsync(dv, v, sizeof(float) * k * D, hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(hipMemcpyAsync(dm, material, sizeof(int) * k, hipMemcpyHostToDevice, s->stream));
+  SubstepBuffers B;
+  fill_buffers(s, B);
+  if (D == 3)
+    launch_append<3>(B, dx, dv, dm, (int)k, (int)s->n, s->stream);
+  else
+    launch_append<2>(B, dx, dv, dm, (int)k, (int)s->n, s->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  dfree(dx);
+  dfree(dv);
+  dfree(dm);
+  for (int64_t i = 0; i < k; i++) s->color.push_back(color ? color[i] : 0xFFFFFF);
+  s->n += k;
+  s->have_perm = false;
+  return MPM_OK;
+}
+
+int mpm_num_particles(mpm_solver* s, int64_t* n) {
+  if (!s || !n) return fail(MPM_EINVAL, "null argument");
+  *n = s->n;
+  return MPM_OK;
+}
+
+int mpm_substeps(mpm_solver* s, int32_t nsub, float dt, double t0) {
+  (void)t0;  // the built-in post-processes do not depend on t
+  if (!s) return fail(MPM_EINVAL, "null argument");
+  if (nsub < 0) return fail(MPM_EINVAL, "negative substep count");
+  HIP_TRY(hipSetDevice(s->device));
+  if (nsub > s->maxv_cap) {
+    dfree(s->maxv);
+    TRY(dalloc(&s->maxv, nsub));
+    s->maxv_cap = nsub;
+  }
+  launch_fill_u32(s->maxv, 0u, std::max(nsub, 1), s->stream);
+  s->last_batch = nsub;
+  if (s->n == 0) return MPM_OK;
+  fill_params(s, dt);
+  Timer* tm = s->timing ? &s->timer : nullptr;
+  for (int k = 0; k < nsub; k++) {
+    SubstepBuffers B;
+    fill_buffers(s, B);
+    if (s->have_perm && s->reorder_interval > 0 && s->since_reorder >= s->reorder_interval) {
+      if (s->D == 3)
+        launch_reorder<3>(B, s->pf_alt, s->pi_alt, s->stream, tm);
+      else
+        launch_reorder<2>(B, s->pf_alt, s->pi_alt, s->stream, tm);
+      std::swap(s->pf, s->pf_alt);
+      std::swap(s->pi, s->pi_alt);
+      s->since_reorder = 0;
+      fill_buffers(s, B);
+    }
+    if (s->D == 3)
+      launch_substep<3>(s->P, B, k, s->debug & 1, s->stream, tm);
+    else
+      launch_substep<2>(s->P, B, k, s->debug & 1, s->stream, tm);
+    take_buffers(s, B);
+    s->have_perm = true;
+    s->since_reorder++;
+  }
+  HIP_TRY(hipGetLastError());
+  return MPM_OK;
+}
+
+int mpm_read_max_velocity(mpm_solver* s, int32_t nsub, float* out) {
+  if (!s || (!out && nsub > 0)) return fail(MPM_EINVAL, "null argument");
+  if (nsub > s->last_batch) return fail(MPM_EINVAL, "more entries than substeps run");
+  int r = check_device_errors(s);
+  if (nsub > 0) HIP_TRY(hipMemcpy(out, s->maxv, sizeof(float) * nsub, hipMemcpyDeviceToHost));
+  return r;
+}
+
+int mpm_sync(mpm_solver* s) {
+  if (!s) return fail(MPM_EINVAL, "null argument");
+  return check_device_errors(s);
+}
+
+static int field_info(mpm_solver* s, int32_t field, int* comp0, int* ncomp, bool* is_int) {
+  const int D = s->D;
+  *is_int = false;
+  switch (field) {
+    case MPM_FIELD_X: *comp0 = 0; *ncomp = D; break;
+    case MPM_FIELD_V: *comp0 = D; *ncomp = D; break;
+    case MPM_FIELD_C: *comp0 = 2 * D; *ncomp = D * D; break;
+    case MPM_FIELD_F: *comp0 = 2 * D + D * D; *ncomp = D * D; break;
+    case MPM_FIELD_JP: *comp0 = 2 * D + 2 * D * D; *ncomp = 1; break;
+    case MPM_FIELD_MATERIAL: *comp0 = 0; *ncomp = 1; *is_int = true; break;
+    default: return fail(MPM_EINVAL, "unknown field " + std::to_string(field));
+  }
+  return MPM_OK;
+}
+
+int mpm_get_state(mpm_solver* s, int32_t field, void* host) {
+  if (!s || !host) return fail(MPM_EINVAL, "null argument");
+  if (field == MPM_FIELD_COLOR) {
+    std::memcpy(host, s->color.data(), sizeof(int32_t) * s->n);
+    return MPM_OK;
+  }
+  int c0, nc;
+  bool is_int;
+  TRY(field_info(s, field, &c0, &nc, &is_int));
+  HIP_TRY(hipSetDevice(s->device));
+  if (s->n == 0) return MPM_OK;
+  void* tmp = nullptr;
+  HIP_TRY(hipMalloc(&tmp, sizeof(float) * s->n * nc));
+  const int* orig = s->pi + s->cap;  // Layout ORIG == 1
+  if (is_int)
+    launch_gather_i(s->pi + (int64_t)c0 * s->cap, s->cap, nc, orig, (int)s->n, (int*)tmp,
+                    s->stream);
+  else
+    launch_gather_f(s->pf + (int64_t)c0 * s->cap, s->cap, nc, orig, (int)s->n, (float*)tmp,
+                    s->stream);
+  int r = check_device_errors(s);
+  hipError_t e = hipMemcpy(host, tmp, sizeof(float) * s->n * nc, hipMemcpyDeviceToHost);
+  hipFree(tmp);
+  if (e != hipSuccess) return fail(MPM_EHIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+  return r;
+}
+
+int mpm_set_state(mpm_solver* s, int32_t field, const void* host) {
+  if (!s || !host) return fail(MPM_EINVAL, "null argument");
+  if (field == MPM_FIELD_COLOR) {
+    std::memcpy(s->color.data(), host, sizeof(int32_t) * s->n);
+    return MPM_OK;
+  }
+  int c0, nc;
+  bool is_int;
+  TRY(field_info(s, field, &c0, &nc, &is_int));
+  if (is_int) {
+    const int32_t* m = (const int32_t*)host;
+    for (int64_t i = 0; i < s->n; i++)
+      if (m[i] < 0 || m[i] > MPM_STATIONARY) return fail(MPM_EINVAL, "unknown material id");
+  }
+  HIP_TRY(hipSetDevice(s->device));
+  if (s->n == 0) return MPM_OK;
+  void* tmp = nullptr;
+  HIP_TRY(hipMalloc(&tmp, sizeof(float) * s->n * nc));
+  HIP_TRY(hipMemcpyAsync(tmp, host, sizeof(float) * s->n * nc, hipMemcpyHostToDevice, s->stream));
+  const int* orig = s->pi + s->cap;
+  if (is_int)
+    launch_scatter_i((const int*)tmp, s->cap, nc, orig, (int)s->n,
+                     s->pi + (int64_t)c0 * s->cap, s->stream);
+  else
+    launch_scatter_f((const float*)tmp, s->cap, nc, orig, (int)s->n,
+                     s->pf + (int64_t)c0 * s->cap, s->stream);
+  int r = check_device_errors(s);
+  hipFree(tmp);
+  return r;
+}
+
+int mpm_particle_info(mpm_solver* s, float* x, float* v, int32_t* material, int32_t* color) {
+  if (!s) return fail(MPM_EINVAL, "null argument");
+  if (x) TRY(mpm_get_state(s, MPM_FIELD_X, x));
+  if (v) TRY(mpm_get_state(s, MPM_FIELD_V, v));
+  if (material) TRY(mpm_get_state(s, MPM_FIELD_MATERIAL, material));
+  if (color) TRY(mpm_get_state(s, MPM_FIELD_COLOR, color));
+  return MPM_OK;
+}
+
+int mpm_read_grid(mpm_solver* s, const int32_t* lo, const int32_t* nn, float* m, float* v) {
+  if (!s || !lo || !nn) return fail(MPM_EINVAL, "null argument");
+  TRY(check_device_errors(s));
+  const int D = s->D;
+  const int NPB = D == 3 ? 64 : 16;
+  const int64_t tb = s->total_blocks;
+  std::vector<float> g((size_t)tb * (D + 1) * NPB);
+  HIP_TRY(hipMemcpy(g.data(), s->grid, sizeof(float) * g.size(), hipMemcpyDeviceToHost));
+  // only blocks active in the last substep hold this substep's values
+  int nact = 0;
+  std::vector<int> tot(16);
+  HIP_TRY(hipMemcpy(tot.data(), s->totals, sizeof(int) * 16, hipMemcpyDeviceToHost));
+  nact = tot[kTotActive];
+  std::vector<int> act(std::max(nact, 1));
+  if (nact > 0)
+    HIP_TRY(hipMemcpy(act.data(), s->active, sizeof(int) * nact, hipMemcpyDeviceToHost));
+  std::vector<char> is_act(tb, 0);
+  for (int i = 0; i < nact; i++) is_act[act[i]] = 1;
+  const int n0 = nn[0], n1 = nn[1], n2 = D == 3 ? nn[2] : 1;
+  for (int k = 0; k < n2; k++)
+    for (int j = 0; j < n1; j++)
+      for (int i = 0; i < n0; i++) {
+        const int I[3] = {lo[0] + i, lo[1] + j, D == 3 ? lo[2] + k : 0};
+        const int64_t o = ((int64_t)k * n1 + j) * n0 + i;
+        float mv = 0, vv[3] = {0, 0, 0};
+        int lb[3], l[3];
+        bool ok = true;
+        for (int a = 0; a < D; a++) {
+          int b = I[a] >> 2;
+          l[a] = I[a] - 4 * b;
+          lb[a] = b - s->blo[a];
+          if (lb[a] < 0 || lb[a] >= s->bn[a]) ok = false;
+        }
+        if (ok) {
+          const int64_t bid = D == 3 ? ((int64_t)lb[2] * s->bn[1] + lb[1]) * s->bn[0] + lb[0]
+                                     : (int64_t)lb[1] * s->bn[0] + lb[0];
+          if (is_act[bid]) {
+            const int node = D == 3 ? l[0] + 4 * (l[1] + 4 * l[2]) : l[0] + 4 * l[1];
+            const float* gp = &g[(size_t)bid * (D + 1) * NPB + node];
+            for (int a = 0; a < D; a++) vv[a] = gp[a * NPB];
+            mv = gp[D * NPB];
+          }
+        }
+        if (m) m[o] = mv;
+        if (v)
+          for (int a = 0; a < D; a++) v[o * D + a] = vv[a];
+      }
+  return MPM_OK;
+}
+
+int mpm_read_binning(mpm_solver* s, int32_t* keys, int32_t* perm, int32_t* slot_to_insertion) {
+  if (!s) return fail(MPM_EINVAL, "null argument");
+  TRY(check_device_errors(s));
+  if (!s->have_perm) return fail(MPM_EINVAL, "no substep has run since the last change");
+  if (s->n == 0) return MPM_OK;
+  if (keys) HIP_TRY(hipMemcpy(keys, s->keys, sizeof(int) * s->n, hipMemcpyDeviceToHost));
+  if (perm) HIP_TRY(hipMemcpy(perm, s->perm, sizeof(int) * s->n, hipMemcpyDeviceToHost));
+  if (slot_to_insertion)
+    HIP_TRY(hipMemcpy(slot_to_insertion, s->pi + s->cap, sizeof(int) * s->n,
+                      hipMemcpyDeviceToHost));
+  return MPM_OK;
+}
+
+int mpm_set_debug(mpm_solver* s, int32_t flags) {
+  if (!s) return fail(MPM_EINVAL, "null argument");
+  s->debug = flags;
+  return MPM_OK;
+}
+
+int mpm_set_timing(mpm_solver* s, int32_t enabled) {
+  if (!s) return fail(MPM_EINVAL, "null argument");
+  TRY(check_device_errors(s));
+  s->timing = enabled != 0;
+  s->timer.acc.clear();
+  s->timer.order.clear();
+  return MPM_OK;
+}
+
+int mpm_kernel_times(mpm_solver* s, char* names, int32_t names_cap, int32_t* counts, double* ms,
+                     int32_t cap, int32_t* n_kernels) {
+  if (!s || !n_kernels) return fail(MPM_EINVAL, "null argument");
+  TRY(check_device_errors(s));
+  std::string all;
+  int k = 0;
+  for (auto& name : s->timer.order) {
+    if (k < cap) {
+      if (counts) counts[k] = s->timer.acc[name].first;
+      if (ms) ms[k] = s->timer.acc[name].second;
+    }
+    all += name;
+    all += "\n";
+    k++;
+  }
+  *n_kernels = k;
+  if (names && names_cap > 0) {
+    std::strncpy(names, all.c_str(), names_cap - 1);
+    names[names_cap - 1] = 0;
+  }
+  return MPM_OK;
+}
+
+int mpm_last_counts(mpm_solver* s, int64_t* out4) {
+  if (!s || !out4) return fail(MPM_EINVAL, "null argument");
+  TRY(check_device_errors(s));
+  int tot[16];
+  HIP_TRY(hipMemcpy(tot, s->totals, sizeof(int) * 16, hipMemcpyDeviceToHost));
+  out4[0] = tot[kTotValid];
+  out4[1] = tot[kTotChunks];
+  out4[2] = tot[kTotActive];
+  out4[3] = tot[kTotPBlocks];
+  return MPM_OK;
+}
+
+}  // extern "C"

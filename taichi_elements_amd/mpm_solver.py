@@ -1,0 +1,539 @@
+"""Drop-in ``MPMSolver`` for taichi_elements (engine/mpm_solver.py) on MI355X.
+
+The class keeps the reference's constructor, constants, attributes and
+methods (``add_cube``, ``add_ellipsoid``, ``add_particles``,
+``read_restart``, ``add_surface_collider``, ``add_sphere_collider``,
+``set_gravity``, ``step``, ``particle_info`` ...).  Host-side bookkeeping
+(constants, seeding, the frame loop of ``step``) is Python, as in the
+reference; every substep runs in hand-written HIP kernels behind the C ABI of
+``libmpm_hip.so`` (include/mpm_abi.h).  No Taichi is involved.
+
+Differences from the reference, all explicit:
+  * particles are seeded on the host with numpy's PCG64 (``seed`` argument)
+    because Taichi's ``ti.random`` stream cannot be reproduced;
+  * the grid is a dense window of 4^d blocks around the ``res`` domain (two
+    blocks of margin below, >= 4 cells above the walls) instead of Taichi's
+    4096^d sparse grid; particles that leave it raise ``DomainError``;
+  * ``unbounded``, ``quant``, ``use_g2p2g``, ``use_adaptive_dt`` and
+    ``support_plasticity=False`` raise ``NotImplementedError``, as do the
+    mesh/texture/polygon seeders and the particle file writers;
+  * ``grid_postprocess`` lists descriptions of the built-in post-processes
+    (bounding box and colliders); arbitrary Taichi callables cannot be added.
+"""
+import math
+import numbers
+import time
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+
+# Print one '.' per substep as the reference's step() does (mpm_solver.py:760).
+PRINT_PROGRESS = True
+
+
+class _Field0D:
+    """Mimics a Taichi 0-D field: ``obj[None]`` reads the value."""
+
+    def __init__(self, getter, setter=None):
+        self._get = getter
+        self._set = setter
+
+    def __getitem__(self, key):
+        if key is not None:
+            raise IndexError("0-D field: index with None")
+        return self._get()
+
+    def __setitem__(self, key, value):
+        if self._set is None:
+            raise TypeError("read-only field")
+        if key is not None:
+            raise IndexError("0-D field: index with None")
+        self._set(value)
+
+
+class MPMSolver:
+    material_water = 0
+    material_elastic = 1
+    material_snow = 2
+    material_sand = 3
+    material_stationary = 4
+    materials = {
+        'WATER': material_water,
+        'ELASTIC': material_elastic,
+        'SNOW': material_snow,
+        'SAND': material_sand,
+        'STATIONARY': material_stationary,
+    }
+
+    surface_sticky = 0
+    surface_slip = 1
+    surface_separate = 2
+    surfaces = {
+        'STICKY': surface_sticky,
+        'SLIP': surface_slip,
+        'SEPARATE': surface_separate
+    }
+
+    def __init__(self,
+                 res,
+                 quant=False,
+                 use_voxelizer=True,
+                 size=1,
+                 max_num_particles=2**30,
+                 padding=3,
+                 unbounded=False,
+                 dt_scale=1,
+                 E_scale=1,
+                 voxelizer_super_sample=2,
+                 use_g2p2g=False,
+                 v_clamp_g2p2g=True,
+                 use_bls=True,
+                 g2p2g_allowed_cfl=0.9,
+                 water_density=1.0,
+                 support_plasticity=True,
+                 use_adaptive_dt=False,
+                 use_ggui=False,
+                 use_emitter_id=False,
+                 *,
+                 device=0,
+                 seed=0,
+                 reorder_interval=0,
+                 chunk=0,
+                 kernel_profiler=False):
+        self.dim = len(res)
+        assert self.dim in (2, 3), "MPM solver supports only 2D and 3D simulations."
+        for flag, name in ((quant, "quant"), (use_g2p2g, "use_g2p2g"), (unbounded, "unbounded"),
+                           (use_adaptive_dt, "use_adaptive_dt")):
+            if flag:
+                raise NotImplementedError(f"{name}=True is not supported by this engine")
+        if not support_plasticity:
+            raise NotImplementedError("support_plasticity=False is not supported by this engine")
+        self.quant = quant
+        self.use_g2p2g = use_g2p2g
+        self.v_clamp_g2p2g = v_clamp_g2p2g
+        self.use_bls = use_bls
+        self.g2p2g_allowed_cfl = g2p2g_allowed_cfl
+        self.water_density = water_density
+        self.grid_size = 4096
+        self.use_voxelizer = use_voxelizer
+        self.voxelizer = None
+        self.voxelizer_super_sample = voxelizer_super_sample
+        self.use_ggui = use_ggui
+        self.support_plasticity = support_plasticity
+        self.use_adaptive_dt = use_adaptive_dt
+        self.use_emitter_id = use_emitter_id
+
+        # Host-side constants (mpm_solver.py:79-99, 198-210).
+        self.t = 0.0
+        self.res = res
+        self.size = size
+        self.dx = size / res[0]
+        self.inv_dx = 1.0 / self.dx
+        self.default_dt = 2e-2 * self.dx / size * dt_scale
+        self.p_vol = self.dx**self.dim
+        self.p_rho = 1000
+        self.p_mass = self.p_vol * self.p_rho
+        self.max_num_particles = max_num_particles
+        self.all_time_max_velocity = 0
+        self.F_bound = 4.0
+        self.padding = padding
+        self.E, self.nu = 1e6 * size * E_scale, 0.2
+        self.mu_0, self.lambda_0 = self.E / (2 * (1 + self.nu)), self.E * self.nu / (
+            (1 + self.nu) * (1 - 2 * self.nu))
+        friction_angle = math.radians(45)
+        sin_phi = math.sin(friction_angle)
+        self.alpha = math.sqrt(2 / 3) * 2 * sin_phi / (3 - sin_phi)
+        self.total_substeps = 0
+        self.unbounded = unbounded
+        self.offset = tuple(-self.grid_size // 2 for _ in range(self.dim))
+
+        self._rng = np.random.default_rng(seed)
+        self._device = device
+        self._reorder_interval = reorder_interval
+        self._chunk = chunk
+        self._kernel_profiler = kernel_profiler
+        self._emitter_ids = np.zeros(0, np.int32)
+        self._source_velocity = [0.0] * self.dim
+        self._gravity = [0.0] * self.dim
+        self._handle = None
+        self._create_handle()
+
+        self.n_particles = _Field0D(self._num_particles)
+        self.gravity = _Field0D(lambda: tuple(self._gravity), self.set_gravity)
+        self.grid_postprocess = []
+        if self.dim == 2:
+            self.set_gravity((0, -9.8))
+        else:
+            self.set_gravity((0, -9.8, 0))
+        self.add_bounding_box(self.unbounded)
+        self.writers = []
+
+    # ------------------------------------------------------------------ #
+    # handle management
+    # ------------------------------------------------------------------ #
+    def _create_handle(self):
+        L = _lib.lib()
+        cfg = _lib.MpmConfig()
+        cfg.dim = self.dim
+        for a in range(3):
+            cfg.res[a] = int(self.res[a]) if a < self.dim else 1
+        cfg.padding = int(self.padding)
+        cfg.device = int(self._device)
+        cfg.reorder_interval = int(self._reorder_interval)
+        cfg.chunk = int(self._chunk)
+        cfg.dx, cfg.inv_dx, cfg.p_vol, cfg.p_mass = self.dx, self.inv_dx, self.p_vol, self.p_mass
+        cfg.mu_0, cfg.lambda_0, cfg.alpha = self.mu_0, self.lambda_0, self.alpha
+        cfg.water_density = self.water_density
+        h = _lib._P()
+        check(L.mpm_create(cfg, h))
+        self._handle = h
+        if self._kernel_profiler:
+            check(L.mpm_set_timing(self._handle, 1))
+
+    def __del__(self):
+        h = getattr(self, "_handle", None)
+        if h is not None and _lib._lib is not None:
+            _lib._lib.mpm_destroy(h)
+            self._handle = None
+
+    def _num_particles(self):
+        n = _lib.ctypes.c_int64()
+        check(_lib.lib().mpm_num_particles(self._handle, n))
+        return int(n.value)
+
+    # ------------------------------------------------------------------ #
+    # configuration (mpm_solver.py:316-319, 618-692)
+    # ------------------------------------------------------------------ #
+    def set_gravity(self, g):
+        assert isinstance(g, (tuple, list))
+        assert len(g) == self.dim
+        self._gravity = [float(v) for v in g]
+        check(_lib.lib().mpm_set_gravity(self._handle, _lib.dvec(self._gravity)))
+
+    def add_bounding_box(self, unbounded):
+        if unbounded:
+            raise NotImplementedError("unbounded simulations are not supported")
+        check(_lib.lib().mpm_add_bounding_box(self._handle, 0))
+        self.grid_postprocess.append(("bounding_box", {}))
+
+    def add_sphere_collider(self, center, radius, surface=surface_sticky):
+        center = list(center)
+        check(_lib.lib().mpm_add_sphere_collider(self._handle, _lib.dvec(center), float(radius),
+                                                 int(surface)))
+        self.grid_postprocess.append(("sphere", dict(center=center, radius=radius,
+                                                     surface=surface)))
+
+    def add_surface_collider(self, point, normal, surface=surface_sticky, friction=0.0):
+        point = list(point)
+        normal_scale = 1.0 / math.sqrt(sum(x**2 for x in normal))
+        normal = list(normal_scale * x for x in normal)
+        if surface == self.surface_sticky and friction != 0:
+            raise ValueError('friction must be 0 on sticky surfaces.')
+        check(_lib.lib().mpm_add_surface_collider(self._handle, _lib.dvec(point),
+                                                  _lib.dvec(normal), int(surface),
+                                                  float(friction)))
+        self.grid_postprocess.append(("surface", dict(point=point, normal=normal,
+                                                      surface=surface, friction=friction)))
+
+    def clear_grid_postprocess(self):
+        check(_lib.lib().mpm_clear_grid_postprocess(self._handle))
+        self.grid_postprocess.clear()
+
+    # ------------------------------------------------------------------ #
+    # seeding (mpm_solver.py:823-1015, 1081-1144)
+    # ------------------------------------------------------------------ #
+    def set_source_velocity(self, velocity):
+        if velocity is not None:
+            velocity = list(velocity)
+            assert len(velocity) == self.dim
+            self._source_velocity = [float(v) for v in velocity]
+        else:
+            self._source_velocity = [0.0] * self.dim
+
+    def _append(self, x, v, material, color, emitter_id=0):
+        n = len(x)
+        if n == 0:
+            return
+        x = _lib.as_f32(x, (n, self.dim))
+        v = _lib.as_f32(v, (n, self.dim)) if v is not None else None
+        mat = _lib.as_i32(np.broadcast_to(np.asarray(material, np.int32), (n,)), (n,))
+        col = _lib.as_i32(np.broadcast_to(np.asarray(color, np.int64).astype(np.int32), (n,)),
+                          (n,))
+        check(_lib.lib().mpm_add_particles(self._handle, n, _lib.fptr(x), _lib.fptr(v),
+                                           _lib.iptr(mat), _lib.iptr(col)))
+        self._emitter_ids = np.concatenate(
+            [self._emitter_ids, np.full(n, emitter_id, np.int32)])
+
+    def _uniform_velocity(self, n):
+        return np.tile(np.asarray(self._source_velocity, np.float32), (n, 1))
+
+    def add_cube(self,
+                 lower_corner,
+                 cube_size,
+                 material,
+                 color=0xFFFFFF,
+                 sample_density=None,
+                 velocity=None):
+        if sample_density is None:
+            sample_density = 2**self.dim
+        vol = 1
+        for i in range(self.dim):
+            vol = vol * cube_size[i]
+        num_new_particles = int(sample_density * vol / self.dx**self.dim + 1)
+        assert self.n_particles[None] + num_new_particles <= self.max_num_particles
+        self.set_source_velocity(velocity=velocity)
+        lo = np.asarray(lower_corner[:self.dim], np.float32)
+        sz = np.asarray(cube_size[:self.dim], np.float32)
+        # x_k = lower_k + random() * size_k in f32 (mpm_solver.py:846-848)
+        x = lo + self._rng.random((num_new_particles, self.dim), dtype=np.float32) * sz
+        self._append(x, self._uniform_velocity(num_new_particles), material, color)
+
+    def _random_point_in_unit_sphere(self, n):
+        out = np.empty((n, self.dim), np.float32)
+        got = 0
+        while got < n:
+            m = max(64, int((n - got) * 2.2))
+            cand = self._rng.random((m, self.dim), dtype=np.float32) * 2 - 1
+            cand = cand[(cand * cand).sum(1) <= 1]  # mpm_solver.py:962-966
+            k = min(len(cand), n - got)
+            out[got:got + k] = cand[:k]
+            got += k
+        return out
+
+    def add_ellipsoid(self,
+                      center,
+                      radius,
+                      material,
+                      color=0xFFFFFF,
+                      sample_density=None,
+                      velocity=None):
+        if sample_density is None:
+            sample_density = 2**self.dim
+        if isinstance(radius, numbers.Number):
+            radius = [radius] * self.dim
+        radius = list(radius)
+        if self.dim == 2:
+            num_particles = math.pi
+        else:
+            num_particles = 4 / 3 * math.pi
+        for i in range(self.dim):
+            num_particles *= radius[i] * self.inv_dx
+        num_particles = int(math.ceil(num_particles * sample_density))
+        self.set_source_velocity(velocity=velocity)
+        assert self.n_particles[None] + num_particles <= self.max_num_particles
+        c = np.asarray(center[:self.dim], np.float32)
+        r = np.asarray(radius[:self.dim], np.float32)
+        x = c + self._random_point_in_unit_sphere(num_particles) * r
+        self._append(x, self._uniform_velocity(num_particles), material, color)
+
+    def add_particles(self, particles, material, color=0xFFFFFF, velocity=None):
+        self.set_source_velocity(velocity=velocity)
+        particles = np.asarray(particles, np.float32).reshape(-1, self.dim)
+        n = len(particles)
+        assert self.n_particles[None] + n <= self.max_num_particles
+        self._append(particles, self._uniform_velocity(n), material, color)
+
+    def read_restart(self, num_particles, pos, vel, material, color):
+        n = int(num_particles)
+        assert self.n_particles[None] + n <= self.max_num_particles
+        self._append(np.asarray(pos, np.float32)[:n], np.asarray(vel, np.float32)[:n],
+                     np.asarray(material, np.int32)[:n], np.asarray(color)[:n])
+
+    def clear_particles(self):
+        """mpm_solver.py:1193-1196: drop every particle (keeps the settings)."""
+        post = list(self.grid_postprocess)
+        _lib.lib().mpm_destroy(self._handle)
+        self._handle = None
+        self._create_handle()
+        check(_lib.lib().mpm_set_gravity(self._handle, _lib.dvec(self._gravity)))
+        self.grid_postprocess = []
+        for kind, kw in post:
+            if kind == "bounding_box":
+                self.add_bounding_box(False)
+            elif kind == "sphere":
+                self.add_sphere_collider(**kw)
+            else:
+                self.add_surface_collider(**kw)
+        self._emitter_ids = np.zeros(0, np.int32)
+
+    def add_mesh(self, *args, **kwargs):
+        raise NotImplementedError("add_mesh (voxelizer seeding) is not part of this engine")
+
+    def add_ngon(self, *args, **kwargs):
+        raise NotImplementedError("add_ngon is not part of this engine")
+
+    def add_texture_2d(self, *args, **kwargs):
+        raise NotImplementedError("add_texture_2d is not part of this engine")
+
+    def write_particles(self, *args, **kwargs):
+        raise NotImplementedError("particle file writers are not part of this engine")
+
+    def write_particles_ply(self, *args, **kwargs):
+        raise NotImplementedError("particle file writers are not part of this engine")
+
+    # ------------------------------------------------------------------ #
+    # time stepping (mpm_solver.py:748-821)
+    # ------------------------------------------------------------------ #
+    def step(self, frame_dt, print_stat=False, smry_writer=None):
+        begin_t = time.time()
+        begin_substep = self.total_substeps
+
+        substeps = int(frame_dt / self.default_dt) + 1
+        dt = frame_dt / substeps
+        frame_time_left = frame_dt
+        if print_stat:
+            print(f'needed substeps: {substeps}')
+        count = 0
+        while frame_time_left > 0:  # float64 loop; may run substeps + 1 times
+            count += 1
+            frame_time_left -= dt
+        if PRINT_PROGRESS:
+            print('.' * count, flush=True)
+        maxv = self._run_substeps(count, dt)
+        for k in range(count):
+            self.total_substeps += 1
+            self.t += dt
+            cur_frame_velocity = float(maxv[k])
+            if smry_writer is not None:
+                smry_writer.add_scalar("substep_max_CFL", cur_frame_velocity * dt / self.dx,
+                                       self.total_substeps)
+            self.all_time_max_velocity = max(self.all_time_max_velocity, cur_frame_velocity)
+
+        if print_stat:
+            self.print_kernel_profiler_info()
+            cur_frame_velocity = float(maxv[-1]) if count else 0.0
+            print(f'CFL: {cur_frame_velocity * dt / self.dx}')
+            print(f'num particles={self.n_particles[None]}')
+            print(f'  frame time {time.time() - begin_t:.3f} s')
+            print(f'  substep time {1000 * (time.time() - begin_t) / max(1, self.total_substeps - begin_substep):.3f} ms')
+
+    def _run_substeps(self, count, dt):
+        L = _lib.lib()
+        check(L.mpm_substeps(self._handle, int(count), float(dt), float(self.t)))
+        out = np.zeros(max(count, 1), np.float32)
+        check(L.mpm_read_max_velocity(self._handle, int(count), _lib.fptr(out)))
+        return out[:count]
+
+    def substep(self, dt):
+        """One substep with exactly ``dt`` (the body of step()'s loop).
+        Returns that substep's max particle velocity component."""
+        maxv = self._run_substeps(1, dt)
+        self.total_substeps += 1
+        self.t += dt
+        self.all_time_max_velocity = max(self.all_time_max_velocity, float(maxv[0]))
+        return float(maxv[0])
+
+    def compute_max_velocity(self):
+        v = self.get_state()["v"]
+        return float(np.abs(v).max()) if len(v) else 0.0
+
+    def print_kernel_profiler_info(self):
+        names, counts, ms = self.kernel_times()
+        print(f"{'kernel':<16}{'calls':>8}{'total ms':>12}{'avg us':>12}")
+        for nm, c, t in zip(names, counts, ms):
+            print(f"{nm:<16}{c:>8}{t:>12.3f}{1000 * t / max(c, 1):>12.2f}")
+
+    def kernel_times(self):
+        L = _lib.lib()
+        cap = 64
+        names = _lib.ctypes.create_string_buffer(4096)
+        counts = (_lib.ctypes.c_int32 * cap)()
+        ms = (_lib.ctypes.c_double * cap)()
+        nk = _lib.ctypes.c_int32()
+        check(L.mpm_kernel_times(self._handle, names, 4096, counts, ms, cap, nk))
+        nm = names.value.decode().split("\n")[:nk.value]
+        return nm, list(counts[:nk.value]), list(ms[:nk.value])
+
+    # ------------------------------------------------------------------ #
+    # readout (mpm_solver.py:1172-1191) and state access
+    # ------------------------------------------------------------------ #
+    def particle_info(self):
+        n = self.n_particles[None]
+        np_x = np.empty((n, self.dim), dtype=np.float32)
+        np_v = np.empty((n, self.dim), dtype=np.float32)
+        np_material = np.empty((n, ), dtype=np.int32)
+        np_color = np.empty((n, ), dtype=np.int32)
+        if n:
+            check(_lib.lib().mpm_particle_info(self._handle, _lib.fptr(np_x), _lib.fptr(np_v),
+                                               _lib.iptr(np_material), _lib.iptr(np_color)))
+        particles_data = {
+            'position': np_x,
+            'velocity': np_v,
+            'material': np_material,
+            'color': np_color
+        }
+        if self.use_emitter_id:
+            particles_data['emitter_ids'] = self._emitter_ids.copy()
+        return particles_data
+
+    _FIELDS = {"x": (_lib.FIELD_X, 1), "v": (_lib.FIELD_V, 1), "C": (_lib.FIELD_C, 2),
+               "F": (_lib.FIELD_F, 2), "Jp": (_lib.FIELD_JP, 0),
+               "material": (_lib.FIELD_MATERIAL, 0), "color": (_lib.FIELD_COLOR, 0)}
+
+    def _shape(self, rank, n):
+        return (n, ) + (self.dim, ) * rank
+
+    def get_state(self):
+        """Full particle state in insertion order: x, v, C, F, Jp, material, color."""
+        n = self.n_particles[None]
+        out = {}
+        for name, (fid, rank) in self._FIELDS.items():
+            dt = np.int32 if name in ("material", "color") else np.float32
+            a = np.empty(self._shape(rank, n), dt)
+            if n:
+                check(_lib.lib().mpm_get_state(self._handle, fid, a.ctypes.data))
+            out[name] = a
+        return out
+
+    def set_state(self, **fields):
+        """Overwrite fields (x, v, C, F, Jp, material, color) in insertion order."""
+        n = self.n_particles[None]
+        for name, value in fields.items():
+            fid, rank = self._FIELDS[name]
+            dt = np.int32 if name in ("material", "color") else np.float32
+            a = np.ascontiguousarray(np.asarray(value, dt).reshape(self._shape(rank, n)))
+            if n:
+                check(_lib.lib().mpm_set_state(self._handle, fid, a.ctypes.data))
+
+    def read_grid(self, lo=None, n=None):
+        """Grid mass and velocity after the last grid operator over the node
+        window [lo, lo + n) (default: the whole domain); arrays are indexed
+        [z, y, x] (3D) or [y, x] (2D)."""
+        if lo is None:
+            lo = [0] * self.dim
+        if n is None:
+            n = [self.res[a] for a in range(self.dim)]
+        lo3 = (_lib.ctypes.c_int32 * 3)(*(list(lo) + [0] * (3 - self.dim)))
+        n3 = (_lib.ctypes.c_int32 * 3)(*(list(n) + [1] * (3 - self.dim)))
+        shape = tuple(int(v) for v in reversed(list(n)))
+        m = np.zeros(shape, np.float32)
+        v = np.zeros(shape + (self.dim, ), np.float32)
+        check(_lib.lib().mpm_read_grid(self._handle, lo3, n3, _lib.fptr(m), _lib.fptr(v)))
+        return m, v
+
+    def read_binning(self):
+        """(keys, perm, slot_to_insertion) of the last substep's binning."""
+        n = self.n_particles[None]
+        keys = np.zeros(n, np.int32)
+        perm = np.zeros(n, np.int32)
+        orig = np.zeros(n, np.int32)
+        check(_lib.lib().mpm_read_binning(self._handle, _lib.iptr(keys), _lib.iptr(perm),
+                                          _lib.iptr(orig)))
+        return keys, perm, orig
+
+    def set_debug(self, flags):
+        check(_lib.lib().mpm_set_debug(self._handle, int(flags)))
+
+    def last_counts(self):
+        out = np.zeros(4, np.int64)
+        check(_lib.lib().mpm_last_counts(self._handle, out.ctypes.data_as(_lib._I64P)))
+        return dict(valid=int(out[0]), chunks=int(out[1]), active_blocks=int(out[2]),
+                    particle_blocks=int(out[3]))
+
+    def block_window(self):
+        """(lo, n) of the dense block window in blocks of 4 nodes per axis."""
+        lo = [-2] * self.dim
+        n = [(int(self.res[a]) + 3) // 4 + 5 for a in range(self.dim)]
+        return lo, n

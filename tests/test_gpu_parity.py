@@ -1,0 +1,288 @@
+"""GPU parity of the HIP substep against the oracle (CPU restatement).
+
+Every test drives the HIP path through the C ABI (ctypes, via the drop-in
+MPMSolver) on seeded inputs and compares with the oracle on the same inputs.
+Tolerances (float32 arithmetic, different summation order on the grid):
+  * one substep from a pre-deformed state: norm-wise relative error
+    <= 1e-5 per field (x, v, C, F, Jp, grid mass and velocity);
+  * binning keys and permutation: bit-exact;
+  * 100 substeps: relative drift <= 1e-4 (stated bound; measured values are
+    printed).
+"""
+import numpy as np
+import pytest
+
+from tests.helpers import cube_particles, deformed_state, oracle_cfg_for, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+DRIFT_TOL = 1e-4
+
+
+@pytest.fixture(autouse=True)
+def _quiet():
+    from taichi_elements_amd import mpm_solver
+    mpm_solver.PRINT_PROGRESS = False
+
+
+def _solver(dim, res, **kw):
+    from taichi_elements_amd import MPMSolver
+    return MPMSolver(res=(res, ) * dim, **kw)
+
+
+def _load(s, st):
+    n = len(st["x"])
+    s.read_restart(n, st["x"], st["v"], st["material"], np.zeros(n, np.int32))
+    s.set_state(F=st["F"], C=st["C"], Jp=st["Jp"])
+
+
+def _compare(got, ref, fields=("x", "v", "C", "F", "Jp"), tol=TOL, label=""):
+    errs = {k: rel_err(got[k], ref[k]) for k in fields}
+    print(label, {k: f"{e:.2e}" for k, e in errs.items()}, flush=True)
+    for k, e in errs.items():
+        assert e <= tol, (label, k, e)
+    return errs
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_one_substep_from_deformed_state(oracle, dim):
+    rng = np.random.default_rng(100 + dim)
+    res = 32
+    n = 6000 if dim == 3 else 3000
+    s = _solver(dim, res)
+    x = cube_particles(rng, n, [0.15] * dim, [0.7] * dim, dim)
+    mat = rng.integers(0, 5, n).astype(np.int32)  # all five materials
+    st = deformed_state(rng, x, mat, dim)
+    _load(s, st)
+    dt = s.default_dt
+    s.substep(dt)
+    got = s.get_state()
+    cfg = oracle_cfg_for(oracle, s)
+    ref, gm, gv, mv = oracle.substep(cfg, st, dt, 0.0, np.float32)
+    _compare(got, ref, label=f"substep {dim}D")
+    # grid after the grid operator over the oracle's window
+    m, v = s.read_grid([-8] * dim, [res + 16] * dim)
+    assert rel_err(m, gm) <= TOL
+    assert rel_err(v, gv) <= TOL
+    # max velocity of the substep (compute_max_velocity)
+    assert abs(s.all_time_max_velocity - mv) <= 1e-6 * max(1.0, mv)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_p2g_grid_mass_and_momentum(oracle, dim):
+    """Raw p2g output (debug mode: grid operator stores mass and momentum)."""
+    rng = np.random.default_rng(200 + dim)
+    res = 32
+    n = 5000
+    s = _solver(dim, res, water_density=1.3)
+    x = cube_particles(rng, n, [0.2] * dim, [0.6] * dim, dim)
+    mat = rng.integers(0, 4, n).astype(np.int32)
+    st = deformed_state(rng, x, mat, dim)
+    _load(s, st)
+    s.set_debug(1)
+    dt = s.default_dt
+    s.substep(dt)
+    m, p = s.read_grid([-8] * dim, [res + 16] * dim)
+    cfg = oracle_cfg_for(oracle, s)
+    gm, gp, new = oracle.p2g(cfg, st, dt, np.float32)
+    assert rel_err(m, gm) <= TOL
+    assert rel_err(p, gp) <= TOL
+    got = s.get_state()
+    _compare(got, new, fields=("F", "Jp"), label=f"p2g {dim}D")
+    mass = np.where(mat == 0, s.p_mass * 1.3, s.p_mass)
+    np.testing.assert_allclose(m.sum(dtype=np.float64), mass.sum(), rtol=1e-5)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_binning_keys_and_permutation_bit_exact(oracle, dim):
+    rng = np.random.default_rng(300 + dim)
+    res = 48
+    n = 20000
+    s = _solver(dim, res)
+    x = cube_particles(rng, n, [0.05] * dim, [0.9] * dim, dim)
+    # particles exactly on cell faces stress the floor()
+    x[:200] = (np.floor(x[:200] * res) + 0.5) / np.float32(res)
+    s.read_restart(n, x, np.zeros_like(x), np.ones(n, np.int32), np.zeros(n, np.int32))
+    s.substep(1e-6)
+    keys, perm, orig = s.read_binning()
+    np.testing.assert_array_equal(orig, np.arange(n))
+    blo, bn = s.block_window()
+    ref_keys = oracle.block_keys(x, np.float32(s.inv_dx), 4, np.array(blo), np.array(bn))
+    np.testing.assert_array_equal(keys, ref_keys)
+    np.testing.assert_array_equal(perm, oracle.sort_by_key(ref_keys, int(np.prod(bn))))
+
+
+def test_binning_after_reorder_is_sorted(oracle):
+    """After physical reorders the slots move; keys and perm still describe a
+    stable sort of the current slots."""
+    rng = np.random.default_rng(310)
+    s = _solver(3, 32, reorder_interval=1)
+    n = 8000
+    x = cube_particles(rng, n, [0.2] * 3, [0.5] * 3, 3)
+    v = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
+    s.read_restart(n, x, v, np.ones(n, np.int32), np.zeros(n, np.int32))
+    for _ in range(3):
+        s.substep(s.default_dt)
+    keys, perm, orig = s.read_binning()
+    assert sorted(orig.tolist()) == list(range(n))
+    assert np.all(np.diff(orig) != 0)
+    np.testing.assert_array_equal(perm, np.argsort(keys, kind="stable"))
+
+
+def test_reorder_does_not_change_results():
+    rng = np.random.default_rng(320)
+    n = 20000
+    x = cube_particles(rng, n, [0.3] * 3, [0.3] * 3, 3)
+    mat = rng.integers(0, 4, n).astype(np.int32)
+    st = deformed_state(rng, x, mat, 3)
+    outs = []
+    for interval in (1, -1):
+        s = _solver(3, 32, reorder_interval=interval)
+        _load(s, st)
+        for _ in range(10):
+            s.substep(s.default_dt)
+        outs.append(s.get_state())
+    _compare(outs[0], outs[1], tol=TOL, label="reorder")
+
+
+def _drift(oracle, s, st, nsteps, dt, label):
+    cfg = oracle_cfg_for(oracle, s)
+    run32 = oracle.Substepper(cfg, st, np.float32)
+    st64 = {k: (np.asarray(v, np.float64) if k != "material" else v) for k, v in st.items()}
+    run64 = oracle.Substepper(cfg, st64, np.float64)
+    for k in range(nsteps):
+        s.substep(dt)
+        run32.substep(dt)
+        run64.substep(dt)
+    got = s.get_state()
+    e32 = {k: rel_err(got[k], run32.s[k]) for k in ("x", "v", "C", "F", "Jp")}
+    e64 = {k: rel_err(got[k], run64.s[k]) for k in ("x", "v", "C", "F", "Jp")}
+    print(label, "vs f32 oracle", {k: f"{e:.2e}" for k, e in e32.items()}, flush=True)
+    print(label, "vs f64 oracle", {k: f"{e:.2e}" for k, e in e64.items()}, flush=True)
+    for k in ("x", "v"):
+        assert e32[k] <= DRIFT_TOL, (k, e32[k])
+    return e32, e64
+
+
+def test_2d_water_100_substeps(oracle):
+    """2D water, 128^2, 20,000 particles in [0.2,0.5]x[0.4,0.8] from rest,
+    gravity -9.8, walls at 3 cells, dt = default_dt, 100 substeps."""
+    rng = np.random.default_rng(0)
+    n = 20000
+    s = _solver(2, 128)
+    x = cube_particles(rng, n, [0.2, 0.4], [0.3, 0.4], 2)
+    s.add_particles(x, s.material_water)
+    st = oracle.rest_state(x, np.zeros_like(x), np.zeros(n, np.int32))
+    _drift(oracle, s, st, 100, s.default_dt, "2D water x100")
+
+
+def test_3d_four_materials_100_substeps(oracle):
+    """3D, 32^3, four cubes (water, elastic, snow, sand) with random velocities."""
+    rng = np.random.default_rng(1)
+    s = _solver(3, 32)
+    xs, vs, ms = [], [], []
+    for mat in range(4):
+        lo = rng.uniform(0.15, 0.6, 3)
+        x = cube_particles(rng, 3000, lo, [0.25] * 3, 3)
+        xs.append(x)
+        vs.append(rng.uniform(-1, 1, (3000, 3)).astype(np.float32))
+        ms.append(np.full(3000, mat, np.int32))
+    x, v, m = np.concatenate(xs), np.concatenate(vs), np.concatenate(ms)
+    s.read_restart(len(x), x, v, m, np.zeros(len(x), np.int32))
+    st = oracle.rest_state(x, v, m)
+    _drift(oracle, s, st, 100, s.default_dt, "3D 4-material x100")
+
+
+@pytest.mark.parametrize("surface,friction", [(0, 0.0), (1, 0.5), (2, 0.3)])
+def test_colliders(oracle, surface, friction):
+    rng = np.random.default_rng(400 + surface)
+    s = _solver(3, 32)
+    s.set_gravity((0, -30, 0))
+    s.add_surface_collider((0.5, 0.3, 0.5), (0.2, 1.0, 0.1), surface=surface, friction=friction)
+    s.add_sphere_collider((0.5, 0.45, 0.5), 0.1, surface=(surface + 1) % 3)
+    n = 6000
+    x = cube_particles(rng, n, [0.3, 0.3, 0.3], [0.4, 0.3, 0.4], 3)
+    v = np.tile(np.array([0.0, -3.0, 0.5], np.float32), (n, 1))
+    s.read_restart(n, x, v, rng.integers(0, 4, n), np.zeros(n, np.int32))
+    st = oracle.rest_state(x, v, s.get_state()["material"])
+    cfg = oracle_cfg_for(oracle, s)
+    run = oracle.Substepper(cfg, st, np.float32)
+    for _ in range(20):
+        s.substep(s.default_dt)
+        run.substep(s.default_dt)
+    _compare(s.get_state(), run.s, tol=1e-4, label=f"colliders s={surface}")
+
+
+def test_step_runs_the_reference_substep_count():
+    s = _solver(3, 24, size=10)
+    s.add_ellipsoid(center=[2, 4, 3], radius=1, material=s.material_snow, velocity=[0, -10, 0])
+    s.add_cube(lower_corner=[2, 6, 3], cube_size=[1, 1, 3], material=s.material_elastic)
+    s.set_gravity((0, -50, 0))
+    frame_dt = 4e-3
+    substeps = int(frame_dt / s.default_dt) + 1
+    dt = frame_dt / substeps
+    left, count = frame_dt, 0
+    while left > 0:
+        count += 1
+        left -= dt
+    for frame in range(3):
+        s.step(frame_dt)
+    assert s.total_substeps == 3 * count
+    info = s.particle_info()
+    n = s.n_particles[None]
+    assert info["position"].shape == (n, 3) and info["color"].shape == (n, )
+    assert np.all(info["color"] == 0xFFFFFF)
+    assert np.all(np.isfinite(info["position"]))
+    assert s.all_time_max_velocity > 0
+
+
+def test_insertion_order_survives_reorders():
+    rng = np.random.default_rng(500)
+    s = _solver(3, 32, reorder_interval=1)
+    xa = cube_particles(rng, 3000, [0.2] * 3, [0.3] * 3, 3)
+    s.add_particles(xa, s.material_elastic, color=0x123456, velocity=[0.5, 0, 0])
+    for _ in range(5):
+        s.substep(s.default_dt)
+    xb = cube_particles(rng, 2000, [0.6] * 3, [0.2] * 3, 3)
+    s.add_particles(xb, s.material_sand, color=0x654321)
+    for _ in range(5):
+        s.substep(s.default_dt)
+    info = s.particle_info()
+    assert np.all(info["material"][:3000] == 1) and np.all(info["material"][3000:] == 3)
+    assert np.all(info["color"][:3000] == 0x123456) and np.all(info["color"][3000:] == 0x654321)
+    # the second cube was added at rest: it moved only under gravity and contact
+    assert np.abs(info["position"][3000:] - xb).max() < 0.05
+
+
+def test_particles_leaving_the_window_raise():
+    from taichi_elements_amd._lib import DomainError
+    s = _solver(3, 32)
+    x = np.array([[0.5, 0.5, 0.5], [5.0, 0.5, 0.5]], np.float32)
+    s.add_particles(x, s.material_elastic)
+    with pytest.raises(DomainError):
+        s.substep(s.default_dt)
+
+
+def test_full_size_mass_and_momentum_conservation():
+    """1M particles on 128^3 (the 3D four-material workload): raw p2g mass
+    equals N * p_mass and momentum equals sum m v."""
+    rng = np.random.default_rng(1)
+    s = _solver(3, 128)
+    xs, vs, ms = [], [], []
+    for mat in range(4):
+        lo = rng.uniform(0.03, 0.77, 3)
+        xs.append(cube_particles(rng, 250000, lo, [0.2] * 3, 3))
+        vs.append(rng.uniform(-1, 1, (250000, 3)).astype(np.float32))
+        ms.append(np.full(250000, mat, np.int32))
+    x, v, m = np.concatenate(xs), np.concatenate(vs), np.concatenate(ms)
+    s.read_restart(len(x), x, v, m, np.zeros(len(x), np.int32))
+    s.set_debug(1)
+    s.substep(s.default_dt)
+    gm, gp = s.read_grid([-8] * 3, [144] * 3)
+    np.testing.assert_allclose(gm.sum(dtype=np.float64), len(x) * s.p_mass, rtol=1e-5)
+    mom = (s.p_mass * v.astype(np.float64)).sum(0)
+    np.testing.assert_allclose(gp.reshape(-1, 3).sum(0, dtype=np.float64), mom,
+                               rtol=1e-4, atol=1e-4 * np.abs(mom).max())
+    c = s.last_counts()
+    assert c["valid"] == len(x)
