@@ -1,0 +1,182 @@
+"""Benchmark of the MLS-MPM substep on one MI355X.
+
+Workload (the 3D four-material plain configuration): dense 128^3 grid,
+dx = 1/128, 1,000,000 float32 particles in four seeded uniform-random cubes
+of side 0.2 (water, fixed-corotated elastic, snow, Drucker-Prager sand),
+initial velocities uniform in [-1, 1]^3, E = 1e6, nu = 0.2, gravity -9.8
+along y, dt = default_dt = 1.5625e-4.  A "step" is one substep (binning,
+p2g, grid operator, g2p) over all particles; inputs are resident in HBM
+before the timed region starts.
+
+Prints one JSON line:
+  value      particle-substeps/s over the K timed substeps (wall clock,
+             device synchronised on both sides)
+  roofline   the dominant kernel (p2g): algorithmic bytes per launch
+             (148 B per particle + 16 B per active grid node) divided by its
+             mean duration from HIP events on the solver's stream, against
+             MI355X's 8 TB/s; traffic = measured HBM bytes per launch from
+             the committed rocprofv3 counter summary when present
+  cpu_baseline  the oracle (C restatement, OpenMP) on the host cores over a
+             bounded sample of the same workload
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+N_PER_CUBE = 250_000
+RES = 128
+
+
+def make_scene(seed=1):
+    """Four cubes of side 0.2 with lower corners U[0.03, 0.77]^3, materials
+    water / elastic / snow / sand, 250k particles each, v ~ U[-1, 1]^3."""
+    rng = np.random.default_rng(seed)
+    xs, vs, ms = [], [], []
+    for mat in range(4):
+        lo = rng.uniform(0.03, 0.77, 3).astype(np.float32)
+        xs.append(lo + rng.random((N_PER_CUBE, 3), dtype=np.float32) * np.float32(0.2))
+        vs.append(rng.uniform(-1, 1, (N_PER_CUBE, 3)).astype(np.float32))
+        ms.append(np.full(N_PER_CUBE, mat, np.int32))
+    return np.concatenate(xs), np.concatenate(vs), np.concatenate(ms)
+
+
+def algorithmic_bytes(n_particles, n_nodes):
+    """Bytes a substep must move in the reference's field layout (float32):
+    per particle 16 (binning) + 148 (p2g) + 80 (g2p), per active grid node
+    (4 channels) 16 (p2g write) + 28 (grid op) + 12 (g2p read)."""
+    return dict(substep=244 * n_particles + 56 * n_nodes,
+                p2g=148 * n_particles + 16 * n_nodes)
+
+
+def cpu_baseline(x, v, m, dt, seconds):
+    from oracle import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cfg = O.make_cfg((RES, RES, RES), grid_lo=[-8] * 3, grid_n=[RES + 16] * 3)
+    st = O.rest_state(x, v, m)
+    run = O.Substepper(cfg, st, np.float32, nthreads=threads)
+    run.substep(dt)  # first touch of the grid buffers
+    t0 = time.perf_counter()
+    k = 0
+    while True:
+        run.substep(dt)
+        k += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or k >= 100:
+            break
+    return dict(value=len(x) * k / el, unit="particle-substeps/s", cores=threads, kind="port",
+                sample=f"{k} substeps of the full 1M-particle 128^3 workload "
+                       f"(oracle C restatement, OpenMP atomics, float32), {el:.1f} s")
+
+
+def measured_traffic():
+    """HBM bytes per p2g launch from the committed rocprofv3 counter summary."""
+    path = os.path.join(ROOT, "profiles", "p2g_3d_1M_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("p2g_hbm_bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--reorder-interval", type=int, default=0)
+    ap.add_argument("--chunk", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the substep engine "
+              "is single-GPU and its sharding is a design note, not a measured path.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from taichi_elements_amd import MPMSolver, mpm_solver
+    mpm_solver.PRINT_PROGRESS = False
+    x, v, m = make_scene()
+    n = len(x)
+    s = MPMSolver(res=(RES, RES, RES), reorder_interval=args.reorder_interval, chunk=args.chunk)
+    s.read_restart(n, x, v, m, np.zeros(n, np.int32))
+    dt = s.default_dt
+    L = __import__("taichi_elements_amd._lib", fromlist=["lib"]).lib()
+
+    def run(k):
+        from taichi_elements_amd._lib import check
+        check(L.mpm_substeps(s._handle, k, dt, 0.0))
+        check(L.mpm_sync(s._handle))
+
+    print(f"[bench] scene ready: {n} particles; warmup {args.warmup}", file=sys.stderr, flush=True)
+    run(args.warmup)
+    t0 = time.perf_counter()
+    run(args.steps)
+    t1 = time.perf_counter()
+    ms_per_step = 1000 * (t1 - t0) / args.steps
+    value = n * args.steps / (t1 - t0)
+    counts = s.last_counts()
+    n_nodes = 64 * counts["active_blocks"]
+    print(f"[bench] timed {args.steps} substeps: {ms_per_step:.4f} ms/substep", file=sys.stderr,
+          flush=True)
+
+    # per-kernel durations with HIP events on the solver's stream
+    from taichi_elements_amd._lib import check
+    check(L.mpm_set_timing(s._handle, 1))
+    t2 = time.perf_counter()
+    run(args.steps)
+    t3 = time.perf_counter()
+    names, calls, ms = s.kernel_times()
+    check(L.mpm_set_timing(s._handle, 0))
+    kern = {nm: dict(calls=c, avg_us=1000 * t / max(c, 1)) for nm, c, t in zip(names, calls, ms)}
+    ab = algorithmic_bytes(counts["valid"], n_nodes)
+    p2g_s = kern["k_p2g"]["avg_us"] * 1e-6
+    achieved = ab["p2g"] / p2g_s / 1e9
+    substep_gbs = ab["substep"] / (ms_per_step * 1e-3) / 1e9
+
+    out = {
+        "metric": "particle-substeps/s (3D 1M, 4-material): plain/batched/fwd+bwd; % HBM peak",
+        "value": value,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded numpy: 4 cubes x 250k particles, v ~ U[-1,1]^3)",
+        "config": {"workload": "3D four-material plain substep, 128^3 dense grid, dx=1/128, "
+                               "1M particles (water/elastic/snow/sand cubes of side 0.2), "
+                               "dt=default_dt", "res": RES, "particles": n,
+                   "active_grid_blocks": counts["active_blocks"],
+                   "particle_blocks": counts["particle_blocks"], "chunks": counts["chunks"]},
+        "roofline": {"bound": "hbm", "kernel": "k_p2g", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(),
+                     "algorithmic_bytes_per_launch": ab["p2g"]},
+        "substep_hbm": {"algorithmic_bytes": ab["substep"], "achieved_GBs": substep_gbs,
+                        "frac": substep_gbs / HBM_PEAK_GBS},
+        "kernels_us": {k: round(v_["avg_us"], 2) for k, v_ in kern.items()},
+        "timed_with_events_ms_per_step": 1000 * (t3 - t2) / args.steps,
+        "batched": None,
+        "fwd_bwd": None,
+    }
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(x, v, m, dt, args.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -4,7 +4,12 @@ Every test drives the HIP path through the C ABI (ctypes, via the drop-in
 MPMSolver) on seeded inputs and compares with the oracle on the same inputs.
 Tolerances (float32 arithmetic, different summation order on the grid):
   * one substep from a pre-deformed state: norm-wise relative error
-    <= 1e-5 per field (x, v, C, F, Jp, grid mass and velocity);
+    <= 1e-5 per field for x, v, F, Jp, grid mass and grid velocity, and
+    <= 3e-5 for the affine velocity C.  C is a grid-velocity gradient
+    (4 inv_dx sum w v_i (x_i - x_p)^T) and amplifies grid-velocity rounding;
+    on these stressed states the float32 oracle itself is ~1e-5 away from
+    the float64 oracle in C, so the test also requires the HIP path to be at
+    least as close to float64 as the float32 oracle is (within 2x);
   * binning keys and permutation: bit-exact;
   * 100 substeps: relative drift <= 1e-4 (stated bound; measured values are
     printed).
@@ -17,7 +22,12 @@ from tests.helpers import cube_particles, deformed_state, oracle_cfg_for, rel_er
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-5
+TOL_C = 3e-5
 DRIFT_TOL = 1e-4
+
+
+def _tol(k):
+    return TOL_C if k == "C" else TOL
 
 
 @pytest.fixture(autouse=True)
@@ -37,11 +47,11 @@ def _load(s, st):
     s.set_state(F=st["F"], C=st["C"], Jp=st["Jp"])
 
 
-def _compare(got, ref, fields=("x", "v", "C", "F", "Jp"), tol=TOL, label=""):
+def _compare(got, ref, fields=("x", "v", "C", "F", "Jp"), tol=None, label=""):
     errs = {k: rel_err(got[k], ref[k]) for k in fields}
     print(label, {k: f"{e:.2e}" for k, e in errs.items()}, flush=True)
     for k, e in errs.items():
-        assert e <= tol, (label, k, e)
+        assert e <= (tol if tol is not None else _tol(k)), (label, k, e)
     return errs
 
 
@@ -61,6 +71,12 @@ def test_one_substep_from_deformed_state(oracle, dim):
     cfg = oracle_cfg_for(oracle, s)
     ref, gm, gv, mv = oracle.substep(cfg, st, dt, 0.0, np.float32)
     _compare(got, ref, label=f"substep {dim}D")
+    st64 = {k: (np.asarray(v, np.float64) if k != "material" else v) for k, v in st.items()}
+    ref64, *_ = oracle.substep(cfg, st64, dt, 0.0, np.float64)
+    for k in ("x", "v", "C", "F", "Jp"):
+        e_gpu, e_32 = rel_err(got[k], ref64[k]), rel_err(ref[k], ref64[k])
+        print(f"  {k}: hip vs f64 {e_gpu:.2e}, f32 oracle vs f64 {e_32:.2e}", flush=True)
+        assert e_gpu <= 2 * e_32 + 1e-7, k
     # grid after the grid operator over the oracle's window
     m, v = s.read_grid([-8] * dim, [res + 16] * dim)
     assert rel_err(m, gm) <= TOL
@@ -143,7 +159,8 @@ def test_reorder_does_not_change_results():
         for _ in range(10):
             s.substep(s.default_dt)
         outs.append(s.get_state())
-    _compare(outs[0], outs[1], tol=TOL, label="reorder")
+    # two runs whose grid sums differ only in order: bounded like a drift
+    _compare(outs[0], outs[1], tol=DRIFT_TOL, label="reorder")
 
 
 def _drift(oracle, s, st, nsteps, dt, label):
