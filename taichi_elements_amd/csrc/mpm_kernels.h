@@ -19,6 +19,7 @@ struct Layout {
 };
 
 constexpr int kThreads = 256;
+constexpr int kChunkMax = 256;                      // particles per transfer work item (max)
 constexpr int kScanItems = 8;                       // per thread in the block scans
 constexpr int kScanTile = kThreads * kScanItems;    // 2048 blocks per workgroup
 constexpr int kRadixItems = 16;
@@ -48,7 +49,7 @@ struct SubstepBuffers {
   int* totals;      // kTot*
   int* hist;        // radix histograms [256 * ntiles]
   int* hist_partials;
-  float* tiles;     // p2g per-chunk tiles
+  float4* tiles;    // p2g per-chunk node tiles (momentum xyz, mass)
   float* grid;      // dense block-major grid [total_blocks][D+1][nodes]
   int* err;         // error bits
   unsigned* maxv;   // per-substep max |v| (float bits)

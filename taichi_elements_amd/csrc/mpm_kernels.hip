@@ -328,6 +328,19 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
 
 // ---------------------------------------------------------------------------
 // P2G (mpm_solver.py:487-584)
+//
+// One workgroup per chunk (<= kChunkMax particles of one grid block).
+//  phase 1  one particle per thread: constitutive update (F, Jp written back)
+//           and the particle's affine momentum terms staged in LDS;
+//  phase 2  a stable counting sort of the chunk's particles by cell;
+//  phase 3  cell-major scatter: lane l of a wave owns cell l of the block
+//           (3D: 64 cells = 64 lanes; 2D: 16 cells, four lane groups), so the
+//           27 (9) stencil nodes a wave updates in one instruction are
+//           distinct and each wave / lane group accumulates into its own LDS
+//           tile with plain read-modify-write -- no float atomics;
+//  phase 4  the private tiles are summed in a fixed order and the (4+2)^d
+//           node tile (momentum, mass) is written to HBM for the grid pass.
+// The accumulation order is fixed, so p2g is bitwise reproducible.
 // ---------------------------------------------------------------------------
 template <int D>
 __global__ __launch_bounds__(kThreads) void k_p2g(Params P, float* __restrict__ pf,
@@ -335,21 +348,39 @@ __global__ __launch_bounds__(kThreads) void k_p2g(Params P, float* __restrict__ 
                                                   const int* __restrict__ perm,
                                                   const int4* __restrict__ chunks,
                                                   const int* __restrict__ totals,
-                                                  float* __restrict__ tiles) {
+                                                  float4* __restrict__ tiles) {
   using L = Layout<D>;
-  constexpr int TN = Geo<D>::kTileNodes, NC = D + 1;
-  __shared__ float tile[NC * TN];
+  constexpr int TN = Geo<D>::kTileNodes;
+  constexpr int CELLS = Geo<D>::kNodesPerBlock;  // cells of a block (64 / 16)
+  constexpr int GROUPS = 64 / CELLS;             // lane groups per wave
+  constexpr int WAVES = kThreads / 64;
+  constexpr int SUBS = WAVES * GROUPS;           // private tiles
+  __shared__ float4 stage[kChunkMax][4];
+  __shared__ float4 ptile[SUBS][TN];
+  __shared__ int wcnt[WAVES][CELLS];
+  __shared__ int cstart[CELLS], ccount[CELLS];
+  __shared__ unsigned char order[kChunkMax];
+
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   const int nchunks = totals[kTotChunks];
   for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
     const int4 ck = chunks[c];
+    const int nc = ck.z - ck.y;  // <= kChunkMax
     int b[3];
     decode_local<D>(ck.x, P, b);
 #pragma unroll
     for (int a = 0; a < D; a++) b[a] = (b[a] + P.blo[a]) * kBlock;  // first node of the block
-    for (int i = threadIdx.x; i < NC * TN; i += kThreads) tile[i] = 0.0f;
-    __syncthreads();
-    for (int j = ck.y + threadIdx.x; j < ck.z; j += kThreads) {
-      const int p = perm[j];
+    for (int i = threadIdx.x; i < SUBS * TN; i += kThreads)
+      (&ptile[0][0])[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (int i = threadIdx.x; i < WAVES * CELLS; i += kThreads) (&wcnt[0][0])[i] = 0;
+
+    // ---- phase 1: constitutive update and staging ----
+    const int t = threadIdx.x;
+    const bool valid = t < nc;
+    int cell = 0;
+    if (valid) {
+      const int p = perm[ck.y + t];
       const int mat = pi[(int64_t)L::MAT * cap + p];
       float xp[3], vp[3], C[3][3], F[3][3], stress[3][3];
 #pragma unroll
@@ -365,61 +396,160 @@ __global__ __launch_bounds__(kThreads) void k_p2g(Params P, float* __restrict__ 
           F[a][e] = pf[(int64_t)(L::F + a * D + e) * cap + p];
         }
       float Jp = pf[(int64_t)L::JP * cap + p];
-      int base[3], l[3];
-      float fx[3], w[3][3];
-      bool inside = true;
+      float fx[3] = {0.0f, 0.0f, 0.0f};
+      int l[3] = {0, 0, 0};
 #pragma unroll
       for (int a = 0; a < D; a++) {
-        base_fx(xp[a], P.inv_dx, base[a], fx[a]);
-        bspline(fx[a], w[a]);
-        l[a] = base[a] - b[a];
-        inside &= (l[a] >= 0) & (l[a] < kBlock);
+        int base;
+        base_fx(xp[a], P.inv_dx, base, fx[a]);
+        // the binning pass put this particle in this block from the same x
+        l[a] = min(max(base - b[a], 0), kBlock - 1);
       }
+      cell = D == 3 ? l[0] + 4 * (l[1] + 4 * l[2]) : l[0] + 4 * l[1];
       constitutive<D>(mat, P, F, C, Jp, stress);
 #pragma unroll
       for (int a = 0; a < D; a++)
 #pragma unroll
         for (int e = 0; e < D; e++) pf[(int64_t)(L::F + a * D + e) * cap + p] = F[a][e];
       pf[(int64_t)L::JP * cap + p] = Jp;
-      if (!inside) continue;  // unreachable: keys and base come from the same x
-
       float mass = P.p_mass;  // mpm_solver.py:571-573
       if (mat == kWater) mass *= P.water_density;
-      float A[3][3], mv[3];
+      // contribution to node o: w_o * (mass v + A (o - fx) dx) = w_o * (q + Adx o)
+      // with A = stress_scale tau + mass C (mpm_solver.py:569-583)
+      float Ad[3][3] = {{0.0f}}, q[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
       for (int a = 0; a < D; a++) {
-        mv[a] = mass * vp[a];
 #pragma unroll
-        for (int e = 0; e < D; e++) A[a][e] = P.stress_scale * stress[a][e] + mass * C[a][e];
+        for (int e = 0; e < D; e++) Ad[a][e] = (P.stress_scale * stress[a][e] + mass * C[a][e]) * P.dx;
+        float s = mass * vp[a];
+#pragma unroll
+        for (int e = 0; e < D; e++) s -= Ad[a][e] * fx[e];
+        q[a] = s;
       }
-#pragma unroll
-      for (int oz = 0; oz < (D == 3 ? 3 : 1); oz++)
-#pragma unroll
-        for (int oy = 0; oy < 3; oy++)
-#pragma unroll
-          for (int ox = 0; ox < 3; ox++) {
-            const int o[3] = {ox, oy, oz};
-            float weight = 1.0f, dpos[3];
-#pragma unroll
-            for (int a = 0; a < D; a++) {
-              weight *= w[a][o[a]];
-              dpos[a] = ((float)o[a] - fx[a]) * P.dx;
-            }
-            const int node = D == 3 ? (l[0] + ox) + kTile * ((l[1] + oy) + kTile * (l[2] + oz))
-                                    : (l[0] + ox) + kTile * (l[1] + oy);
-#pragma unroll
-            for (int a = 0; a < D; a++) {
-              float ad = 0.0f;
-#pragma unroll
-              for (int e = 0; e < D; e++) ad += A[a][e] * dpos[e];
-              atomicAdd(&tile[a * TN + node], weight * (mv[a] + ad));
-            }
-            atomicAdd(&tile[D * TN + node], weight * mass);
-          }
+      if (D == 3) {
+        stage[t][0] = make_float4(Ad[0][0], Ad[0][1], Ad[0][2], Ad[1][0]);
+        stage[t][1] = make_float4(Ad[1][1], Ad[1][2], Ad[2][0], Ad[2][1]);
+        stage[t][2] = make_float4(Ad[2][2], q[0], q[1], q[2]);
+        stage[t][3] = make_float4(fx[0], fx[1], fx[2], mass);
+      } else {
+        stage[t][0] = make_float4(Ad[0][0], Ad[0][1], Ad[1][0], Ad[1][1]);
+        stage[t][1] = make_float4(q[0], q[1], fx[0], fx[1]);
+        stage[t][2] = make_float4(mass, 0.0f, 0.0f, 0.0f);
+      }
     }
     __syncthreads();
-    float* out = tiles + (int64_t)c * NC * TN;
-    for (int i = threadIdx.x; i < NC * TN; i += kThreads) out[i] = tile[i];
+
+    // ---- phase 2: stable counting sort of the chunk by cell ----
+    unsigned long long peers = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < (D == 3 ? 6 : 4); bit++) {
+      const bool on = (cell >> bit) & 1;
+      const unsigned long long bal = __ballot(on);
+      peers &= on ? bal : ~bal;
+    }
+    const int rank = (int)__popcll(peers & lt);
+    if (valid && rank == 0) wcnt[wave][cell] = (int)__popcll(peers);
+    __syncthreads();
+    if (threadIdx.x < 64) {  // one wave: per-cell totals and exclusive scan
+      int run = 0;
+      if (lane < CELLS) {
+#pragma unroll
+        for (int w = 0; w < WAVES; w++) {
+          const int cnt = wcnt[w][lane];
+          wcnt[w][lane] = run;
+          run += cnt;
+        }
+      }
+      int inc = run;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(inc, off);
+        if (lane >= off) inc += y;
+      }
+      if (lane < CELLS) {
+        ccount[lane] = run;
+        cstart[lane] = inc - run;
+      }
+    }
+    __syncthreads();
+    if (valid) order[cstart[cell] + wcnt[wave][cell] + rank] = (unsigned char)t;
+    __syncthreads();
+
+    // ---- phase 3: cell-major scatter into private tiles ----
+    {
+      const int my_cell = lane % CELLS;
+      const int sub = wave * GROUPS + lane / CELLS;
+      const int lc[3] = {my_cell & 3, (my_cell >> 2) & 3, my_cell >> 4};
+      const int cbeg = cstart[my_cell], ccnt = ccount[my_cell];
+      float4* tp = ptile[sub];
+      for (int k = sub; k < ccnt; k += SUBS) {
+        const int pt = order[cbeg + k];
+        const float4 s0 = stage[pt][0], s1 = stage[pt][1], s2 = stage[pt][2];
+        float Ad[3][3], q[3], fx[3], mass, w[3][3];
+        if (D == 3) {
+          const float4 s3 = stage[pt][3];
+          Ad[0][0] = s0.x; Ad[0][1] = s0.y; Ad[0][2] = s0.z;
+          Ad[1][0] = s0.w; Ad[1][1] = s1.x; Ad[1][2] = s1.y;
+          Ad[2][0] = s1.z; Ad[2][1] = s1.w; Ad[2][2] = s2.x;
+          q[0] = s2.y; q[1] = s2.z; q[2] = s2.w;
+          fx[0] = s3.x; fx[1] = s3.y; fx[2] = s3.z;
+          mass = s3.w;
+        } else {
+          Ad[0][0] = s0.x; Ad[0][1] = s0.y; Ad[1][0] = s0.z; Ad[1][1] = s0.w;
+          Ad[0][2] = Ad[1][2] = Ad[2][0] = Ad[2][1] = Ad[2][2] = 0.0f;
+          q[0] = s1.x; q[1] = s1.y; q[2] = 0.0f;
+          fx[0] = s1.z; fx[1] = s1.w; fx[2] = 0.0f;
+          mass = s2.x;
+        }
+#pragma unroll
+        for (int a = 0; a < D; a++) bspline(fx[a], w[a]);
+#pragma unroll
+        for (int oz = 0; oz < (D == 3 ? 3 : 1); oz++) {
+          float mz[3];
+#pragma unroll
+          for (int a = 0; a < 3; a++) mz[a] = q[a] + (float)oz * Ad[a][2];
+#pragma unroll
+          for (int oy = 0; oy < 3; oy++) {
+            float my[3];
+#pragma unroll
+            for (int a = 0; a < 3; a++) my[a] = mz[a] + (float)oy * Ad[a][1];
+#pragma unroll
+            for (int ox = 0; ox < 3; ox++) {
+              float weight = w[0][ox] * w[1][oy];
+              if (D == 3) weight *= w[2][oz];
+              const int node = D == 3 ? (lc[0] + ox) + kTile * ((lc[1] + oy) + kTile * (lc[2] + oz))
+                                      : (lc[0] + ox) + kTile * (lc[1] + oy);
+              float4 acc = tp[node];
+              acc.x += weight * (my[0] + (float)ox * Ad[0][0]);
+              acc.y += weight * (my[1] + (float)ox * Ad[1][0]);
+              if (D == 3) acc.z += weight * (my[2] + (float)ox * Ad[2][0]);
+              acc.w += weight * mass;
+              tp[node] = acc;
+              // Lanes of a wave update distinct nodes in one instruction, but a
+              // later node of one lane can be an earlier node of another: keep
+              // every LDS access in program order (LDS serves a wave in order).
+              __asm__ volatile("" ::: "memory");
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+
+    // ---- phase 4: sum the private tiles, write the node tile ----
+    float4* out = tiles + (int64_t)c * TN;
+    for (int i = threadIdx.x; i < TN; i += kThreads) {
+      float4 acc = ptile[0][i];
+#pragma unroll
+      for (int sdx = 1; sdx < SUBS; sdx++) {
+        const float4 v = ptile[sdx][i];
+        acc.x += v.x;
+        acc.y += v.y;
+        acc.z += v.z;
+        acc.w += v.w;
+      }
+      out[i] = acc;
+    }
     __syncthreads();
   }
 }
@@ -432,7 +562,7 @@ __global__ __launch_bounds__(kThreads) void k_grid_op(Params P, const int* __res
                                                       const int* __restrict__ totals,
                                                       const int* __restrict__ pchunk_begin,
                                                       const int* __restrict__ pchunk_cnt,
-                                                      const float* __restrict__ tiles,
+                                                      const float4* __restrict__ tiles,
                                                       float* __restrict__ grid, int raw) {
   constexpr int NPB = Geo<D>::kNodesPerBlock, TN = Geo<D>::kTileNodes, NC = D + 1;
   const int64_t total = (int64_t)totals[kTotActive] * NPB;
@@ -464,10 +594,11 @@ __global__ __launch_bounds__(kThreads) void k_grid_op(Params P, const int* __res
           const int c0 = pchunk_begin[nb], cn = pchunk_cnt[nb];
           const int tn = D == 3 ? tl[0] + kTile * (tl[1] + kTile * tl[2]) : tl[0] + kTile * tl[1];
           for (int k = c0; k < c0 + cn; k++) {
-            const float* tp = tiles + (int64_t)k * NC * TN + tn;
-#pragma unroll
-            for (int a = 0; a < D; a++) p[a] += tp[a * TN];
-            m += tp[D * TN];
+            const float4 tv = tiles[(int64_t)k * TN + tn];
+            p[0] += tv.x;
+            p[1] += tv.y;
+            if (D == 3) p[2] += tv.z;
+            m += tv.w;
           }
         }
     if (!raw) {

@@ -117,7 +117,7 @@ struct mpm_solver {
   int* perm = nullptr;
   int* perm_alt = nullptr;
   int* hist = nullptr;
-  float* tiles = nullptr;
+  float4* tiles = nullptr;
   int4* chunks = nullptr;
   int64_t chunk_cap = 0;
   // buffers sized by the block window
@@ -138,7 +138,7 @@ struct mpm_solver {
   bool have_perm = false;  // perm of the previous substep matches the slots
   int since_reorder = 0;
   int reorder_interval = 16;
-  int chunk = 512;
+  int chunk = kChunkMax;
   int persistent_groups = 2048;
   int debug = 0;
   bool timing = false;
@@ -261,7 +261,7 @@ int grow(mpm_solver* s, int64_t need) {
   TRY(dalloc(&s->hist, kRadixBins * ntiles));
   s->chunk_cap = std::min<int64_t>(s->total_blocks, ncap) + (ncap + s->chunk - 1) / s->chunk;
   const int TN = s->D == 3 ? Geo<3>::kTileNodes : Geo<2>::kTileNodes;
-  TRY(dalloc(&s->tiles, s->chunk_cap * (s->D + 1) * TN));
+  TRY(dalloc(&s->tiles, s->chunk_cap * TN));
   TRY(dalloc(&s->chunks, s->chunk_cap));
   // partials: block-window scan parts + radix-histogram scan parts
   const int64_t nparts = (s->total_blocks + kScanTile - 1) / kScanTile;
@@ -333,7 +333,7 @@ int mpm_create(const mpm_config* cfg, mpm_solver** out) {
   s->device = cfg->device;
   if (cfg->reorder_interval > 0) s->reorder_interval = cfg->reorder_interval;
   if (cfg->reorder_interval < 0) s->reorder_interval = 0;
-  if (cfg->chunk > 0) s->chunk = cfg->chunk;
+  if (cfg->chunk > 0) s->chunk = std::min(cfg->chunk, kChunkMax);
   s->gravity[1] = -9.8;  // mpm_solver.py:283, 296
   // Block window: 2 blocks below the domain, enough above it for the wall
   // padding plus a margin; a particle block must also have its +1 neighbour

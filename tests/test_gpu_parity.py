@@ -82,7 +82,7 @@ def test_one_substep_from_deformed_state(oracle, dim):
     assert rel_err(m, gm) <= TOL
     assert rel_err(v, gv) <= TOL
     # max velocity of the substep (compute_max_velocity)
-    assert abs(s.all_time_max_velocity - mv) <= 1e-6 * max(1.0, mv)
+    assert abs(s.all_time_max_velocity - mv) <= TOL * max(1.0, mv)
 
 
 @pytest.mark.parametrize("dim", [2, 3])
