@@ -76,6 +76,21 @@ def cpu_baseline(x, v, m, dt, seconds):
                        f"(oracle C restatement, OpenMP atomics, float32), {el:.1f} s")
 
 
+def copy_calibration(nbytes=256 << 20):
+    """One device-to-device hipMemcpy of nbytes (reads nbytes, writes nbytes)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return nbytes
+
+
 def measured_traffic():
     """HBM bytes per p2g launch from the committed rocprofv3 counter summary."""
     path = os.path.join(ROOT, "profiles", "p2g_3d_1M_traffic.json")
@@ -94,6 +109,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--reorder-interval", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 256 MiB device-to-device copy (hipMemcpy) so that "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE can be scaled by a kernel of "
+                         "known traffic")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the substep engine "
@@ -170,6 +189,8 @@ def main():
         "batched": None,
         "fwd_bwd": None,
     }
+    if args.copy_calibration:
+        out["copy_calibration_bytes"] = copy_calibration()
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(x, v, m, dt, args.cpu_seconds)

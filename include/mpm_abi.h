@@ -161,6 +161,11 @@ int mpm_set_timing(mpm_solver* s, int32_t enabled);
 int mpm_kernel_times(mpm_solver* s, char* names, int32_t names_cap, int32_t* counts,
                      double* ms, int32_t cap, int32_t* n_kernels);
 
+/* The engine's 2x2 / 3x3 SVD (the replacement of ti.svd, mpm_solver.py:525)
+ * evaluated on the host CPU with the same code the kernels inline, for tests:
+ * A = U diag(sig) V^T, row-major dim x dim. */
+int mpm_host_svd(int32_t dim, const float* A, float* U, float* sig, float* V);
+
 /* Bookkeeping of the last substep: number of valid particles, transfer
  * work items (chunks), active grid blocks, and particle blocks. */
 int mpm_last_counts(mpm_solver* s, int64_t* out4);

@@ -55,6 +55,7 @@ SIGNATURES = [
     ("mpm_kernel_times", ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int32, _I32P,
                                         ctypes.POINTER(ctypes.c_double), ctypes.c_int32, _I32P]),
     ("mpm_last_counts", ctypes.c_int, [_P, _I64P]),
+    ("mpm_host_svd", ctypes.c_int, [ctypes.c_int32, _FP, _FP, _FP, _FP]),
 ]
 
 _lib = None

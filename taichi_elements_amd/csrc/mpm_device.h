@@ -80,45 +80,63 @@ __device__ __forceinline__ int floor_div4(int b) { return b >= 0 ? (b >> 2) : -(
 // SVD F = U diag(s) V^T, U, V in SO(d), s decreasing, s[d-1] signed.
 // Replaces ti.svd (mpm_solver.py:525).  3x3: Jacobi eigen-analysis of F^T F
 // with the approximate Givens angles of McAdams et al. 2011 (UW-Madison
-// TR1690), 5 cyclic sweeps, a column sort and a Givens QR of F V.
+// TR1690), 5 cyclic sweeps, the rotations accumulated as a quaternion (so V
+// is exactly orthonormal after one final normalisation), a column sort and a
+// Givens QR of F V.  Reciprocal square roots use the hardware rsq.
 // --------------------------------------------------------------------------
-__device__ __forceinline__ void approx_givens(float app, float aqq, float apq, float& c, float& s) {
-  const float gamma = 5.82842712474619f;  // 3 + 2 sqrt 2
-  const float cstar = 0.9238795325112867f, sstar = 0.3826834323650898f;  // cos, sin pi/8
-  float ch = 2.0f * (app - aqq), sh = apq;
-  bool ok = gamma * sh * sh < ch * ch;
-  float w = 1.0f / sqrtf(ch * ch + sh * sh);
-  ch = ok ? w * ch : cstar;
-  sh = ok ? w * sh : sstar;
-  c = ch * ch - sh * sh;
-  s = 2.0f * ch * sh;
+__host__ __device__ __forceinline__ float fast_rsqrt(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return __builtin_amdgcn_rsqf(x);
+#else
+  return 1.0f / sqrtf(x);
+#endif
 }
 
+// One Jacobi conjugation of the symmetric S in the (P, Q) plane; the
+// rotation is also applied to the quaternion q = (w, x, y, z) of V.
 template <int P, int Q>
-__device__ __forceinline__ void jacobi_step(float S[3][3], float V[3][3]) {
+__host__ __device__ __forceinline__ void jacobi_q(float S[3][3], float q[4]) {
   constexpr int R = 3 - P - Q;
-  float c, s;
-  approx_givens(S[P][P], S[Q][Q], S[P][Q], c, s);
-  float spp = S[P][P], sqq = S[Q][Q], spq = S[P][Q];
-  float npp = c * c * spp + 2.0f * c * s * spq + s * s * sqq;
-  float nqq = s * s * spp - 2.0f * c * s * spq + c * c * sqq;
-  float npq = (c * c - s * s) * spq + c * s * (sqq - spp);
-  float srp = S[R][P], srq = S[R][Q];
-  S[P][P] = npp;
-  S[Q][Q] = nqq;
-  S[P][Q] = S[Q][P] = npq;
+  const float gamma = 5.82842712474619f;  // 3 + 2 sqrt 2
+  const float cstar = 0.9238795325112867f, sstar = 0.3826834323650898f;  // cos, sin pi/8
+  float ch = 2.0f * (S[P][P] - S[Q][Q]), sh = S[P][Q];
+  const bool ok = gamma * sh * sh < ch * ch;
+  const float w = fast_rsqrt(ch * ch + sh * sh);
+  ch = ok ? w * ch : cstar;
+  sh = ok ? w * sh : sstar;
+  const float c = ch * ch - sh * sh, s = 2.0f * ch * sh;  // cos, sin of the full angle
+  // S <- G^T S G with G e_P = c e_P + s e_Q, G e_Q = -s e_P + c e_Q
+  const float spp = S[P][P], sqq = S[Q][Q], spq = S[P][Q];
+  const float srp = S[R][P], srq = S[R][Q];
+  S[P][P] = c * c * spp + 2.0f * c * s * spq + s * s * sqq;
+  S[Q][Q] = s * s * spp - 2.0f * c * s * spq + c * c * sqq;
+  S[P][Q] = S[Q][P] = (c * c - s * s) * spq + c * s * (sqq - spp);
   S[R][P] = S[P][R] = c * srp + s * srq;
   S[R][Q] = S[Q][R] = -s * srp + c * srq;
-#pragma unroll
-  for (int i = 0; i < 3; i++) {
-    float vp = V[i][P], vq = V[i][Q];
-    V[i][P] = c * vp + s * vq;
-    V[i][Q] = -s * vp + c * vq;
+  // G is a rotation about axis R by the full angle; its sign is + for the
+  // cyclic planes (0,1), (1,2) and - for (0,2).  q <- q (x) (ch, t e_R).
+  const float t = (P == 0 && Q == 2) ? -sh : sh;
+  const float qw = q[0], qx = q[1], qy = q[2], qz = q[3];
+  if (R == 2) {
+    q[0] = ch * qw - t * qz;
+    q[1] = ch * qx + t * qy;
+    q[2] = ch * qy - t * qx;
+    q[3] = ch * qz + t * qw;
+  } else if (R == 0) {
+    q[0] = ch * qw - t * qx;
+    q[1] = ch * qx + t * qw;
+    q[2] = ch * qy + t * qz;
+    q[3] = ch * qz - t * qy;
+  } else {
+    q[0] = ch * qw - t * qy;
+    q[1] = ch * qx - t * qz;
+    q[2] = ch * qy + t * qw;
+    q[3] = ch * qz + t * qx;
   }
 }
 
 template <int I, int J>
-__device__ __forceinline__ void sort_cols(float B[3][3], float V[3][3]) {
+__host__ __device__ __forceinline__ void sort_cols(float B[3][3], float V[3][3]) {
   float ni = B[0][I] * B[0][I] + B[1][I] * B[1][I] + B[2][I] * B[2][I];
   float nj = B[0][J] * B[0][J] + B[1][J] * B[1][J] + B[2][J] * B[2][J];
   if (ni < nj) {
@@ -135,12 +153,12 @@ __device__ __forceinline__ void sort_cols(float B[3][3], float V[3][3]) {
 }
 
 template <int P, int Q, int D>
-__device__ __forceinline__ void givens_qr(float B[3][3], float U[3][3]) {
+__host__ __device__ __forceinline__ void givens_qr(float B[3][3], float U[3][3]) {
   float a = B[P][P], b = B[Q][P];
   float r2 = a * a + b * b;
   float c = 1.0f, s = 0.0f;
-  if (r2 > 0.0f) {
-    float ir = 1.0f / sqrtf(r2);
+  if (r2 > 1e-30f) {
+    float ir = fast_rsqrt(r2);
     c = a * ir;
     s = b * ir;
   }
@@ -155,21 +173,35 @@ __device__ __forceinline__ void givens_qr(float B[3][3], float U[3][3]) {
   }
 }
 
-__device__ __forceinline__ void svd3(const float A[3][3], float U[3][3], float sig[3], float V[3][3]) {
+__host__ __device__ __forceinline__ void svd3(const float A[3][3], float U[3][3], float sig[3],
+                                              float V[3][3]) {
   float S[3][3];
 #pragma unroll
   for (int i = 0; i < 3; i++)
 #pragma unroll
     for (int j = 0; j < 3; j++) {
       S[i][j] = A[0][i] * A[0][j] + A[1][i] * A[1][j] + A[2][i] * A[2][j];
-      V[i][j] = i == j ? 1.0f : 0.0f;
       U[i][j] = i == j ? 1.0f : 0.0f;
     }
+  float q[4] = {1.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
   for (int sweep = 0; sweep < 5; sweep++) {
-    jacobi_step<0, 1>(S, V);
-    jacobi_step<1, 2>(S, V);
-    jacobi_step<0, 2>(S, V);
+    jacobi_q<0, 1>(S, q);
+    jacobi_q<1, 2>(S, q);
+    jacobi_q<0, 2>(S, q);
+  }
+  {
+    const float in = fast_rsqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    const float w = q[0] * in, x = q[1] * in, y = q[2] * in, z = q[3] * in;
+    V[0][0] = 1.0f - 2.0f * (y * y + z * z);
+    V[0][1] = 2.0f * (x * y - w * z);
+    V[0][2] = 2.0f * (x * z + w * y);
+    V[1][0] = 2.0f * (x * y + w * z);
+    V[1][1] = 1.0f - 2.0f * (x * x + z * z);
+    V[1][2] = 2.0f * (y * z - w * x);
+    V[2][0] = 2.0f * (x * z - w * y);
+    V[2][1] = 2.0f * (y * z + w * x);
+    V[2][2] = 1.0f - 2.0f * (x * x + y * y);
   }
   float B[3][3];
 #pragma unroll
@@ -188,7 +220,7 @@ __device__ __forceinline__ void svd3(const float A[3][3], float U[3][3], float s
 }
 
 // 2x2: one exact Jacobi rotation of F^T F, column sort, one Givens QR.
-__device__ __forceinline__ void svd2(const float A[3][3], float U[3][3], float sig[3], float V[3][3]) {
+__host__ __device__ __forceinline__ void svd2(const float A[3][3], float U[3][3], float sig[3], float V[3][3]) {
   float s00 = A[0][0] * A[0][0] + A[1][0] * A[1][0];
   float s11 = A[0][1] * A[0][1] + A[1][1] * A[1][1];
   float s01 = A[0][0] * A[0][1] + A[1][0] * A[1][1];
@@ -287,11 +319,33 @@ __device__ __forceinline__ void constitutive(int mat, const Params& P, float F[3
 #pragma unroll
     for (int j = 0; j < D; j++) F[i][j] = G[i][j];
 
-  float h = 1.0f;  // mpm_solver.py:515-524
-  if (mat != kWater) h = expf(10.0f * (1.0f - Jp));
+  if (mat == kWater) {
+    // Water (mpm_solver.py:515-551 with mu = 0, h = 1): the singular values
+    // are never clamped, so J = prod(sigma) = det F and the deviatoric term
+    // vanishes; F = diag(J, 1, ..), Jp = J, tau = lambda_0 J (J - 1) I.
+    const float J = D == 3 ? F[0][0] * (F[1][1] * F[2][2] - F[1][2] * F[2][1]) -
+                                 F[0][1] * (F[1][0] * F[2][2] - F[1][2] * F[2][0]) +
+                                 F[0][2] * (F[1][0] * F[2][1] - F[1][1] * F[2][0])
+                           : F[0][0] * F[1][1] - F[0][1] * F[1][0];
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+      for (int j = 0; j < D; j++) {
+        F[i][j] = i == j ? 1.0f : 0.0f;
+        stress[i][j] = 0.0f;
+      }
+    F[0][0] = J;
+    Jp = J;
+    const float iso = P.lambda_0 * J * (J - 1.0f);
+#pragma unroll
+    for (int i = 0; i < D; i++) stress[i][i] = iso;
+    return;
+  }
+
+  float h = 1.0f;  // mpm_solver.py:515-524 (sand ignores h: its stress uses mu_0, lambda_0)
+  if (mat == kSnow) h = __expf(10.0f * (1.0f - Jp));
   if (mat == kElastic) h = 0.3f;
   float mu = P.mu_0 * h, la = P.lambda_0 * h;
-  if (mat == kWater) mu = 0.0f;
 
   float U[3][3], V[3][3], sig[3];
   if (D == 3)

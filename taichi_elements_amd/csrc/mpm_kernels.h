@@ -20,11 +20,20 @@ struct Layout {
 
 constexpr int kThreads = 256;
 constexpr int kChunkMax = 256;                      // particles per transfer work item (max)
-constexpr int kScanItems = 8;                       // per thread in the block scans
-constexpr int kScanTile = kThreads * kScanItems;    // 2048 blocks per workgroup
+constexpr int kScanItems = 2;                       // per thread in the block scan
+constexpr int kScanTile = kThreads * kScanItems;    // 512 blocks per workgroup
 constexpr int kRadixItems = 16;
-constexpr int kRadixTile = kThreads * kRadixItems;  // 4096 keys per workgroup
+constexpr int kRadixTile = kThreads * kRadixItems;  // 2048 keys per workgroup
 constexpr int kRadixBins = 256;
+constexpr int kMaxRadixPasses = 3;                  // block keys up to 24 bits
+
+// Ticket / epoch state shared by the single-pass (look-back) scans.
+struct LBState {
+  unsigned long long tile_ctr;
+  unsigned long long done_ctr;
+  unsigned epoch;
+  unsigned pad;
+};
 
 // Totals written on the device by the block scan (read by persistent kernels).
 enum Total : int { kTotValid = 0, kTotChunks = 1, kTotActive = 2, kTotPBlocks = 3, kTotCount = 4 };
@@ -47,8 +56,10 @@ struct SubstepBuffers {
   int* active;      // active grid blocks
   int4* partials;   // block-scan partial sums
   int* totals;      // kTot*
-  int* hist;        // radix histograms [256 * ntiles]
-  int* hist_partials;
+  int* ghist;       // radix digit histograms, 2 x [kMaxRadixPasses][256] (substep parity)
+  unsigned long long* lb_status_scan;   // look-back status of k_bscan  [4 * tiles]
+  unsigned long long* lb_status_radix;  // look-back status of k_onesweep [256 * tiles]
+  LBState* lbs;
   float4* tiles;    // p2g per-chunk node tiles (momentum xyz, mass)
   float* grid;      // dense block-major grid [total_blocks][D+1][nodes]
   int* err;         // error bits
@@ -57,6 +68,7 @@ struct SubstepBuffers {
   int key_bits;
   int chunk;
   int persistent_groups;
+  int parity;  // substep counter parity (selects the digit-histogram copy)
 };
 
 class Timer;

@@ -104,12 +104,14 @@ __global__ __launch_bounds__(kThreads) void k_keys(Params P, const float* __rest
       keys[i] = key;
       if (!valid) atomicOr(err, 1);
     }
-    // wave-aggregated counting: sorted input gives 1-3 distinct keys per wave
-    unsigned long long pending = __ballot(valid);
+    // wave-aggregated counting (count[total_blocks] counts escaped particles):
+    // sorted input gives 1-3 distinct keys per wave
+    const bool in = i < n;
+    unsigned long long pending = __ballot(in);
     while (pending) {
       const int leader = __ffsll(pending) - 1;
       const int lk = __shfl(key, leader);
-      const unsigned long long same = __ballot(valid && key == lk);
+      const unsigned long long same = __ballot(in && key == lk);
       if (lane == leader) atomicAdd(&count[lk], (int)__popcll(same));
       pending &= ~same;
     }
@@ -139,50 +141,102 @@ __device__ __forceinline__ int4 block_values(int bid, const int* __restrict__ co
   return make_int4(c, (c + chunk - 1) / chunk, act ? 1 : 0, c > 0 ? 1 : 0);
 }
 
-template <int D>
-__global__ __launch_bounds__(kThreads) void k_bscan_reduce(Params P, const int* __restrict__ count,
-                                                           int total_blocks, int chunk,
-                                                           int4* __restrict__ partials) {
-  __shared__ int4 wsum[kThreads / 64];
-  int4 acc = make_int4(0, 0, 0, 0);
-#pragma unroll
-  for (int k = 0; k < kScanItems; k++) {
-    const int bid = blockIdx.x * kScanTile + k * kThreads + threadIdx.x;
-    if (bid < total_blocks) acc = add4(acc, block_values<D>(bid, count, P, chunk));
-  }
-  int4 tot;
-  block_exscan4(acc, &tot, wsum);
-  if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+// ---------------------------------------------------------------------------
+// single-pass scans with decoupled look-back (Merrill & Garland 2016)
+//
+// Status word: value in bits 0-31, flag in bits 32-33 (1 = tile aggregate,
+// 2 = inclusive prefix), launch epoch in bits 34-63.  Words left by earlier
+// launches carry an older epoch and read as "not ready", so the status
+// arrays are never cleared.  Tiles are numbered in the order workgroups start
+// (an atomic ticket), so a tile only waits on tiles that are already running;
+// the last workgroup to finish resets the ticket and advances the epoch,
+// which keeps the kernels free of host-side per-launch state.
+// ---------------------------------------------------------------------------
+constexpr unsigned long long kFlagAgg = 1ull << 32, kFlagPre = 2ull << 32;
+constexpr int kSpinLimit = 1 << 18;
+
+__device__ __forceinline__ unsigned long long lb_epoch_bits(unsigned epoch) {
+  return (unsigned long long)(epoch & 0x3fffffffu) << 34;
 }
 
-__global__ __launch_bounds__(kThreads) void k_bscan_top(int4* __restrict__ partials, int nparts,
-                                                        int* __restrict__ totals) {
-  __shared__ int4 wsum[kThreads / 64];
-  int4 carry = make_int4(0, 0, 0, 0);
-  for (int base = 0; base < nparts; base += kThreads) {
-    const int i = base + threadIdx.x;
-    int4 v = i < nparts ? partials[i] : make_int4(0, 0, 0, 0);
-    int4 tot;
-    int4 ex = block_exscan4(v, &tot, wsum);
-    if (i < nparts) partials[i] = add4(ex, carry);
-    carry = add4(carry, tot);
+__device__ __forceinline__ void lb_publish(unsigned long long* p, unsigned long long flag,
+                                           unsigned epoch, unsigned value) {
+  __hip_atomic_store(p, lb_epoch_bits(epoch) | flag | (unsigned long long)value, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Exclusive prefix of channel `chan` for tile `tile` (status[t * stride + chan]).
+// A bounded wait: past kSpinLimit polls it flags err bit 1 and returns what it has.
+__device__ __forceinline__ unsigned lb_lookback(const unsigned long long* status, int stride,
+                                                int tile, int chan, unsigned epoch, int* err) {
+  const unsigned long long ep = lb_epoch_bits(epoch);
+  unsigned excl = 0;
+  for (int t = tile - 1; t >= 0; t--) {
+    unsigned long long w = 0;
+    int spins = 0;
+    for (;;) {
+      w = __hip_atomic_load(&status[(int64_t)t * stride + chan], __ATOMIC_RELAXED,
+                            __HIP_MEMORY_SCOPE_AGENT);
+      if ((w & ~0x3ffffffffull) == ep && (w & (kFlagAgg | kFlagPre))) break;
+      if (++spins > kSpinLimit) {
+        atomicOr(err, 2);
+        return excl;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    excl += (unsigned)w;
+    if (w & kFlagPre) break;
   }
+  return excl;
+}
+
+struct LBTicket {
+  int tile;
+  unsigned epoch;
+};
+
+__device__ __forceinline__ LBTicket lb_acquire(LBState* st, LBTicket* sh) {
   if (threadIdx.x == 0) {
-    totals[kTotValid] = carry.x;
-    totals[kTotChunks] = carry.y;
-    totals[kTotActive] = carry.z;
-    totals[kTotPBlocks] = carry.w;
+    sh->tile = (int)atomicAdd(&st->tile_ctr, 1ull);
+    sh->epoch = __hip_atomic_load(&st->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  return *sh;
+}
+
+__device__ __forceinline__ void lb_release(LBState* st, int ntiles, unsigned epoch) {
+  if (threadIdx.x == 0) {
+    const unsigned long long d = atomicAdd(&st->done_ctr, 1ull);
+    if (d == (unsigned long long)(ntiles - 1)) {
+      __hip_atomic_store(&st->tile_ctr, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&st->done_ctr, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&st->epoch, epoch + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
 }
 
+// ---------------------------------------------------------------------------
+// scan of the dense block window (one launch): particle offsets, chunk list,
+// active grid blocks, and the radix digit histograms of the block keys
+// weighted by the particle counts (count[total_blocks] holds the particles
+// outside the window, whose key is total_blocks).
+// ---------------------------------------------------------------------------
 template <int D>
-__global__ __launch_bounds__(kThreads) void k_bscan_down(
+__global__ __launch_bounds__(kThreads) void k_bscan(
     Params P, const int* __restrict__ count, int* __restrict__ count_next, int total_blocks,
-    int chunk, const int4* __restrict__ partials, int* __restrict__ pstart,
-    int* __restrict__ pchunk_begin, int* __restrict__ pchunk_cnt, int4* __restrict__ chunks,
-    int* __restrict__ active) {
+    int chunk, int ntiles, int passes, int* __restrict__ pstart, int* __restrict__ pchunk_begin,
+    int* __restrict__ pchunk_cnt, int4* __restrict__ chunks, int* __restrict__ active,
+    int* __restrict__ totals, int* __restrict__ ghist, int* __restrict__ ghist_next,
+    unsigned long long* __restrict__ status, LBState* __restrict__ lbs, int* __restrict__ err) {
+  __shared__ LBTicket tk_sh;
   __shared__ int4 wsum[kThreads / 64];
-  const int first = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  __shared__ unsigned excl_sh[4];
+  __shared__ int lh[kMaxRadixPasses][kRadixBins];
+  const LBTicket tk = lb_acquire(lbs, &tk_sh);
+  const int tile = tk.tile;
+  for (int i = threadIdx.x; i < kMaxRadixPasses * kRadixBins; i += kThreads) (&lh[0][0])[i] = 0;
+
+  const int first = tile * kScanTile + threadIdx.x * kScanItems;
   int4 vals[kScanItems];
   int4 acc = make_int4(0, 0, 0, 0);
 #pragma unroll
@@ -192,7 +246,37 @@ __global__ __launch_bounds__(kThreads) void k_bscan_down(
     acc = add4(acc, vals[k]);
   }
   int4 tot;
-  int4 pre = add4(block_exscan4(acc, &tot, wsum), partials[blockIdx.x]);
+  int4 pre = block_exscan4(acc, &tot, wsum);  // contains __syncthreads (lh zeroed)
+
+  // digit histograms of the keys, weighted by particle counts
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) {
+    const int bid = first + k;
+    if (bid < total_blocks && vals[k].x > 0)
+      for (int ps = 0; ps < passes; ps++) atomicAdd(&lh[ps][(bid >> (8 * ps)) & 255], vals[k].x);
+  }
+  if (tile == 0 && threadIdx.x == 0) {
+    const int inval = count[total_blocks];
+    if (inval > 0)
+      for (int ps = 0; ps < passes; ps++)
+        atomicAdd(&lh[ps][(total_blocks >> (8 * ps)) & 255], inval);
+  }
+
+  if (threadIdx.x < 4) {
+    const int c = threadIdx.x;
+    const unsigned agg = (unsigned)(c == 0 ? tot.x : c == 1 ? tot.y : c == 2 ? tot.z : tot.w);
+    unsigned excl = 0;
+    if (tile == 0) {
+      lb_publish(&status[c], kFlagPre, tk.epoch, agg);
+    } else {
+      lb_publish(&status[(int64_t)tile * 4 + c], kFlagAgg, tk.epoch, agg);
+      excl = lb_lookback(status, 4, tile, c, tk.epoch, err);
+      lb_publish(&status[(int64_t)tile * 4 + c], kFlagPre, tk.epoch, excl + agg);
+    }
+    excl_sh[c] = excl;
+  }
+  __syncthreads();
+  pre = add4(pre, make_int4((int)excl_sh[0], (int)excl_sh[1], (int)excl_sh[2], (int)excl_sh[3]));
 #pragma unroll
   for (int k = 0; k < kScanItems; k++) {
     const int bid = first + k;
@@ -210,92 +294,66 @@ __global__ __launch_bounds__(kThreads) void k_bscan_down(
     count_next[bid] = 0;
     pre = add4(pre, v);
   }
-}
-
-// ---------------------------------------------------------------------------
-// flat exclusive scan of ints (radix histograms)
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void k_scan_reduce(const int* __restrict__ in, int n,
-                                                          int4* __restrict__ partials) {
-  __shared__ int4 wsum[kThreads / 64];
-  int acc = 0;
-#pragma unroll
-  for (int k = 0; k < kScanItems; k++) {
-    const int i = blockIdx.x * kScanTile + k * kThreads + threadIdx.x;
-    if (i < n) acc += in[i];
+  if (tile == 0 && threadIdx.x == 0) count_next[total_blocks] = 0;
+  if (tile == ntiles - 1 && threadIdx.x == 0) {
+    totals[kTotValid] = (int)excl_sh[0] + tot.x;
+    totals[kTotChunks] = (int)excl_sh[1] + tot.y;
+    totals[kTotActive] = (int)excl_sh[2] + tot.z;
+    totals[kTotPBlocks] = (int)excl_sh[3] + tot.w;
   }
-  int4 tot;
-  block_exscan4(make_int4(acc, 0, 0, 0), &tot, wsum);
-  if (threadIdx.x == 0) partials[blockIdx.x] = tot;
-}
-
-__global__ __launch_bounds__(kThreads) void k_scan_down(int* __restrict__ data, int n,
-                                                        const int4* __restrict__ partials) {
-  __shared__ int4 wsum[kThreads / 64];
-  const int first = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-  int vals[kScanItems];
-  int acc = 0;
-#pragma unroll
-  for (int k = 0; k < kScanItems; k++) {
-    vals[k] = first + k < n ? data[first + k] : 0;
-    acc += vals[k];
+  // flush the histograms; clear next substep's copy
+  for (int i = threadIdx.x; i < passes * kRadixBins; i += kThreads) {
+    const int h = (&lh[0][0])[i];
+    if (h) atomicAdd(&ghist[i], h);
   }
-  int4 tot;
-  int pre = block_exscan4(make_int4(acc, 0, 0, 0), &tot, wsum).x + partials[blockIdx.x].x;
-#pragma unroll
-  for (int k = 0; k < kScanItems; k++) {
-    if (first + k < n) data[first + k] = pre;
-    pre += vals[k];
-  }
-}
-
-// ---------------------------------------------------------------------------
-// stable LSD radix sort, 8-bit digits, 4096 keys per workgroup
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void k_radix_hist(const int* __restrict__ keys, int n,
-                                                         int shift, int ntiles,
-                                                         int* __restrict__ hist) {
-  __shared__ int cnt[kRadixBins];
-  const int lane = threadIdx.x & 63;
-  const int tile = blockIdx.x;
-  cnt[threadIdx.x] = 0;
+  if (tile == 0)
+    for (int i = threadIdx.x; i < kMaxRadixPasses * kRadixBins; i += kThreads) ghist_next[i] = 0;
   __syncthreads();
-#pragma unroll 4
-  for (int r = 0; r < kRadixItems; r++) {
-    const int i = tile * kRadixTile + r * kThreads + threadIdx.x;
-    const bool valid = i < n;
-    const int d = valid ? (keys[i] >> shift) & (kRadixBins - 1) : 0;
-    unsigned long long pending = __ballot(valid);
-    while (pending) {
-      const int leader = __ffsll(pending) - 1;
-      const int ld = __shfl(d, leader);
-      const unsigned long long same = __ballot(valid && d == ld);
-      if (lane == leader) atomicAdd(&cnt[ld], (int)__popcll(same));
-      pending &= ~same;
-    }
-  }
-  __syncthreads();
-  hist[threadIdx.x * ntiles + tile] = cnt[threadIdx.x];
+  lb_release(lbs, ntiles, tk.epoch);
 }
 
-__global__ __launch_bounds__(kThreads) void k_radix_scatter(
+// ---------------------------------------------------------------------------
+// one pass of the stable LSD radix sort (8-bit digit), single launch.  Each
+// wave ranks its own contiguous quarter of the tile with ballots and
+// wave-private LDS counters (no workgroup barrier per round); one prefix over
+// the four waves gives tile-local stable ranks; decoupled look-back over
+// tiles gives the per-digit tile offsets; the digit totals come from the
+// histogram k_bscan built.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_onesweep(
     const int* __restrict__ keys_in, const int* __restrict__ vals_in, int* __restrict__ keys_out,
-    int* __restrict__ vals_out, int n, int shift, int ntiles, const int* __restrict__ hist_ex) {
-  __shared__ int wcnt[kThreads / 64][kRadixBins];
-  __shared__ int run[kRadixBins];
-  const int tile = blockIdx.x;
+    int* __restrict__ vals_out, int n, int shift, int ntiles, const int* __restrict__ ghist,
+    unsigned long long* __restrict__ status, LBState* __restrict__ lbs, int* __restrict__ err) {
+  constexpr int WAVES = kThreads / 64;
+  constexpr int PER_WAVE = kRadixTile / WAVES;
+  constexpr int ROUNDS = PER_WAVE / 64;
+  __shared__ LBTicket tk_sh;
+  __shared__ int wcnt[WAVES][kRadixBins];
+  __shared__ int gbase[kRadixBins];
+  __shared__ int4 wsum[WAVES];
+  const LBTicket tk = lb_acquire(lbs, &tk_sh);
+  const int tile = tk.tile;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-  run[threadIdx.x] = hist_ex[threadIdx.x * ntiles + tile];
-  for (int r = 0; r < kRadixItems; r++) {
-    const int i = tile * kRadixTile + r * kThreads + threadIdx.x;
-    const bool valid = i < n;
-    const int key = valid ? keys_in[i] : 0;
-    const int val = valid ? (vals_in ? vals_in[i] : i) : 0;
-    const int d = (key >> shift) & (kRadixBins - 1);
+  const int i0 = tile * kRadixTile + wave * PER_WAVE + lane;
+
+  int key[ROUNDS], val[ROUNDS], lr[ROUNDS];
 #pragma unroll
-    for (int w = 0; w < kThreads / 64; w++) wcnt[w][threadIdx.x] = 0;
-    __syncthreads();
+  for (int r = 0; r < ROUNDS; r++) {
+    const int i = i0 + r * 64;
+    key[r] = i < n ? keys_in[i] : 0;
+    val[r] = i < n ? (vals_in ? vals_in[i] : i) : 0;
+  }
+#pragma unroll
+  for (int w = 0; w < WAVES; w++) wcnt[w][threadIdx.x] = 0;
+  int4 gtot;
+  // block_exscan4 synchronises, which also publishes the zeroed counters
+  const int gex = block_exscan4(make_int4(ghist[threadIdx.x], 0, 0, 0), &gtot, wsum).x;
+  int* my = wcnt[wave];
+#pragma unroll
+  for (int r = 0; r < ROUNDS; r++) {
+    const bool valid = i0 + r * 64 < n;
+    const int d = (key[r] >> shift) & (kRadixBins - 1);
     unsigned long long peers = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; b++) {
@@ -304,26 +362,50 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
       peers &= bit ? bal : ~bal;
     }
     const int rank = (int)__popcll(peers & lt);
-    if (valid && rank == 0) wcnt[wave][d] = (int)__popcll(peers);
-    __syncthreads();
-    {
-      int x = run[threadIdx.x];
-#pragma unroll
-      for (int w = 0; w < kThreads / 64; w++) {
-        const int c = wcnt[w][threadIdx.x];
-        wcnt[w][threadIdx.x] = x;
-        x += c;
-      }
-      run[threadIdx.x] = x;
-    }
-    __syncthreads();
-    if (valid) {
-      const int pos = wcnt[wave][d] + rank;
-      keys_out[pos] = key;
-      vals_out[pos] = val;
-    }
-    __syncthreads();
+    // every lane reads its digit's counter before the group leader bumps it;
+    // a wave's LDS accesses are served in program order
+    const int base = my[d];
+    __asm__ volatile("" ::: "memory");
+    if (valid && rank == 0) my[d] = base + (int)__popcll(peers);
+    __asm__ volatile("" ::: "memory");
+    lr[r] = base + rank;
   }
+  __syncthreads();
+  int agg = 0;
+#pragma unroll
+  for (int w = 0; w < WAVES; w++) {
+    const int c = wcnt[w][threadIdx.x];
+    wcnt[w][threadIdx.x] = agg;
+    agg += c;
+  }
+  {
+    unsigned excl = 0;
+    unsigned long long* st = &status[(int64_t)tile * kRadixBins + threadIdx.x];
+    if (tile == 0) {
+      lb_publish(st, kFlagPre, tk.epoch, (unsigned)agg);
+    } else {
+      lb_publish(st, kFlagAgg, tk.epoch, (unsigned)agg);
+      excl = lb_lookback(status, kRadixBins, tile, threadIdx.x, tk.epoch, err);
+      lb_publish(st, kFlagPre, tk.epoch, excl + (unsigned)agg);
+    }
+    gbase[threadIdx.x] = gex + (int)excl;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < ROUNDS; r++) {
+    if (i0 + r * 64 < n) {
+      const int d = (key[r] >> shift) & (kRadixBins - 1);
+      const int pos = gbase[d] + my[d] + lr[r];
+      if (pos >= 0 && pos < n) {
+        keys_out[pos] = key[r];
+        vals_out[pos] = val[r];
+      } else {
+        atomicOr(err, 4);  // histogram and keys disagree (cannot happen)
+      }
+    }
+  }
+  __syncthreads();
+  lb_release(lbs, ntiles, tk.epoch);
 }
 
 // ---------------------------------------------------------------------------
@@ -355,6 +437,7 @@ __global__ __launch_bounds__(kThreads) void k_p2g(Params P, float* __restrict__ 
   constexpr int GROUPS = 64 / CELLS;             // lane groups per wave
   constexpr int WAVES = kThreads / 64;
   constexpr int SUBS = WAVES * GROUPS;           // private tiles
+  constexpr int kBatch = 2;                      // particles per lane per scatter pass
   __shared__ float4 stage[kChunkMax][4];
   __shared__ float4 ptile[SUBS][TN];
   __shared__ int wcnt[WAVES][CELLS];
@@ -482,56 +565,78 @@ __global__ __launch_bounds__(kThreads) void k_p2g(Params P, float* __restrict__ 
       const int lc[3] = {my_cell & 3, (my_cell >> 2) & 3, my_cell >> 4};
       const int cbeg = cstart[my_cell], ccnt = ccount[my_cell];
       float4* tp = ptile[sub];
-      for (int k = sub; k < ccnt; k += SUBS) {
-        const int pt = order[cbeg + k];
-        const float4 s0 = stage[pt][0], s1 = stage[pt][1], s2 = stage[pt][2];
-        float Ad[3][3], q[3], fx[3], mass, w[3][3];
-        if (D == 3) {
-          const float4 s3 = stage[pt][3];
-          Ad[0][0] = s0.x; Ad[0][1] = s0.y; Ad[0][2] = s0.z;
-          Ad[1][0] = s0.w; Ad[1][1] = s1.x; Ad[1][2] = s1.y;
-          Ad[2][0] = s1.z; Ad[2][1] = s1.w; Ad[2][2] = s2.x;
-          q[0] = s2.y; q[1] = s2.z; q[2] = s2.w;
-          fx[0] = s3.x; fx[1] = s3.y; fx[2] = s3.z;
-          mass = s3.w;
-        } else {
-          Ad[0][0] = s0.x; Ad[0][1] = s0.y; Ad[1][0] = s0.z; Ad[1][1] = s0.w;
-          Ad[0][2] = Ad[1][2] = Ad[2][0] = Ad[2][1] = Ad[2][2] = 0.0f;
-          q[0] = s1.x; q[1] = s1.y; q[2] = 0.0f;
-          fx[0] = s1.z; fx[1] = s1.w; fx[2] = 0.0f;
-          mass = s2.x;
+      // A lane's particles share its cell, hence its 27 (9) nodes: take them
+      // kBatch at a time and do one read-modify-write per node per batch.
+      for (int k0 = sub; k0 < ccnt; k0 += SUBS * kBatch) {
+        float Ad[kBatch][3][3], q[kBatch][3], mass[kBatch], w[kBatch][3][3];
+#pragma unroll
+        for (int bi = 0; bi < kBatch; bi++) {
+          const int k = k0 + bi * SUBS;
+          float4 s0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), s1 = s0, s2 = s0, s3 = s0;
+          if (k < ccnt) {
+            const int pt = order[cbeg + k];
+            s0 = stage[pt][0];
+            s1 = stage[pt][1];
+            s2 = stage[pt][2];
+            s3 = stage[pt][3];
+          }
+          float fx[3];
+          if (D == 3) {
+            Ad[bi][0][0] = s0.x; Ad[bi][0][1] = s0.y; Ad[bi][0][2] = s0.z;
+            Ad[bi][1][0] = s0.w; Ad[bi][1][1] = s1.x; Ad[bi][1][2] = s1.y;
+            Ad[bi][2][0] = s1.z; Ad[bi][2][1] = s1.w; Ad[bi][2][2] = s2.x;
+            q[bi][0] = s2.y; q[bi][1] = s2.z; q[bi][2] = s2.w;
+            fx[0] = s3.x; fx[1] = s3.y; fx[2] = s3.z;
+            mass[bi] = s3.w;
+          } else {
+            Ad[bi][0][0] = s0.x; Ad[bi][0][1] = s0.y; Ad[bi][1][0] = s0.z; Ad[bi][1][1] = s0.w;
+            Ad[bi][0][2] = Ad[bi][1][2] = Ad[bi][2][0] = Ad[bi][2][1] = Ad[bi][2][2] = 0.0f;
+            q[bi][0] = s1.x; q[bi][1] = s1.y; q[bi][2] = 0.0f;
+            fx[0] = s1.z; fx[1] = s1.w; fx[2] = 0.0f;
+            mass[bi] = s2.x;
+          }
+#pragma unroll
+          for (int a = 0; a < 3; a++) bspline(fx[a], w[bi][a]);
+          if (k >= ccnt) {  // an empty slot contributes exactly nothing
+#pragma unroll
+            for (int a = 0; a < 3; a++) w[bi][a][0] = w[bi][a][1] = w[bi][a][2] = 0.0f;
+          }
         }
 #pragma unroll
-        for (int a = 0; a < D; a++) bspline(fx[a], w[a]);
+        for (int oz = 0; oz < (D == 3 ? 3 : 1); oz++)
 #pragma unroll
-        for (int oz = 0; oz < (D == 3 ? 3 : 1); oz++) {
-          float mz[3];
-#pragma unroll
-          for (int a = 0; a < 3; a++) mz[a] = q[a] + (float)oz * Ad[a][2];
-#pragma unroll
-          for (int oy = 0; oy < 3; oy++) {
-            float my[3];
-#pragma unroll
-            for (int a = 0; a < 3; a++) my[a] = mz[a] + (float)oy * Ad[a][1];
+          for (int oy = 0; oy < 3; oy++)
 #pragma unroll
             for (int ox = 0; ox < 3; ox++) {
-              float weight = w[0][ox] * w[1][oy];
-              if (D == 3) weight *= w[2][oz];
+              float4 add = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+              for (int bi = 0; bi < kBatch; bi++) {
+                float weight = w[bi][0][ox] * w[bi][1][oy];
+                if (D == 3) weight *= w[bi][2][oz];
+                const float fo[3] = {(float)ox, (float)oy, (float)oz};
+                float m3[3];
+#pragma unroll
+                for (int a = 0; a < 3; a++)
+                  m3[a] = q[bi][a] + fo[0] * Ad[bi][a][0] + fo[1] * Ad[bi][a][1] +
+                          fo[2] * Ad[bi][a][2];
+                add.x += weight * m3[0];
+                add.y += weight * m3[1];
+                if (D == 3) add.z += weight * m3[2];
+                add.w += weight * mass[bi];
+              }
               const int node = D == 3 ? (lc[0] + ox) + kTile * ((lc[1] + oy) + kTile * (lc[2] + oz))
                                       : (lc[0] + ox) + kTile * (lc[1] + oy);
               float4 acc = tp[node];
-              acc.x += weight * (my[0] + (float)ox * Ad[0][0]);
-              acc.y += weight * (my[1] + (float)ox * Ad[1][0]);
-              if (D == 3) acc.z += weight * (my[2] + (float)ox * Ad[2][0]);
-              acc.w += weight * mass;
+              acc.x += add.x;
+              acc.y += add.y;
+              acc.z += add.z;
+              acc.w += add.w;
               tp[node] = acc;
               // Lanes of a wave update distinct nodes in one instruction, but a
               // later node of one lane can be an earlier node of another: keep
               // every LDS access in program order (LDS serves a wave in order).
               __asm__ volatile("" ::: "memory");
             }
-          }
-        }
       }
     }
     __syncthreads();
@@ -838,31 +943,27 @@ void launch_substep(const Params& P, SubstepBuffers& B, int s_index, int raw, hi
         P, B.pf, B.cap, n, B.total_blocks, B.keys, B.count, B.err);
   }
   const int nparts = div_up(B.total_blocks, kScanTile);
+  const int passes = (B.key_bits + 7) / 8;
+  int* ghist = B.ghist + (B.parity & 1) * kMaxRadixPasses * kRadixBins;
+  int* ghist_next = B.ghist + ((B.parity & 1) ^ 1) * kMaxRadixPasses * kRadixBins;
   {
     TimerScope ts(tm, "k_bscan", st);
-    k_bscan_reduce<D><<<nparts, kThreads, 0, st>>>(P, B.count, B.total_blocks, B.chunk, B.partials);
-    k_bscan_top<<<1, kThreads, 0, st>>>(B.partials, nparts, B.totals);
-    k_bscan_down<D><<<nparts, kThreads, 0, st>>>(P, B.count, B.count_next, B.total_blocks, B.chunk,
-                                                 B.partials, B.pstart, B.pchunk_begin,
-                                                 B.pchunk_cnt, B.chunks, B.active);
+    k_bscan<D><<<nparts, kThreads, 0, st>>>(P, B.count, B.count_next, B.total_blocks, B.chunk,
+                                           nparts, passes, B.pstart, B.pchunk_begin, B.pchunk_cnt,
+                                           B.chunks, B.active, B.totals, ghist, ghist_next,
+                                           B.lb_status_scan, B.lbs, B.err);
   }
   {
     TimerScope ts(tm, "k_radix", st);
     const int ntiles = div_up(n, kRadixTile);
-    const int hn = kRadixBins * ntiles;
-    const int hparts = div_up(hn, kScanTile);
-    const int passes = (B.key_bits + 7) / 8;
     const int* kin = B.keys;
     const int* vin = nullptr;
     int* kbuf[2] = {B.keys_alt, B.keys_alt + B.cap};
     int* vbuf[2] = {B.perm, B.perm_alt};
     for (int pass = 0; pass < passes; pass++) {
-      k_radix_hist<<<ntiles, kThreads, 0, st>>>(kin, n, pass * 8, ntiles, B.hist);
-      k_scan_reduce<<<hparts, kThreads, 0, st>>>(B.hist, hn, B.partials + nparts);
-      k_bscan_top<<<1, kThreads, 0, st>>>(B.partials + nparts, hparts, B.totals + kTotCount);
-      k_scan_down<<<hparts, kThreads, 0, st>>>(B.hist, hn, B.partials + nparts);
-      k_radix_scatter<<<ntiles, kThreads, 0, st>>>(kin, vin, kbuf[pass & 1], vbuf[pass & 1], n,
-                                                    pass * 8, ntiles, B.hist);
+      k_onesweep<<<ntiles, kThreads, 0, st>>>(kin, vin, kbuf[pass & 1], vbuf[pass & 1], n,
+                                               pass * 8, ntiles, ghist + pass * kRadixBins,
+                                               B.lb_status_radix, B.lbs, B.err);
       kin = kbuf[pass & 1];
       vin = vbuf[pass & 1];
     }

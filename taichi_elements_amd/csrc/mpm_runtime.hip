@@ -116,7 +116,7 @@ struct mpm_solver {
   int* keys_alt = nullptr;  // 2 * cap
   int* perm = nullptr;
   int* perm_alt = nullptr;
-  int* hist = nullptr;
+  unsigned long long* lb_status_radix = nullptr;
   float4* tiles = nullptr;
   int4* chunks = nullptr;
   int64_t chunk_cap = 0;
@@ -127,7 +127,9 @@ struct mpm_solver {
   int* pchunk_begin = nullptr;
   int* pchunk_cnt = nullptr;
   int* active = nullptr;
-  int4* partials = nullptr;
+  unsigned long long* lb_status_scan = nullptr;
+  LBState* lbs = nullptr;
+  int* ghist = nullptr;
   float* grid = nullptr;
   int* totals = nullptr;
   int* err = nullptr;
@@ -136,7 +138,8 @@ struct mpm_solver {
   int last_batch = 0;
 
   bool have_perm = false;  // perm of the previous substep matches the slots
-  int since_reorder = 0;
+  int since_reorder = 1 << 30;  // reorder at the first opportunity
+  int64_t substep_counter = 0;
   int reorder_interval = 16;
   int chunk = kChunkMax;
   int persistent_groups = 2048;
@@ -223,7 +226,7 @@ int free_particle_buffers(mpm_solver* s) {
   dfree(s->keys_alt);
   dfree(s->perm);
   dfree(s->perm_alt);
-  dfree(s->hist);
+  dfree(s->lb_status_radix);
   dfree(s->tiles);
   dfree(s->chunks);
   return MPM_OK;
@@ -258,16 +261,14 @@ int grow(mpm_solver* s, int64_t need) {
   TRY(dalloc(&s->perm, ncap));
   TRY(dalloc(&s->perm_alt, ncap));
   const int64_t ntiles = (ncap + kRadixTile - 1) / kRadixTile;
-  TRY(dalloc(&s->hist, kRadixBins * ntiles));
+  TRY(dalloc(&s->lb_status_radix, kRadixBins * ntiles));
+  HIP_TRY(hipMemsetAsync(s->lb_status_radix, 0, sizeof(unsigned long long) * kRadixBins * ntiles,
+                         s->stream));
   s->chunk_cap = std::min<int64_t>(s->total_blocks, ncap) + (ncap + s->chunk - 1) / s->chunk;
   const int TN = s->D == 3 ? Geo<3>::kTileNodes : Geo<2>::kTileNodes;
   TRY(dalloc(&s->tiles, s->chunk_cap * TN));
   TRY(dalloc(&s->chunks, s->chunk_cap));
-  // partials: block-window scan parts + radix-histogram scan parts
-  const int64_t nparts = (s->total_blocks + kScanTile - 1) / kScanTile;
-  const int64_t hparts = (kRadixBins * ntiles + kScanTile - 1) / kScanTile;
-  dfree(s->partials);
-  TRY(dalloc(&s->partials, nparts + hparts + 1));
+  HIP_TRY(hipStreamSynchronize(s->stream));
   s->have_perm = false;
   return MPM_OK;
 }
@@ -288,9 +289,11 @@ void fill_buffers(mpm_solver* s, SubstepBuffers& B) {
   B.pchunk_cnt = s->pchunk_cnt;
   B.chunks = s->chunks;
   B.active = s->active;
-  B.partials = s->partials;
+  B.lb_status_scan = s->lb_status_scan;
+  B.lb_status_radix = s->lb_status_radix;
+  B.lbs = s->lbs;
+  B.ghist = s->ghist;
   B.totals = s->totals;
-  B.hist = s->hist;
   B.tiles = s->tiles;
   B.grid = s->grid;
   B.err = s->err;
@@ -299,6 +302,7 @@ void fill_buffers(mpm_solver* s, SubstepBuffers& B) {
   B.key_bits = s->key_bits;
   B.chunk = s->chunk;
   B.persistent_groups = s->persistent_groups;
+  B.parity = (int)(s->substep_counter & 1);
 }
 
 void take_buffers(mpm_solver* s, const SubstepBuffers& B) {
@@ -348,7 +352,7 @@ int mpm_create(const mpm_config* cfg, mpm_solver** out) {
     }
   }
   int64_t tb = (int64_t)s->bn[0] * s->bn[1] * s->bn[2];
-  if (tb >= (1 << 28)) {
+  if (tb >= (1 << (8 * kMaxRadixPasses)) - 1) {
     delete s;
     return fail(MPM_EINVAL, "grid too large for the dense block window");
   }
@@ -361,7 +365,10 @@ int mpm_create(const mpm_config* cfg, mpm_solver** out) {
     return fail(MPM_EHIP, "cannot create a HIP stream");
   }
   const int NPB = s->D == 3 ? Geo<3>::kNodesPerBlock : Geo<2>::kNodesPerBlock;
-  if ((r = dalloc(&s->count, tb)) || (r = dalloc(&s->count_next, tb)) ||
+  const int64_t nparts = (tb + kScanTile - 1) / kScanTile;
+  if ((r = dalloc(&s->count, tb + 1)) || (r = dalloc(&s->count_next, tb + 1)) ||
+      (r = dalloc(&s->lb_status_scan, 4 * nparts)) || (r = dalloc(&s->lbs, 1)) ||
+      (r = dalloc(&s->ghist, 2 * kMaxRadixPasses * kRadixBins)) ||
       (r = dalloc(&s->pstart, tb)) || (r = dalloc(&s->pchunk_begin, tb)) ||
       (r = dalloc(&s->pchunk_cnt, tb)) || (r = dalloc(&s->active, tb)) ||
       (r = dalloc(&s->grid, tb * (s->D + 1) * NPB)) || (r = dalloc(&s->totals, 16)) ||
@@ -370,8 +377,13 @@ int mpm_create(const mpm_config* cfg, mpm_solver** out) {
     return r;
   }
   s->maxv_cap = 1024;
-  if (hipMemsetAsync(s->count, 0, sizeof(int) * tb, s->stream) != hipSuccess ||
-      hipMemsetAsync(s->count_next, 0, sizeof(int) * tb, s->stream) != hipSuccess ||
+  if (hipMemsetAsync(s->count, 0, sizeof(int) * (tb + 1), s->stream) != hipSuccess ||
+      hipMemsetAsync(s->count_next, 0, sizeof(int) * (tb + 1), s->stream) != hipSuccess ||
+      hipMemsetAsync(s->lb_status_scan, 0, sizeof(unsigned long long) * 4 * nparts, s->stream) !=
+          hipSuccess ||
+      hipMemsetAsync(s->lbs, 0, sizeof(LBState), s->stream) != hipSuccess ||
+      hipMemsetAsync(s->ghist, 0, sizeof(int) * 2 * kMaxRadixPasses * kRadixBins, s->stream) !=
+          hipSuccess ||
       hipMemsetAsync(s->grid, 0, sizeof(float) * tb * (s->D + 1) * NPB, s->stream) != hipSuccess ||
       hipMemsetAsync(s->totals, 0, sizeof(int) * 16, s->stream) != hipSuccess ||
       hipMemsetAsync(s->err, 0, sizeof(int), s->stream) != hipSuccess ||
@@ -394,7 +406,9 @@ int mpm_destroy(mpm_solver* s) {
   dfree(s->pchunk_begin);
   dfree(s->pchunk_cnt);
   dfree(s->active);
-  dfree(s->partials);
+  dfree(s->lb_status_scan);
+  dfree(s->lbs);
+  dfree(s->ghist);
   dfree(s->grid);
   dfree(s->totals);
   dfree(s->err);
@@ -497,6 +511,7 @@ int mpm_add_particles(mpm_solver* s, int64_t k, const float* x, const float* v,
   for (int64_t i = 0; i < k; i++) s->color.push_back(color ? color[i] : 0xFFFFFF);
   s->n += k;
   s->have_perm = false;
+  s->since_reorder = 1 << 30;  // the new particles sit at the end: reorder soon
   return MPM_OK;
 }
 
@@ -539,6 +554,7 @@ int mpm_substeps(mpm_solver* s, int32_t nsub, float dt, double t0) {
     else
       launch_substep<2>(s->P, B, k, s->debug & 1, s->stream, tm);
     take_buffers(s, B);
+    s->substep_counter++;
     s->have_perm = true;
     s->since_reorder++;
   }
@@ -738,6 +754,25 @@ int mpm_kernel_times(mpm_solver* s, char* names, int32_t names_cap, int32_t* cou
   if (names && names_cap > 0) {
     std::strncpy(names, all.c_str(), names_cap - 1);
     names[names_cap - 1] = 0;
+  }
+  return MPM_OK;
+}
+
+int mpm_host_svd(int32_t dim, const float* A, float* U, float* sig, float* V) {
+  if (!A || !U || !sig || !V || (dim != 2 && dim != 3)) return fail(MPM_EINVAL, "bad argument");
+  float a[3][3] = {{0}}, u[3][3], v[3][3], s3[3] = {0, 0, 0};
+  for (int i = 0; i < dim; i++)
+    for (int j = 0; j < dim; j++) a[i][j] = A[i * dim + j];
+  if (dim == 3)
+    svd3(a, u, s3, v);
+  else
+    svd2(a, u, s3, v);
+  for (int i = 0; i < dim; i++) {
+    sig[i] = s3[i];
+    for (int j = 0; j < dim; j++) {
+      U[i * dim + j] = u[i][j];
+      V[i * dim + j] = v[i][j];
+    }
   }
   return MPM_OK;
 }

@@ -80,3 +80,40 @@ def test_step_substep_count_rule():
             assert count in (substeps, substeps + 1)
             extra += count - substeps
     assert extra > 0  # the quirk is real for some of these frame times
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_engine_svd_on_host_against_lapack(dim):
+    """The kernels' SVD (replacement of ti.svd) evaluated on the host through
+    mpm_host_svd: reconstruction, orthogonality, det = +1, singular values
+    against numpy.linalg.svd and U V^T against scipy.linalg.polar."""
+    import numpy as np
+    import scipy.linalg
+    from taichi_elements_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libmpm_hip.so not built")
+    L = _lib.lib()
+    rng = np.random.default_rng(20 + dim)
+    for t in range(1500):
+        if t % 3 == 0:
+            A = np.eye(dim) + 0.05 * rng.normal(size=(dim, dim))
+        elif t % 3 == 1:
+            A = rng.normal(size=(dim, dim))
+        else:
+            A = np.diag(rng.uniform(0.5, 1.5, dim))
+        A = A.astype(np.float32)
+        U = np.zeros((dim, dim), np.float32)
+        V = np.zeros((dim, dim), np.float32)
+        s = np.zeros(dim, np.float32)
+        assert L.mpm_host_svd(dim, _lib.fptr(A), _lib.fptr(U), _lib.fptr(s), _lib.fptr(V)) == 0
+        A64, U, V, s = A.astype(float), U.astype(float), V.astype(float), s.astype(float)
+        scale = max(1.0, np.abs(A64).max())
+        assert np.abs((U * s) @ V.T - A64).max() <= 2e-5 * scale
+        assert np.abs(U.T @ U - np.eye(dim)).max() <= 2e-6
+        assert np.abs(V.T @ V - np.eye(dim)).max() <= 2e-6
+        assert abs(np.linalg.det(U) - 1) <= 1e-5 and abs(np.linalg.det(V) - 1) <= 1e-5
+        ref = np.linalg.svd(A64, compute_uv=False)
+        assert np.abs(np.abs(s) - ref).max() <= 2e-5 * ref[0]
+        if np.linalg.det(A64) > 1e-2:
+            R, _ = scipy.linalg.polar(A64)
+            assert np.abs(U @ V.T - R).max() <= 1e-4
