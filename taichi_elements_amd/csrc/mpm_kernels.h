@@ -61,9 +61,9 @@ struct SubstepBuffers {
   unsigned long long* lb_status_radix;  // look-back status of k_onesweep [256 * tiles]
   LBState* lbs;
   float4* tiles;    // p2g per-chunk node tiles (momentum xyz, mass)
-  float* grid;      // dense block-major grid [total_blocks][D+1][nodes]
+  float4* grid;     // dense block-major grid [total_blocks][nodes] of (v, mass)
   int* err;         // error bits
-  unsigned* maxv;   // per-substep max |v| (float bits)
+  unsigned* maxv;   // per-substep, per-g2p-workgroup max |v| (float bits)
   int total_blocks;
   int key_bits;
   int chunk;

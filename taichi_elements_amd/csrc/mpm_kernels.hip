@@ -668,7 +668,7 @@ __global__ __launch_bounds__(kThreads) void k_grid_op(Params P, const int* __res
                                                       const int* __restrict__ pchunk_begin,
                                                       const int* __restrict__ pchunk_cnt,
                                                       const float4* __restrict__ tiles,
-                                                      float* __restrict__ grid, int raw) {
+                                                      float4* __restrict__ grid, int raw) {
   constexpr int NPB = Geo<D>::kNodesPerBlock, TN = Geo<D>::kTileNodes, NC = D + 1;
   const int64_t total = (int64_t)totals[kTotActive] * NPB;
   for (int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x; t < total;
@@ -719,10 +719,7 @@ __global__ __launch_bounds__(kThreads) void k_grid_op(Params P, const int* __res
       for (int a = 0; a < D; a++) I[a] = (lb[a] + P.blo[a]) * kBlock + l[a];
       apply_post<D>(P, I, p);
     }
-    float* g = grid + (int64_t)bid * NC * NPB + node;
-#pragma unroll
-    for (int a = 0; a < D; a++) g[a * NPB] = p[a];
-    g[D * NPB] = m;
+    grid[(int64_t)bid * NPB + node] = make_float4(p[0], p[1], D == 3 ? p[2] : 0.0f, m);
   }
 }
 
@@ -735,11 +732,11 @@ __global__ __launch_bounds__(kThreads) void k_g2p(Params P, float* __restrict__ 
                                                   const int* __restrict__ perm,
                                                   const int4* __restrict__ chunks,
                                                   const int* __restrict__ totals,
-                                                  const float* __restrict__ grid,
+                                                  const float4* __restrict__ grid,
                                                   unsigned* __restrict__ maxv) {
   using L = Layout<D>;
   constexpr int NPB = Geo<D>::kNodesPerBlock, TN = Geo<D>::kTileNodes, NC = D + 1;
-  __shared__ float tile[D * TN];
+  __shared__ float4 tile[TN];
   __shared__ float red[kThreads / 64];
   float vmax = 0.0f;
   const int nchunks = totals[kTotChunks];
@@ -758,9 +755,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p(Params P, float* __restrict__ 
       }
       const int gb = encode_local<D>(q, P);
       const int node = D == 3 ? ln[0] + 4 * (ln[1] + 4 * ln[2]) : ln[0] + 4 * ln[1];
-      const float* g = grid + (int64_t)gb * NC * NPB + node;
-#pragma unroll
-      for (int a = 0; a < D; a++) tile[a * TN + i] = g[a * NPB];
+      tile[i] = grid[(int64_t)gb * NPB + node];
     }
     __syncthreads();
     int b[3];
@@ -781,33 +776,53 @@ __global__ __launch_bounds__(kThreads) void k_g2p(Params P, float* __restrict__ 
         l[a] = base - b[a];
         inside &= (l[a] >= 0) & (l[a] < kBlock);
       }
-      float nv[3] = {0.0f, 0.0f, 0.0f}, nC[3][3] = {{0.0f}};
+      // v = sum_o w_o g_o and C = 4 inv_dx sum_o w_o g_o (o - fx)^T
+      // (mpm_solver.py:714-721), evaluated through the tensor-product
+      // weights w_o = wx[ox] wy[oy] wz[oz]: with P_e = sum_o w_o g_o o_e,
+      // C = 4 inv_dx (P - v fx^T).  One pass over the 27 nodes, per-axis
+      // partial sums instead of a 3x3 outer product per node.
+      float nv[3] = {0.0f, 0.0f, 0.0f}, pe[3][3] = {{0.0f}}, nC[3][3];
       if (inside) {
 #pragma unroll
-        for (int oz = 0; oz < (D == 3 ? 3 : 1); oz++)
+        for (int oz = 0; oz < (D == 3 ? 3 : 1); oz++) {
+          float b0[3] = {0.0f, 0.0f, 0.0f}, bx[3] = {0.0f, 0.0f, 0.0f}, by[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
-          for (int oy = 0; oy < 3; oy++)
+          for (int oy = 0; oy < 3; oy++) {
+            float a0[3] = {0.0f, 0.0f, 0.0f}, ax[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
             for (int ox = 0; ox < 3; ox++) {
-              const int o[3] = {ox, oy, oz};
-              float weight = 1.0f, dpos[3];
-#pragma unroll
-              for (int a = 0; a < D; a++) {
-                weight *= w[a][o[a]];
-                dpos[a] = (float)o[a] - fx[a];
-              }
               const int node = D == 3 ? (l[0] + ox) + kTile * ((l[1] + oy) + kTile * (l[2] + oz))
                                       : (l[0] + ox) + kTile * (l[1] + oy);
-              const float cw = P.c4 * weight;
+              const float4 g4 = tile[node];
+              const float gg[3] = {g4.x, g4.y, g4.z};
 #pragma unroll
-              for (int a = 0; a < D; a++) {
-                const float gv = tile[a * TN + node];
-                nv[a] += weight * gv;
-#pragma unroll
-                for (int e = 0; e < D; e++) nC[a][e] += cw * (gv * dpos[e]);
+              for (int c = 0; c < D; c++) {
+                const float g = gg[c];
+                a0[c] += w[0][ox] * g;
+                if (ox) ax[c] += ((float)ox * w[0][ox]) * g;
               }
             }
+#pragma unroll
+            for (int c = 0; c < D; c++) {
+              b0[c] += w[1][oy] * a0[c];
+              bx[c] += w[1][oy] * ax[c];
+              if (oy) by[c] += ((float)oy * w[1][oy]) * a0[c];
+            }
+          }
+          const float wz = D == 3 ? w[2][oz] : 1.0f;
+#pragma unroll
+          for (int c = 0; c < D; c++) {
+            nv[c] += wz * b0[c];
+            pe[c][0] += wz * bx[c];
+            pe[c][1] += wz * by[c];
+            if (D == 3 && oz) pe[c][2] += ((float)oz * wz) * b0[c];
+          }
+        }
       }
+#pragma unroll
+      for (int c = 0; c < D; c++)
+#pragma unroll
+        for (int e = 0; e < D; e++) nC[c][e] = P.c4 * (pe[c][e] - nv[c] * fx[e]);
       if (mat != kStationary) {  // mpm_solver.py:722-724
 #pragma unroll
         for (int a = 0; a < D; a++) {
@@ -834,7 +849,9 @@ __global__ __launch_bounds__(kThreads) void k_g2p(Params P, float* __restrict__ 
     float m = red[0];
 #pragma unroll
     for (int w = 1; w < kThreads / 64; w++) m = fmaxf(m, red[w]);
-    atomicMax(maxv, __float_as_uint(m));
+    // one partial maximum per workgroup, reduced by the reader: a single
+    // device-wide atomic serialised thousands of workgroups
+    maxv[blockIdx.x] = __float_as_uint(m);
   }
 }
 
@@ -978,13 +995,14 @@ void launch_substep(const Params& P, SubstepBuffers& B, int s_index, int raw, hi
   {
     TimerScope ts(tm, "k_grid_op", st);
     const int64_t node_bound = (int64_t)B.total_blocks * Geo<D>::kNodesPerBlock;
-    k_grid_op<D><<<clamp_groups(div_up(node_bound, kThreads), 4 * G), kThreads, 0, st>>>(
+    k_grid_op<D><<<clamp_groups(div_up(node_bound, kThreads), G / 2), kThreads, 0, st>>>(
         P, B.active, B.totals, B.pchunk_begin, B.pchunk_cnt, B.tiles, B.grid, raw);
   }
   {
     TimerScope ts(tm, "k_g2p", st);
     k_g2p<D><<<clamp_groups(chunk_bound, G), kThreads, 0, st>>>(
-        P, B.pf, B.pi, B.cap, B.perm, B.chunks, B.totals, B.grid, B.maxv + s_index);
+        P, B.pf, B.pi, B.cap, B.perm, B.chunks, B.totals, B.grid,
+        B.maxv + (int64_t)s_index * B.persistent_groups);
   }
   std::swap(B.count, B.count_next);
   (void)L::NF;

@@ -130,7 +130,7 @@ struct mpm_solver {
   unsigned long long* lb_status_scan = nullptr;
   LBState* lbs = nullptr;
   int* ghist = nullptr;
-  float* grid = nullptr;
+  float4* grid = nullptr;
   int* totals = nullptr;
   int* err = nullptr;
   unsigned* maxv = nullptr;
@@ -371,12 +371,12 @@ int mpm_create(const mpm_config* cfg, mpm_solver** out) {
       (r = dalloc(&s->ghist, 2 * kMaxRadixPasses * kRadixBins)) ||
       (r = dalloc(&s->pstart, tb)) || (r = dalloc(&s->pchunk_begin, tb)) ||
       (r = dalloc(&s->pchunk_cnt, tb)) || (r = dalloc(&s->active, tb)) ||
-      (r = dalloc(&s->grid, tb * (s->D + 1) * NPB)) || (r = dalloc(&s->totals, 16)) ||
-      (r = dalloc(&s->err, 1)) || (r = dalloc(&s->maxv, 1024))) {
+      (r = dalloc(&s->grid, tb * NPB)) || (r = dalloc(&s->totals, 16)) ||
+      (r = dalloc(&s->err, 1)) || (r = dalloc(&s->maxv, 16 * s->persistent_groups))) {
     mpm_destroy(s);
     return r;
   }
-  s->maxv_cap = 1024;
+  s->maxv_cap = 16 * s->persistent_groups;
   if (hipMemsetAsync(s->count, 0, sizeof(int) * (tb + 1), s->stream) != hipSuccess ||
       hipMemsetAsync(s->count_next, 0, sizeof(int) * (tb + 1), s->stream) != hipSuccess ||
       hipMemsetAsync(s->lb_status_scan, 0, sizeof(unsigned long long) * 4 * nparts, s->stream) !=
@@ -384,7 +384,7 @@ int mpm_create(const mpm_config* cfg, mpm_solver** out) {
       hipMemsetAsync(s->lbs, 0, sizeof(LBState), s->stream) != hipSuccess ||
       hipMemsetAsync(s->ghist, 0, sizeof(int) * 2 * kMaxRadixPasses * kRadixBins, s->stream) !=
           hipSuccess ||
-      hipMemsetAsync(s->grid, 0, sizeof(float) * tb * (s->D + 1) * NPB, s->stream) != hipSuccess ||
+      hipMemsetAsync(s->grid, 0, sizeof(float4) * tb * NPB, s->stream) != hipSuccess ||
       hipMemsetAsync(s->totals, 0, sizeof(int) * 16, s->stream) != hipSuccess ||
       hipMemsetAsync(s->err, 0, sizeof(int), s->stream) != hipSuccess ||
       hipStreamSynchronize(s->stream) != hipSuccess) {
@@ -526,12 +526,13 @@ int mpm_substeps(mpm_solver* s, int32_t nsub, float dt, double t0) {
   if (!s) return fail(MPM_EINVAL, "null argument");
   if (nsub < 0) return fail(MPM_EINVAL, "negative substep count");
   HIP_TRY(hipSetDevice(s->device));
-  if (nsub > s->maxv_cap) {
+  if ((int64_t)nsub * s->persistent_groups > s->maxv_cap) {
+    HIP_TRY(hipStreamSynchronize(s->stream));
     dfree(s->maxv);
-    TRY(dalloc(&s->maxv, nsub));
-    s->maxv_cap = nsub;
+    TRY(dalloc(&s->maxv, (int64_t)nsub * s->persistent_groups));
+    s->maxv_cap = nsub * s->persistent_groups;
   }
-  launch_fill_u32(s->maxv, 0u, std::max(nsub, 1), s->stream);
+  launch_fill_u32(s->maxv, 0u, (int64_t)std::max(nsub, 1) * s->persistent_groups, s->stream);
   s->last_batch = nsub;
   if (s->n == 0) return MPM_OK;
   fill_params(s, dt);
@@ -566,7 +567,16 @@ int mpm_read_max_velocity(mpm_solver* s, int32_t nsub, float* out) {
   if (!s || (!out && nsub > 0)) return fail(MPM_EINVAL, "null argument");
   if (nsub > s->last_batch) return fail(MPM_EINVAL, "more entries than substeps run");
   int r = check_device_errors(s);
-  if (nsub > 0) HIP_TRY(hipMemcpy(out, s->maxv, sizeof(float) * nsub, hipMemcpyDeviceToHost));
+  if (nsub > 0) {
+    const int G = s->persistent_groups;
+    std::vector<float> part((size_t)nsub * G);
+    HIP_TRY(hipMemcpy(part.data(), s->maxv, sizeof(float) * part.size(), hipMemcpyDeviceToHost));
+    for (int k = 0; k < nsub; k++) {
+      float m = 0.0f;
+      for (int g = 0; g < G; g++) m = std::max(m, part[(size_t)k * G + g]);
+      out[k] = m;
+    }
+  }
   return r;
 }
 
@@ -663,7 +673,7 @@ int mpm_read_grid(mpm_solver* s, const int32_t* lo, const int32_t* nn, float* m,
   const int D = s->D;
   const int NPB = D == 3 ? 64 : 16;
   const int64_t tb = s->total_blocks;
-  std::vector<float> g((size_t)tb * (D + 1) * NPB);
+  std::vector<float> g((size_t)tb * 4 * NPB);
   HIP_TRY(hipMemcpy(g.data(), s->grid, sizeof(float) * g.size(), hipMemcpyDeviceToHost));
   // only blocks active in the last substep hold this substep's values
   int nact = 0;
@@ -695,9 +705,9 @@ int mpm_read_grid(mpm_solver* s, const int32_t* lo, const int32_t* nn, float* m,
                                      : (int64_t)lb[1] * s->bn[0] + lb[0];
           if (is_act[bid]) {
             const int node = D == 3 ? l[0] + 4 * (l[1] + 4 * l[2]) : l[0] + 4 * l[1];
-            const float* gp = &g[(size_t)bid * (D + 1) * NPB + node];
-            for (int a = 0; a < D; a++) vv[a] = gp[a * NPB];
-            mv = gp[D * NPB];
+            const float* gp = &g[((size_t)bid * NPB + node) * 4];
+            for (int a = 0; a < D; a++) vv[a] = gp[a];
+            mv = gp[3];
           }
         }
         if (m) m[o] = mv;
