@@ -48,6 +48,34 @@ def make_scene(seed=1):
     return np.concatenate(xs), np.concatenate(vs), np.concatenate(ms)
 
 
+def make_sparse_scene(seed=2, total=16_000_000, res=512):
+    """64 ellipsoids, centres U[0.1, 0.9]^3, semi-axes U[0.02, 0.08]^3, random
+    material in {water, elastic, snow, sand}; per-ellipsoid counts from
+    add_ellipsoid's rule at 8 particles per cell (mpm_solver.py:997-1005),
+    scaled so that the total is exactly `total`; uniform interior sampling by
+    rejection in the unit ball (mpm_solver.py:959-967); zero velocity."""
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform(0.1, 0.9, (64, 3))
+    radii = rng.uniform(0.02, 0.08, (64, 3))
+    mats = rng.integers(0, 4, 64)
+    counts = np.ceil(4 / 3 * np.pi * np.prod(radii * res, axis=1) * 8)
+    counts = np.floor(counts * total / counts.sum()).astype(np.int64)
+    counts[-1] += total - counts.sum()
+    x = np.empty((total, 3), np.float32)
+    m = np.empty(total, np.int32)
+    at = 0
+    for c, r, mat, k in zip(centres, radii, mats, counts):
+        got = 0
+        while got < k:
+            cand = rng.random((int((k - got) * 2.0) + 64, 3), dtype=np.float32) * 2 - 1
+            cand = cand[(cand * cand).sum(1) <= 1][:k - got]
+            x[at + got:at + got + len(cand)] = c.astype(np.float32) + cand * r.astype(np.float32)
+            got += len(cand)
+        m[at:at + k] = mat
+        at += k
+    return x, np.zeros_like(x), m
+
+
 def algorithmic_bytes(n_particles, n_nodes):
     """Bytes a substep must move in the reference's field layout (float32):
     per particle 16 (binning) + 148 (p2g) + 80 (g2p), per active grid node
@@ -109,6 +137,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--reorder-interval", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--config", choices=["dense1m", "sparse16m"], default="dense1m",
+                    help="dense1m: the 3D four-material 128^3 workload (default, the "
+                         "headline line); sparse16m: 512^3 window, 16M particles in 64 "
+                         "ellipsoids with a plane collider")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device-to-device copy (hipMemcpy) so that "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE can be scaled by a kernel of "
@@ -122,9 +154,16 @@ def main():
 
     from taichi_elements_amd import MPMSolver, mpm_solver
     mpm_solver.PRINT_PROGRESS = False
-    x, v, m = make_scene()
+    res = RES
+    if args.config == "sparse16m":
+        res = 512
+        x, v, m = make_sparse_scene()
+    else:
+        x, v, m = make_scene()
     n = len(x)
-    s = MPMSolver(res=(RES, RES, RES), reorder_interval=args.reorder_interval, chunk=args.chunk)
+    s = MPMSolver(res=(res, res, res), reorder_interval=args.reorder_interval, chunk=args.chunk)
+    if args.config == "sparse16m":
+        s.add_surface_collider((0.0, 0.05, 0.0), (0.0, 1.0, 0.0), surface=s.surface_slip)
     s.read_restart(n, x, v, m, np.zeros(n, np.int32))
     dt = s.default_dt
     L = __import__("taichi_elements_amd._lib", fromlist=["lib"]).lib()
@@ -172,10 +211,16 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded numpy: 4 cubes x 250k particles, v ~ U[-1,1]^3)",
-        "config": {"workload": "3D four-material plain substep, 128^3 dense grid, dx=1/128, "
-                               "1M particles (water/elastic/snow/sand cubes of side 0.2), "
-                               "dt=default_dt", "res": RES, "particles": n,
+        "data": ("synthetic (seeded numpy: 4 cubes x 250k particles, v ~ U[-1,1]^3)"
+                 if args.config == "dense1m" else
+                 "synthetic (seeded numpy: 64 ellipsoids, 16M particles, at rest)"),
+        "config": {"workload": ("3D four-material plain substep, 128^3 dense grid, dx=1/128, "
+                                "1M particles (water/elastic/snow/sand cubes of side 0.2), "
+                                "dt=default_dt") if args.config == "dense1m" else
+                               ("3D sparse large domain, 512^3 window of 4^3 blocks, 16M "
+                                "particles in 64 random ellipsoids (8 per cell), plane "
+                                "collider, dt=default_dt"),
+                   "res": res, "particles": n,
                    "active_grid_blocks": counts["active_blocks"],
                    "particle_blocks": counts["particle_blocks"], "chunks": counts["chunks"]},
         "roofline": {"bound": "hbm", "kernel": "k_p2g", "achieved": achieved,
@@ -191,7 +236,7 @@ def main():
     }
     if args.copy_calibration:
         out["copy_calibration_bytes"] = copy_calibration()
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.config == "dense1m":
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(x, v, m, dt, args.cpu_seconds)
     else:
