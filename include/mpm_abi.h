@@ -140,10 +140,12 @@ int mpm_set_state(mpm_solver* s, int32_t field, const void* host);
  * m (prod n) and v (prod n * dim); nodes outside the active blocks read 0. */
 int mpm_read_grid(mpm_solver* s, const int32_t* lo, const int32_t* n, float* m, float* v);
 
-/* Binning of the last substep (build_pid, mpm_solver.py:344-361): keys[i] is
- * the block key of the particle in physical slot i, perm the stable
- * order of slots by key, slot_to_insertion[i] the insertion index of the
- * particle in slot i.  Arrays hold mpm_num_particles entries. */
+/* Binning of the last substep (build_pid, mpm_solver.py:344-361): perm is the
+ * order of the physical slots sorted by block key -- a stable sort of the
+ * previous substep's order (of the slots themselves on the first substep and
+ * after positions are set), keys[k] the block key of slot perm[k], and
+ * slot_to_insertion[i] the insertion index of the particle in slot i.
+ * Arrays hold mpm_num_particles entries. */
 int mpm_read_binning(mpm_solver* s, int32_t* keys, int32_t* perm, int32_t* slot_to_insertion);
 
 /* Debug switches: bit 0 = grid operator writes raw p2g output (mass and

@@ -69,6 +69,21 @@ struct SubstepBuffers {
   int chunk;
   int persistent_groups;
   int parity;  // substep counter parity (selects the digit-histogram copy)
+  // neighbour-bounded sort: the previous substep's chunks and block tables,
+  // direction codes / new keys per previous sorted position, per-chunk
+  // direction counts and output offsets
+  int4* chunks_prev;
+  int* pchunk_begin_prev;
+  int* pchunk_cnt_prev;
+  int* totals_prev;
+  int* code;
+  int* nkey;
+  int* dcount;      // [chunk_cap][27]
+  int* doff;        // [chunk_cap][27]
+  int4* tile_prefix;  // two-level block scan
+  bool full_sort;     // radix-sort the slots (first substep / positions set by the host)
+  bool perm_identity; // the slots were just reordered into the previous order
+  int* keys_sorted;   // output: block keys in sorted order
 };
 
 class Timer;

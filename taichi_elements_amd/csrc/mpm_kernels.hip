@@ -227,12 +227,22 @@ __global__ __launch_bounds__(kThreads) void k_bscan(
     int chunk, int ntiles, int passes, int* __restrict__ pstart, int* __restrict__ pchunk_begin,
     int* __restrict__ pchunk_cnt, int4* __restrict__ chunks, int* __restrict__ active,
     int* __restrict__ totals, int* __restrict__ ghist, int* __restrict__ ghist_next,
-    unsigned long long* __restrict__ status, LBState* __restrict__ lbs, int* __restrict__ err) {
+    unsigned long long* __restrict__ status, LBState* __restrict__ lbs,
+    const int4* __restrict__ tile_prefix, int* __restrict__ err) {
   __shared__ LBTicket tk_sh;
   __shared__ int4 wsum[kThreads / 64];
   __shared__ unsigned excl_sh[4];
   __shared__ int lh[kMaxRadixPasses][kRadixBins];
-  const LBTicket tk = lb_acquire(lbs, &tk_sh);
+  // tile_prefix != nullptr: exclusive tile prefixes were computed by
+  // k_bscan_reduce + k_bscan_top (large windows, where a look-back chain over
+  // thousands of tiles is slower than two extra launches)
+  LBTicket tk;
+  if (tile_prefix) {
+    tk.tile = blockIdx.x;
+    tk.epoch = 0;
+  } else {
+    tk = lb_acquire(lbs, &tk_sh);
+  }
   const int tile = tk.tile;
   for (int i = threadIdx.x; i < kMaxRadixPasses * kRadixBins; i += kThreads) (&lh[0][0])[i] = 0;
 
@@ -262,7 +272,11 @@ __global__ __launch_bounds__(kThreads) void k_bscan(
         atomicAdd(&lh[ps][(total_blocks >> (8 * ps)) & 255], inval);
   }
 
-  if (threadIdx.x < 4) {
+  if (threadIdx.x < 4 && tile_prefix) {
+    const int4 tp = tile_prefix[tile];
+    const int c = threadIdx.x;
+    excl_sh[c] = (unsigned)(c == 0 ? tp.x : c == 1 ? tp.y : c == 2 ? tp.z : tp.w);
+  } else if (threadIdx.x < 4) {
     const int c = threadIdx.x;
     const unsigned agg = (unsigned)(c == 0 ? tot.x : c == 1 ? tot.y : c == 2 ? tot.z : tot.w);
     unsigned excl = 0;
@@ -309,7 +323,37 @@ __global__ __launch_bounds__(kThreads) void k_bscan(
   if (tile == 0)
     for (int i = threadIdx.x; i < kMaxRadixPasses * kRadixBins; i += kThreads) ghist_next[i] = 0;
   __syncthreads();
-  lb_release(lbs, ntiles, tk.epoch);
+  if (!tile_prefix) lb_release(lbs, ntiles, tk.epoch);
+}
+
+// Two-level mode of the block-window scan: per-tile sums, then one workgroup
+// turns them into exclusive tile prefixes.
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_bscan_reduce(Params P, const int* __restrict__ count,
+                                                           int total_blocks, int chunk,
+                                                           int4* __restrict__ partials) {
+  __shared__ int4 wsum[kThreads / 64];
+  int4 acc = make_int4(0, 0, 0, 0);
+  const int first = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++)
+    if (first + k < total_blocks) acc = add4(acc, block_values<D>(first + k, count, P, chunk));
+  int4 tot;
+  block_exscan4(acc, &tot, wsum);
+  if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kThreads) void k_bscan_top(int4* __restrict__ partials, int nparts) {
+  __shared__ int4 wsum[kThreads / 64];
+  int4 carry = make_int4(0, 0, 0, 0);
+  for (int base = 0; base < nparts; base += kThreads) {
+    const int i = base + threadIdx.x;
+    const int4 v = i < nparts ? partials[i] : make_int4(0, 0, 0, 0);
+    int4 tot;
+    const int4 ex = block_exscan4(v, &tot, wsum);
+    if (i < nparts) partials[i] = add4(ex, carry);
+    carry = add4(carry, tot);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -733,17 +777,25 @@ __global__ __launch_bounds__(kThreads) void k_g2p(Params P, float* __restrict__ 
                                                   const int4* __restrict__ chunks,
                                                   const int* __restrict__ totals,
                                                   const float4* __restrict__ grid,
-                                                  unsigned* __restrict__ maxv) {
+                                                  unsigned* __restrict__ maxv,
+                                                  int* __restrict__ code, int* __restrict__ nkey,
+                                                  int* __restrict__ dcount,
+                                                  int* __restrict__ count_next,
+                                                  int* __restrict__ err) {
   using L = Layout<D>;
   constexpr int NPB = Geo<D>::kNodesPerBlock, TN = Geo<D>::kTileNodes, NC = D + 1;
   __shared__ float4 tile[TN];
   __shared__ float red[kThreads / 64];
+  __shared__ int dwcnt[kThreads / 64][32];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   float vmax = 0.0f;
   const int nchunks = totals[kTotChunks];
   for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
     const int4 ck = chunks[c];
     int lb[3];
     decode_local<D>(ck.x, P, lb);
+    for (int i = threadIdx.x; i < (kThreads / 64) * 32; i += kThreads) (&dwcnt[0][0])[i] = 0;
     for (int i = threadIdx.x; i < TN; i += kThreads) {
       const int tl[3] = {i % kTile, (i / kTile) % kTile, i / (kTile * kTile)};
       int q[3], ln[3];
@@ -761,7 +813,12 @@ __global__ __launch_bounds__(kThreads) void k_g2p(Params P, float* __restrict__ 
     int b[3];
 #pragma unroll
     for (int a = 0; a < D; a++) b[a] = (lb[a] + P.blo[a]) * kBlock;
-    for (int j = ck.y + threadIdx.x; j < ck.z; j += kThreads) {
+    // one particle per thread (chunks hold at most kThreads particles)
+    const int j = ck.y + threadIdx.x;
+    const bool live = j < ck.z;
+    int dir = 13;  // block offset of the particle after advection, (dz+1)*9 + (dy+1)*3 + (dx+1)
+    int key_new = ck.x;
+    if (live) {
       const int p = perm[j];
       const int mat = pi[(int64_t)L::MAT * cap + p];
       float xp[3], fx[3], w[3][3];
@@ -823,11 +880,13 @@ __global__ __launch_bounds__(kThreads) void k_g2p(Params P, float* __restrict__ 
       for (int c = 0; c < D; c++)
 #pragma unroll
         for (int e = 0; e < D; e++) nC[c][e] = P.c4 * (pe[c][e] - nv[c] * fx[e]);
+      float xn[3] = {xp[0], xp[1], xp[2]};
       if (mat != kStationary) {  // mpm_solver.py:722-724
 #pragma unroll
         for (int a = 0; a < D; a++) {
+          xn[a] = xp[a] + P.dt * nv[a];
           pf[(int64_t)(L::V + a) * cap + p] = nv[a];
-          pf[(int64_t)(L::X + a) * cap + p] = xp[a] + P.dt * nv[a];
+          pf[(int64_t)(L::X + a) * cap + p] = xn[a];
           vmax = fmaxf(vmax, fabsf(nv[a]));
 #pragma unroll
           for (int e = 0; e < D; e++) pf[(int64_t)(L::C + a * D + e) * cap + p] = nC[a][e];
@@ -836,6 +895,63 @@ __global__ __launch_bounds__(kThreads) void k_g2p(Params P, float* __restrict__ 
 #pragma unroll
         for (int a = 0; a < D; a++) vmax = fmaxf(vmax, fabsf(pf[(int64_t)(L::V + a) * cap + p]));
       }
+      // block of the new position for the next substep's binning
+      // (build_pid, mpm_solver.py:357-361); particles move less than one block
+      // per substep under any sane CFL number
+      int dd[3] = {0, 0, 0}, q[3] = {0, 0, 0};
+      bool near = true;
+#pragma unroll
+      for (int a = 0; a < D; a++) {
+        int base;
+        float f;
+        base_fx(xn[a], P.inv_dx, base, f);
+        q[a] = (base >> 2) - P.blo[a];
+        dd[a] = q[a] - lb[a];
+        near &= (dd[a] >= -1) & (dd[a] <= 1) & (q[a] >= 0) & (q[a] < P.bn[a] - 1);
+      }
+      if (near) {
+        dir = (dd[2] + 1) * 9 + (dd[1] + 1) * 3 + (dd[0] + 1);
+        key_new = encode_local<D>(q, P);
+      } else {
+        // left the window (bit 0) or moved more than one block (bit 3): the
+        // particle stays binned in its old block and the caller is told
+        atomicOr(err, (q[0] < 0 || q[0] >= P.bn[0] - 1 || q[1] < 0 || q[1] >= P.bn[1] - 1 ||
+                       (D == 3 && (q[2] < 0 || q[2] >= P.bn[2] - 1)))
+                          ? 1
+                          : 8);
+      }
+    }
+    // stable rank of the particle among the chunk's particles with the same
+    // direction; per-chunk direction counts; block counts of the next substep
+    unsigned long long peers = __ballot(live);
+#pragma unroll
+    for (int bit = 0; bit < 5; bit++) {
+      const bool on = (dir >> bit) & 1;
+      const unsigned long long bal = __ballot(on);
+      peers &= on ? bal : ~bal;
+    }
+    const int rank = (int)__popcll(peers & lt);
+    if (live && rank == 0) dwcnt[wave][dir] = (int)__popcll(peers);
+    __syncthreads();
+    if (threadIdx.x < 27) {
+      const int d = threadIdx.x;
+      int run = 0;
+#pragma unroll
+      for (int w = 0; w < kThreads / 64; w++) {
+        const int cnt = dwcnt[w][d];
+        dwcnt[w][d] = run;
+        run += cnt;
+      }
+      dcount[(int64_t)c * 27 + d] = run;
+      if (run > 0) {
+        int q[3] = {lb[0] + d % 3 - 1, lb[1] + (d / 3) % 3 - 1, lb[2] + d / 9 - 1};
+        atomicAdd(&count_next[encode_local<D>(q, P)], run);
+      }
+    }
+    __syncthreads();
+    if (live) {
+      code[j] = dir | ((dwcnt[wave][dir] + rank) << 8);
+      nkey[j] = key_new;
     }
     __syncthreads();
   }
@@ -852,6 +968,96 @@ __global__ __launch_bounds__(kThreads) void k_g2p(Params P, float* __restrict__ 
     // one partial maximum per workgroup, reduced by the reader: a single
     // device-wide atomic serialised thousands of workgroups
     maxv[blockIdx.x] = __float_as_uint(m);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// neighbour-bounded counting sort (steady-state binning)
+//
+// The order of a substep is the previous order stably sorted by the new
+// block key.  Particles move less than one block per substep, so the members
+// of new block k come from the 3^d old blocks k - d; in the previous order
+// they are grouped by old block (ascending key), then by old chunk, then by
+// position.  k_g2p already gave every particle its direction d and its
+// stable rank among its old chunk's particles with the same d; k_offsets
+// walks each new block's 3^d old neighbours in ascending key order and hands
+// every (old chunk, d) pair its first output position; k_nscatter moves the
+// particle indices.
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_offsets(
+    Params P, const int4* __restrict__ chunks, const int* __restrict__ totals,
+    const int* __restrict__ pstart, const int* __restrict__ count,
+    const int* __restrict__ pchunk_begin_prev, const int* __restrict__ pchunk_cnt_prev,
+    const int* __restrict__ dcount, int* __restrict__ doff, int* __restrict__ err) {
+  // one wave per new chunk; the wave of a block's first chunk does the block:
+  // lane t < 3^d takes old neighbour t in ascending key order, sums its
+  // chunks' counts toward this block, and a wave scan orders the neighbours
+  const int nch = totals[kTotChunks];
+  const int lane = threadIdx.x & 63;
+  const int waves = gridDim.x * (kThreads / 64);
+  for (int c = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6); c < nch; c += waves) {
+    const int4 ck = chunks[c];
+    const int bid = ck.x;
+    if (ck.y != pstart[bid]) continue;  // not the first chunk of its block (wave-uniform)
+    int lb[3];
+    decode_local<D>(bid, P, lb);
+    constexpr int NB = D == 3 ? 27 : 9;
+    int c0 = 0, cn = 0, dir = 0, mine = 0;
+    if (lane < NB) {
+      // ascending old key <=> descending offset (dz, dy, dx) in lexicographic order
+      const int dx = 1 - lane % 3, dy = 1 - (lane / 3) % 3, dz = D == 3 ? 1 - lane / 9 : 0;
+      const int q[3] = {lb[0] - dx, lb[1] - dy, lb[2] - dz};
+      if (q[0] >= 0 && q[0] < P.bn[0] && q[1] >= 0 && q[1] < P.bn[1] &&
+          (D == 2 || (q[2] >= 0 && q[2] < P.bn[2]))) {
+        const int ob = encode_local<D>(q, P);
+        c0 = pchunk_begin_prev[ob];
+        cn = pchunk_cnt_prev[ob];
+        dir = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+        for (int oc = c0; oc < c0 + cn; oc++) mine += dcount[(int64_t)oc * 27 + dir];
+      }
+    }
+    int inc = mine;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int y = __shfl_up(inc, off);
+      if (lane >= off) inc += y;
+    }
+    int run = ck.y + inc - mine;
+    for (int oc = c0; oc < c0 + cn; oc++) {
+      doff[(int64_t)oc * 27 + dir] = run;
+      run += dcount[(int64_t)oc * 27 + dir];
+    }
+    const int total = __shfl(inc, 63);
+    if (lane == 0 && total != count[bid]) atomicOr(err, 16);  // counts disagree (cannot happen)
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_nscatter(
+    const int4* __restrict__ chunks_prev, const int* __restrict__ totals_prev,
+    const int* __restrict__ code, const int* __restrict__ nkey, const int* __restrict__ doff,
+    const int* __restrict__ perm_old, int* __restrict__ perm_new, int* __restrict__ keys_new,
+    int n, int total_blocks, int* __restrict__ err) {
+  const int nch = totals_prev[kTotChunks];
+  for (int c = blockIdx.x; c < nch; c += gridDim.x) {
+    const int4 ck = chunks_prev[c];
+    for (int j = ck.y + threadIdx.x; j < ck.z; j += kThreads) {
+      const int cd = code[j];
+      const int pos = doff[(int64_t)c * 27 + (cd & 255)] + (cd >> 8);
+      if (pos >= 0 && pos < n) {
+        perm_new[pos] = perm_old ? perm_old[j] : j;
+        keys_new[pos] = nkey[j];
+      } else {
+        atomicOr(err, 16);
+      }
+    }
+  }
+  // particles outside the window (frozen since an earlier full sort) keep
+  // their places at the end
+  for (int j = totals_prev[kTotValid] + blockIdx.x * kThreads + threadIdx.x; j < n;
+       j += gridDim.x * kThreads) {
+    perm_new[j] = perm_old ? perm_old[j] : j;
+    keys_new[j] = total_blocks;
   }
 }
 
@@ -950,27 +1156,36 @@ struct TimerScope {
 template <int D>
 void launch_substep(const Params& P, SubstepBuffers& B, int s_index, int raw, hipStream_t st,
                     Timer* tm) {
-  using L = Layout<D>;
   const int n = B.n;
   const int G = B.persistent_groups;
   if (n == 0) return;
-  {
-    TimerScope ts(tm, "k_keys", st);
-    k_keys<D><<<clamp_groups(div_up(n, kThreads), 8192), kThreads, 0, st>>>(
-        P, B.pf, B.cap, n, B.total_blocks, B.keys, B.count, B.err);
-  }
   const int nparts = div_up(B.total_blocks, kScanTile);
   const int passes = (B.key_bits + 7) / 8;
   int* ghist = B.ghist + (B.parity & 1) * kMaxRadixPasses * kRadixBins;
   int* ghist_next = B.ghist + ((B.parity & 1) ^ 1) * kMaxRadixPasses * kRadixBins;
-  {
-    TimerScope ts(tm, "k_bscan", st);
-    k_bscan<D><<<nparts, kThreads, 0, st>>>(P, B.count, B.count_next, B.total_blocks, B.chunk,
-                                           nparts, passes, B.pstart, B.pchunk_begin, B.pchunk_cnt,
-                                           B.chunks, B.active, B.totals, ghist, ghist_next,
-                                           B.lb_status_scan, B.lbs, B.err);
+  const int chunk_bound = std::min(B.total_blocks, n) + div_up(n, B.chunk);
+  if (B.full_sort) {
+    // first substep (or positions changed from the host): block counts from
+    // scratch, then a stable radix sort of the slots
+    TimerScope ts(tm, "k_keys", st);
+    launch_fill_u32((unsigned*)B.count, 0u, B.total_blocks + 1, st);
+    k_keys<D><<<clamp_groups(div_up(n, kThreads), 8192), kThreads, 0, st>>>(
+        P, B.pf, B.cap, n, B.total_blocks, B.keys, B.count, B.err);
   }
   {
+    TimerScope ts(tm, "k_bscan", st);
+    const bool two_level = nparts > 256;
+    if (two_level) {
+      k_bscan_reduce<D><<<nparts, kThreads, 0, st>>>(P, B.count, B.total_blocks, B.chunk,
+                                                     B.tile_prefix);
+      k_bscan_top<<<1, kThreads, 0, st>>>(B.tile_prefix, nparts);
+    }
+    k_bscan<D><<<nparts, kThreads, 0, st>>>(
+        P, B.count, B.count_next, B.total_blocks, B.chunk, nparts, B.full_sort ? passes : 0,
+        B.pstart, B.pchunk_begin, B.pchunk_cnt, B.chunks, B.active, B.totals, ghist, ghist_next,
+        B.lb_status_scan, B.lbs, two_level ? B.tile_prefix : nullptr, B.err);
+  }
+  if (B.full_sort) {
     TimerScope ts(tm, "k_radix", st);
     const int ntiles = div_up(n, kRadixTile);
     const int* kin = B.keys;
@@ -985,8 +1200,21 @@ void launch_substep(const Params& P, SubstepBuffers& B, int s_index, int raw, hi
       vin = vbuf[pass & 1];
     }
     if (vin != B.perm) std::swap(B.perm, B.perm_alt);
+    B.keys_sorted = (int*)kin;
+  } else {
+    TimerScope ts(tm, "k_nsort", st);
+    k_offsets<D><<<clamp_groups(div_up(chunk_bound, kThreads / 64), G), kThreads, 0, st>>>(
+        P, B.chunks, B.totals, B.pstart, B.count, B.pchunk_begin_prev, B.pchunk_cnt_prev,
+        B.dcount, B.doff, B.err);
+    // the previous order: the last substep's permutation, or the slots
+    // themselves right after a physical reorder
+    const int* perm_old = B.perm_identity ? nullptr : B.perm;
+    k_nscatter<<<clamp_groups(chunk_bound, G), kThreads, 0, st>>>(
+        B.chunks_prev, B.totals_prev, B.code, B.nkey, B.doff, perm_old, B.perm_alt, B.keys_alt,
+        n, B.total_blocks, B.err);
+    std::swap(B.perm, B.perm_alt);
+    B.keys_sorted = B.keys_alt;
   }
-  const int chunk_bound = std::min(B.total_blocks, n) + div_up(n, B.chunk);
   {
     TimerScope ts(tm, "k_p2g", st);
     k_p2g<D><<<clamp_groups(chunk_bound, G), kThreads, 0, st>>>(P, B.pf, B.pi, B.cap, B.perm,
@@ -1002,10 +1230,10 @@ void launch_substep(const Params& P, SubstepBuffers& B, int s_index, int raw, hi
     TimerScope ts(tm, "k_g2p", st);
     k_g2p<D><<<clamp_groups(chunk_bound, G), kThreads, 0, st>>>(
         P, B.pf, B.pi, B.cap, B.perm, B.chunks, B.totals, B.grid,
-        B.maxv + (int64_t)s_index * B.persistent_groups);
+        B.maxv + (int64_t)s_index * B.persistent_groups, B.code, B.nkey, B.dcount, B.count_next,
+        B.err);
   }
   std::swap(B.count, B.count_next);
-  (void)L::NF;
 }
 
 template <int D>

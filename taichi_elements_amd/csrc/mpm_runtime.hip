@@ -118,20 +118,28 @@ struct mpm_solver {
   int* perm_alt = nullptr;
   unsigned long long* lb_status_radix = nullptr;
   float4* tiles = nullptr;
-  int4* chunks = nullptr;
+  int4* chunks = nullptr;  // 2 x chunk_cap (substep parity)
   int64_t chunk_cap = 0;
+  int* code = nullptr;     // cap
+  int* nkey = nullptr;     // cap
+  int* dcount = nullptr;   // 27 x chunk_cap
+  int* doff = nullptr;     // 27 x chunk_cap
+  int* keys_sorted = nullptr;
   // buffers sized by the block window
   int* count = nullptr;
   int* count_next = nullptr;
   int* pstart = nullptr;
-  int* pchunk_begin = nullptr;
-  int* pchunk_cnt = nullptr;
+  int* pchunk_begin = nullptr;  // 2 x total_blocks (substep parity)
+  int* pchunk_cnt = nullptr;    // 2 x total_blocks
+  int4* tile_prefix = nullptr;
   int* active = nullptr;
   unsigned long long* lb_status_scan = nullptr;
   LBState* lbs = nullptr;
   int* ghist = nullptr;
   float4* grid = nullptr;
-  int* totals = nullptr;
+  int* totals = nullptr;  // 2 x 16 (substep parity)
+  int last_parity = 0;
+  bool reordered = false;
   int* err = nullptr;
   unsigned* maxv = nullptr;
   int maxv_cap = 0;
@@ -181,12 +189,19 @@ int check_device_errors(mpm_solver* s) {
   if (s->timing) s->timer.collect();
   int e = 0;
   HIP_TRY(hipMemcpy(&e, s->err, sizeof(int), hipMemcpyDeviceToHost));
-  if (e & 1) {
+  if (e) {
     int z = 0;
     HIP_TRY(hipMemcpy(s->err, &z, sizeof(int), hipMemcpyHostToDevice));
-    return fail(MPM_EDOMAIN,
-                "particles left the simulated window (positions outside the grid around the "
-                "domain); they were frozen");
+    if (e & 1)
+      return fail(MPM_EDOMAIN,
+                  "particles left the simulated window (positions outside the grid around the "
+                  "domain); results are no longer valid");
+    if (e & 8)
+      return fail(MPM_EDOMAIN,
+                  "a particle moved more than one 4-cell grid block in one substep (CFL number "
+                  "above 4); reduce the time step");
+    return fail(MPM_EHIP, "internal consistency check failed in the binning kernels (flags " +
+                              std::to_string(e) + ")");
   }
   return MPM_OK;
 }
@@ -229,6 +244,10 @@ int free_particle_buffers(mpm_solver* s) {
   dfree(s->lb_status_radix);
   dfree(s->tiles);
   dfree(s->chunks);
+  dfree(s->code);
+  dfree(s->nkey);
+  dfree(s->dcount);
+  dfree(s->doff);
   return MPM_OK;
 }
 
@@ -267,7 +286,11 @@ int grow(mpm_solver* s, int64_t need) {
   s->chunk_cap = std::min<int64_t>(s->total_blocks, ncap) + (ncap + s->chunk - 1) / s->chunk;
   const int TN = s->D == 3 ? Geo<3>::kTileNodes : Geo<2>::kTileNodes;
   TRY(dalloc(&s->tiles, s->chunk_cap * TN));
-  TRY(dalloc(&s->chunks, s->chunk_cap));
+  TRY(dalloc(&s->chunks, 2 * s->chunk_cap));
+  TRY(dalloc(&s->code, ncap));
+  TRY(dalloc(&s->nkey, ncap));
+  TRY(dalloc(&s->dcount, 27 * s->chunk_cap));
+  TRY(dalloc(&s->doff, 27 * s->chunk_cap));
   HIP_TRY(hipStreamSynchronize(s->stream));
   s->have_perm = false;
   return MPM_OK;
@@ -282,18 +305,32 @@ void fill_buffers(mpm_solver* s, SubstepBuffers& B) {
   B.keys_alt = s->keys_alt;
   B.perm = s->perm;
   B.perm_alt = s->perm_alt;
+  const int cur = (int)(s->substep_counter & 1), prev = cur ^ 1;
+  const int64_t tb = s->total_blocks;
   B.count = s->count;
   B.count_next = s->count_next;
   B.pstart = s->pstart;
-  B.pchunk_begin = s->pchunk_begin;
-  B.pchunk_cnt = s->pchunk_cnt;
-  B.chunks = s->chunks;
+  B.pchunk_begin = s->pchunk_begin + cur * tb;
+  B.pchunk_cnt = s->pchunk_cnt + cur * tb;
+  B.chunks = s->chunks + cur * s->chunk_cap;
+  B.pchunk_begin_prev = s->pchunk_begin + prev * tb;
+  B.pchunk_cnt_prev = s->pchunk_cnt + prev * tb;
+  B.chunks_prev = s->chunks + prev * s->chunk_cap;
+  B.totals_prev = s->totals + prev * 16;
+  B.code = s->code;
+  B.nkey = s->nkey;
+  B.dcount = s->dcount;
+  B.doff = s->doff;
+  B.tile_prefix = s->tile_prefix;
+  B.full_sort = !s->have_perm;
+  B.perm_identity = s->reordered;
+  B.keys_sorted = nullptr;
   B.active = s->active;
   B.lb_status_scan = s->lb_status_scan;
   B.lb_status_radix = s->lb_status_radix;
   B.lbs = s->lbs;
   B.ghist = s->ghist;
-  B.totals = s->totals;
+  B.totals = s->totals + cur * 16;
   B.tiles = s->tiles;
   B.grid = s->grid;
   B.err = s->err;
@@ -308,6 +345,8 @@ void fill_buffers(mpm_solver* s, SubstepBuffers& B) {
 void take_buffers(mpm_solver* s, const SubstepBuffers& B) {
   s->perm = B.perm;
   s->perm_alt = B.perm_alt;
+  s->keys_sorted = B.keys_sorted;
+  s->last_parity = B.parity;
   s->count = B.count;
   s->count_next = B.count_next;
 }
@@ -369,9 +408,10 @@ int mpm_create(const mpm_config* cfg, mpm_solver** out) {
   if ((r = dalloc(&s->count, tb + 1)) || (r = dalloc(&s->count_next, tb + 1)) ||
       (r = dalloc(&s->lb_status_scan, 4 * nparts)) || (r = dalloc(&s->lbs, 1)) ||
       (r = dalloc(&s->ghist, 2 * kMaxRadixPasses * kRadixBins)) ||
-      (r = dalloc(&s->pstart, tb)) || (r = dalloc(&s->pchunk_begin, tb)) ||
-      (r = dalloc(&s->pchunk_cnt, tb)) || (r = dalloc(&s->active, tb)) ||
-      (r = dalloc(&s->grid, tb * NPB)) || (r = dalloc(&s->totals, 16)) ||
+      (r = dalloc(&s->pstart, tb)) || (r = dalloc(&s->pchunk_begin, 2 * tb)) ||
+      (r = dalloc(&s->pchunk_cnt, 2 * tb)) || (r = dalloc(&s->active, tb)) ||
+      (r = dalloc(&s->tile_prefix, nparts)) ||
+      (r = dalloc(&s->grid, tb * NPB)) || (r = dalloc(&s->totals, 32)) ||
       (r = dalloc(&s->err, 1)) || (r = dalloc(&s->maxv, 16 * s->persistent_groups))) {
     mpm_destroy(s);
     return r;
@@ -385,7 +425,7 @@ int mpm_create(const mpm_config* cfg, mpm_solver** out) {
       hipMemsetAsync(s->ghist, 0, sizeof(int) * 2 * kMaxRadixPasses * kRadixBins, s->stream) !=
           hipSuccess ||
       hipMemsetAsync(s->grid, 0, sizeof(float4) * tb * NPB, s->stream) != hipSuccess ||
-      hipMemsetAsync(s->totals, 0, sizeof(int) * 16, s->stream) != hipSuccess ||
+      hipMemsetAsync(s->totals, 0, sizeof(int) * 32, s->stream) != hipSuccess ||
       hipMemsetAsync(s->err, 0, sizeof(int), s->stream) != hipSuccess ||
       hipStreamSynchronize(s->stream) != hipSuccess) {
     mpm_destroy(s);
@@ -405,6 +445,7 @@ int mpm_destroy(mpm_solver* s) {
   dfree(s->pstart);
   dfree(s->pchunk_begin);
   dfree(s->pchunk_cnt);
+  dfree(s->tile_prefix);
   dfree(s->active);
   dfree(s->lb_status_scan);
   dfree(s->lbs);
@@ -548,6 +589,7 @@ int mpm_substeps(mpm_solver* s, int32_t nsub, float dt, double t0) {
       std::swap(s->pf, s->pf_alt);
       std::swap(s->pi, s->pi_alt);
       s->since_reorder = 0;
+      s->reordered = true;
       fill_buffers(s, B);
     }
     if (s->D == 3)
@@ -557,6 +599,7 @@ int mpm_substeps(mpm_solver* s, int32_t nsub, float dt, double t0) {
     take_buffers(s, B);
     s->substep_counter++;
     s->have_perm = true;
+    s->reordered = false;
     s->since_reorder++;
   }
   HIP_TRY(hipGetLastError());
@@ -646,6 +689,7 @@ int mpm_set_state(mpm_solver* s, int32_t field, const void* host) {
   void* tmp = nullptr;
   HIP_TRY(hipMalloc(&tmp, sizeof(float) * s->n * nc));
   HIP_TRY(hipMemcpyAsync(tmp, host, sizeof(float) * s->n * nc, hipMemcpyHostToDevice, s->stream));
+  if (field == MPM_FIELD_X) s->have_perm = false;  // block keys must be recomputed
   const int* orig = s->pi + s->cap;
   if (is_int)
     launch_scatter_i((const int*)tmp, s->cap, nc, orig, (int)s->n,
@@ -678,7 +722,8 @@ int mpm_read_grid(mpm_solver* s, const int32_t* lo, const int32_t* nn, float* m,
   // only blocks active in the last substep hold this substep's values
   int nact = 0;
   std::vector<int> tot(16);
-  HIP_TRY(hipMemcpy(tot.data(), s->totals, sizeof(int) * 16, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(tot.data(), s->totals + s->last_parity * 16, sizeof(int) * 16,
+                    hipMemcpyDeviceToHost));
   nact = tot[kTotActive];
   std::vector<int> act(std::max(nact, 1));
   if (nact > 0)
@@ -722,7 +767,7 @@ int mpm_read_binning(mpm_solver* s, int32_t* keys, int32_t* perm, int32_t* slot_
   TRY(check_device_errors(s));
   if (!s->have_perm) return fail(MPM_EINVAL, "no substep has run since the last change");
   if (s->n == 0) return MPM_OK;
-  if (keys) HIP_TRY(hipMemcpy(keys, s->keys, sizeof(int) * s->n, hipMemcpyDeviceToHost));
+  if (keys) HIP_TRY(hipMemcpy(keys, s->keys_sorted, sizeof(int) * s->n, hipMemcpyDeviceToHost));
   if (perm) HIP_TRY(hipMemcpy(perm, s->perm, sizeof(int) * s->n, hipMemcpyDeviceToHost));
   if (slot_to_insertion)
     HIP_TRY(hipMemcpy(slot_to_insertion, s->pi + s->cap, sizeof(int) * s->n,
@@ -791,7 +836,8 @@ int mpm_last_counts(mpm_solver* s, int64_t* out4) {
   if (!s || !out4) return fail(MPM_EINVAL, "null argument");
   TRY(check_device_errors(s));
   int tot[16];
-  HIP_TRY(hipMemcpy(tot, s->totals, sizeof(int) * 16, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(tot, s->totals + s->last_parity * 16, sizeof(int) * 16,
+                    hipMemcpyDeviceToHost));
   out4[0] = tot[kTotValid];
   out4[1] = tot[kTotChunks];
   out4[2] = tot[kTotActive];

@@ -125,25 +125,55 @@ def test_binning_keys_and_permutation_bit_exact(oracle, dim):
     np.testing.assert_array_equal(orig, np.arange(n))
     blo, bn = s.block_window()
     ref_keys = oracle.block_keys(x, np.float32(s.inv_dx), 4, np.array(blo), np.array(bn))
-    np.testing.assert_array_equal(keys, ref_keys)
-    np.testing.assert_array_equal(perm, oracle.sort_by_key(ref_keys, int(np.prod(bn))))
+    ref_perm = oracle.sort_by_key(ref_keys, int(np.prod(bn)))
+    np.testing.assert_array_equal(perm, ref_perm)
+    np.testing.assert_array_equal(keys, ref_keys[ref_perm])
 
 
-def test_binning_after_reorder_is_sorted(oracle):
-    """After physical reorders the slots move; keys and perm still describe a
-    stable sort of the current slots."""
+@pytest.mark.parametrize("reorder", [False, True])
+def test_neighbour_sort_is_stable_sort_of_previous_order(oracle, reorder):
+    """From the second substep on, the binning is a stable sort of the
+    previous substep's order by the new block keys (computed by g2p); with a
+    physical reorder in between, the previous order is the reordered slots."""
     rng = np.random.default_rng(310)
-    s = _solver(3, 32, reorder_interval=1)
-    n = 8000
+    s = _solver(3, 32, reorder_interval=1 if reorder else -1)
+    n = 20000
     x = cube_particles(rng, n, [0.2] * 3, [0.5] * 3, 3)
-    v = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
+    v = (3 * rng.uniform(-1, 1, (n, 3))).astype(np.float32)
     s.read_restart(n, x, v, np.ones(n, np.int32), np.zeros(n, np.int32))
-    for _ in range(3):
+    blo, bn = s.block_window()
+    for step in range(6):
+        _, perm_prev, orig_prev = s.read_binning() if step else (None, None, None)
+        x_now = s.get_state()["x"]  # insertion order, before this substep
         s.substep(s.default_dt)
-    keys, perm, orig = s.read_binning()
+        keys, perm, orig = s.read_binning()
+        key_ins = oracle.block_keys(x_now, np.float32(s.inv_dx), 4, np.array(blo), np.array(bn))
+        key_slot = key_ins[orig]  # key of the particle in each slot
+        if step == 0:
+            prev_order = np.arange(n)
+        elif reorder:
+            # slots were rewritten in the previous order at the start of this substep
+            prev_order = np.arange(n)
+        else:
+            np.testing.assert_array_equal(orig, orig_prev)
+            prev_order = perm_prev
+        expected = prev_order[np.argsort(key_slot[prev_order], kind="stable")]
+        np.testing.assert_array_equal(perm, expected)
+        np.testing.assert_array_equal(keys, key_slot[perm])
+        assert np.all(np.diff(keys) >= 0)
     assert sorted(orig.tolist()) == list(range(n))
-    assert np.all(np.diff(orig) != 0)
-    np.testing.assert_array_equal(perm, np.argsort(keys, kind="stable"))
+
+
+def test_cfl_violation_is_reported():
+    """A particle that crosses more than one 4-cell block in a substep cannot
+    be binned by the neighbour sort; the engine reports it."""
+    from taichi_elements_amd._lib import DomainError
+    s = _solver(3, 32)
+    rng = np.random.default_rng(311)
+    x = cube_particles(rng, 4000, [0.3] * 3, [0.1] * 3, 3)
+    s.add_particles(x, s.material_elastic, velocity=[400.0, 0.0, 0.0])  # ~8 cells per substep
+    with pytest.raises(DomainError):  # g2p of this substep finds the far moves
+        s.substep(s.default_dt)
 
 
 def test_reorder_does_not_change_results():
