@@ -82,6 +82,14 @@ typedef struct {
  * empty grid post-process list (the Python host adds the bounding box
  * first, as mpm_solver.py:300-302 does). */
 int mpm_create(const mpm_config* cfg, mpm_solver** out);
+/* Batched scenes (no reference counterpart: the reference simulates one
+ * scene per MPMSolver).  nscenes independent scenes of the same dim, each
+ * with its own constants (res, dx, dt, Lame parameters...), are advanced
+ * together: every stage of a substep is one launch over all of them.
+ * Configuration calls (gravity, post-processes, mpm_add_particles) apply to
+ * the scene chosen with mpm_set_active_scene (0 after creation). */
+int mpm_create_batch(const mpm_config* cfgs, int32_t nscenes, mpm_solver** out);
+int mpm_set_active_scene(mpm_solver* s, int32_t scene);
 int mpm_destroy(mpm_solver* s);
 /* Message of the last failing call on this thread ("" if none). */
 const char* mpm_last_error(void);
@@ -118,6 +126,9 @@ int mpm_num_particles(mpm_solver* s, int64_t* n);
  * compute_max_velocity -- all with the same dt; t0 is the time handed to
  * the post-process of the first substep.  Asynchronous. */
 int mpm_substeps(mpm_solver* s, int32_t n, float dt, double t0);
+/* Batched form of mpm_substeps: one substep of every scene per iteration,
+ * scene k with time step dts[k]. */
+int mpm_substeps_batch(mpm_solver* s, int32_t n, const float* dts);
 /* Per-substep max over particles of max_i |v_i| (compute_max_velocity,
  * mpm_solver.py:726-735) for the substeps of the last mpm_substeps call;
  * out has room for that call's n entries.  Synchronises. */

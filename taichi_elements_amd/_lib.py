@@ -33,6 +33,10 @@ _I64P = ctypes.POINTER(ctypes.c_int64)
 _FP = ctypes.POINTER(ctypes.c_float)
 SIGNATURES = [
     ("mpm_create", ctypes.c_int, [ctypes.POINTER(MpmConfig), ctypes.POINTER(_P)]),
+    ("mpm_create_batch", ctypes.c_int, [ctypes.POINTER(MpmConfig), ctypes.c_int32,
+                                        ctypes.POINTER(_P)]),
+    ("mpm_set_active_scene", ctypes.c_int, [_P, ctypes.c_int32]),
+    ("mpm_substeps_batch", ctypes.c_int, [_P, ctypes.c_int32, _FP]),
     ("mpm_destroy", ctypes.c_int, [_P]),
     ("mpm_last_error", ctypes.c_char_p, []),
     ("mpm_set_gravity", ctypes.c_int, [_P, _D3]),

@@ -50,8 +50,10 @@ struct Params {
   float gravity[3];
   int res[3];
   int padding;
-  int blo[3];           // block window origin (in blocks)
-  int bn[3];            // block window extent (in blocks)
+  int blo[3];           // origin of the scene's block window (in blocks)
+  int bnl[3];           // extent of the scene's block window (in blocks)
+  int bn[3];            // extent of the combined window of all scenes (in blocks)
+  int zoff;             // the scene's first block along the stacking axis (last axis)
   int n_post;
   PostOp post[kMaxPost];
 };

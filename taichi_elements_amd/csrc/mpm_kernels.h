@@ -15,7 +15,7 @@ template <int D>
 struct Layout {
   static constexpr int X = 0, V = D, C = 2 * D, F = 2 * D + D * D, JP = 2 * D + 2 * D * D;
   static constexpr int NF = 2 * D + 2 * D * D + 1;
-  static constexpr int MAT = 0, ORIG = 1, NI = 2;
+  static constexpr int MAT = 0, ORIG = 1, SCENE = 2, NI = 3;
 };
 
 constexpr int kThreads = 256;
@@ -88,16 +88,18 @@ struct SubstepBuffers {
 
 class Timer;
 
+// P: device table of the scenes' parameters; slab: scene of each combined
+// block coordinate along the stacking axis.
 template <int D>
-void launch_substep(const Params& P, SubstepBuffers& B, int substep_index, int raw_grid,
-                    hipStream_t st, Timer* timer);
+void launch_substep(const Params* P, const int* slab, SubstepBuffers& B, int substep_index,
+                    int raw_grid, hipStream_t st, Timer* timer);
 
 template <int D>
 void launch_reorder(SubstepBuffers& B, float* pf_alt, int* pi_alt, hipStream_t st, Timer* timer);
 
 template <int D>
 void launch_append(SubstepBuffers& B, const float* x, const float* v, const int* mat, int k,
-                   int n_old, hipStream_t st);
+                   int n_old, int scene, hipStream_t st);
 
 void launch_gather_f(const float* src, int64_t cap, int ncomp, const int* orig, int n,
                      float* dst, hipStream_t st);

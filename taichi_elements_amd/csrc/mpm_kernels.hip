@@ -73,13 +73,30 @@ __device__ __forceinline__ int encode_local(const int lb[3], const Params& P) {
   return D == 3 ? (lb[2] * P.bn[1] + lb[1]) * P.bn[0] + lb[0] : lb[1] * P.bn[0] + lb[0];
 }
 
+// Scenes are stacked along the last axis of one combined block window; the
+// slab table maps a combined block coordinate on that axis to its scene.
+template <int D>
+__device__ __forceinline__ int scene_of(const int lb[3], const int* __restrict__ slab) {
+  return slab[lb[D - 1]];
+}
+
+// scene-local block coordinates of a combined block
+template <int D>
+__device__ __forceinline__ void to_local(const int lb[3], const Params& P, int out[3]) {
+  out[0] = lb[0];
+  out[1] = lb[1];
+  out[2] = lb[2];
+  out[D - 1] -= P.zoff;
+}
+
 // ---------------------------------------------------------------------------
 // binning: block keys and per-block counts (build_pid, mpm_solver.py:357-361)
 // ---------------------------------------------------------------------------
 template <int D>
-__global__ __launch_bounds__(kThreads) void k_keys(Params P, const float* __restrict__ pf,
-                                                   int64_t cap, int n, int total_blocks,
-                                                   int* __restrict__ keys,
+__global__ __launch_bounds__(kThreads) void k_keys(const Params* __restrict__ scenes,
+                                                   const float* __restrict__ pf,
+                                                   const int* __restrict__ pi, int64_t cap, int n,
+                                                   int total_blocks, int* __restrict__ keys,
                                                    int* __restrict__ count, int* __restrict__ err) {
   using L = Layout<D>;
   const int lane = threadIdx.x & 63;
@@ -88,6 +105,7 @@ __global__ __launch_bounds__(kThreads) void k_keys(Params P, const float* __rest
     bool valid = false;
     int key = total_blocks;
     if (i < n) {
+      const Params& P = scenes[pi[(int64_t)L::SCENE * cap + i]];
       int lin = 0;
       valid = true;
 #pragma unroll
@@ -96,9 +114,9 @@ __global__ __launch_bounds__(kThreads) void k_keys(Params P, const float* __rest
         float fx;
         base_fx(pf[(int64_t)(L::X + a) * cap + i], P.inv_dx, base, fx);
         const int lb = (base >> 2) - P.blo[a];
-        // the block and its +1 neighbour must lie inside the window
-        if (lb < 0 || lb >= P.bn[a] - 1) valid = false;
-        lin = lin * P.bn[a] + lb;
+        // the block and its +1 neighbour must lie inside the scene's window
+        if (lb < 0 || lb >= P.bnl[a] - 1) valid = false;
+        lin = lin * P.bn[a] + lb + (a == D - 1 ? P.zoff : 0);
       }
       key = valid ? lin : total_blocks;
       keys[i] = key;
@@ -223,7 +241,7 @@ __device__ __forceinline__ void lb_release(LBState* st, int ntiles, unsigned epo
 // ---------------------------------------------------------------------------
 template <int D>
 __global__ __launch_bounds__(kThreads) void k_bscan(
-    Params P, const int* __restrict__ count, int* __restrict__ count_next, int total_blocks,
+    const Params* __restrict__ scenes, const int* __restrict__ count, int* __restrict__ count_next, int total_blocks,
     int chunk, int ntiles, int passes, int* __restrict__ pstart, int* __restrict__ pchunk_begin,
     int* __restrict__ pchunk_cnt, int4* __restrict__ chunks, int* __restrict__ active,
     int* __restrict__ totals, int* __restrict__ ghist, int* __restrict__ ghist_next,
@@ -233,6 +251,7 @@ __global__ __launch_bounds__(kThreads) void k_bscan(
   __shared__ int4 wsum[kThreads / 64];
   __shared__ unsigned excl_sh[4];
   __shared__ int lh[kMaxRadixPasses][kRadixBins];
+  const Params& P = scenes[0];  // the combined window extents
   // tile_prefix != nullptr: exclusive tile prefixes were computed by
   // k_bscan_reduce + k_bscan_top (large windows, where a look-back chain over
   // thousands of tiles is slower than two extra launches)
@@ -329,10 +348,12 @@ __global__ __launch_bounds__(kThreads) void k_bscan(
 // Two-level mode of the block-window scan: per-tile sums, then one workgroup
 // turns them into exclusive tile prefixes.
 template <int D>
-__global__ __launch_bounds__(kThreads) void k_bscan_reduce(Params P, const int* __restrict__ count,
+__global__ __launch_bounds__(kThreads) void k_bscan_reduce(const Params* __restrict__ scenes,
+                                                           const int* __restrict__ count,
                                                            int total_blocks, int chunk,
                                                            int4* __restrict__ partials) {
   __shared__ int4 wsum[kThreads / 64];
+  const Params& P = scenes[0];  // the combined window extents
   int4 acc = make_int4(0, 0, 0, 0);
   const int first = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
 #pragma unroll
@@ -469,7 +490,9 @@ __global__ __launch_bounds__(kThreads) void k_onesweep(
 // The accumulation order is fixed, so p2g is bitwise reproducible.
 // ---------------------------------------------------------------------------
 template <int D>
-__global__ __launch_bounds__(kThreads) void k_p2g(Params P, float* __restrict__ pf,
+__global__ __launch_bounds__(kThreads) void k_p2g(const Params* __restrict__ scenes,
+                                                  const int* __restrict__ slab,
+                                                  float* __restrict__ pf,
                                                   const int* __restrict__ pi, int64_t cap,
                                                   const int* __restrict__ perm,
                                                   const int4* __restrict__ chunks,
@@ -494,8 +517,10 @@ __global__ __launch_bounds__(kThreads) void k_p2g(Params P, float* __restrict__ 
   for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
     const int4 ck = chunks[c];
     const int nc = ck.z - ck.y;  // <= kChunkMax
-    int b[3];
-    decode_local<D>(ck.x, P, b);
+    int lbc[3], b[3];
+    decode_local<D>(ck.x, scenes[0], lbc);
+    const Params& P = scenes[scene_of<D>(lbc, slab)];
+    to_local<D>(lbc, P, b);
 #pragma unroll
     for (int a = 0; a < D; a++) b[a] = (b[a] + P.blo[a]) * kBlock;  // first node of the block
     for (int i = threadIdx.x; i < SUBS * TN; i += kThreads)
@@ -707,7 +732,9 @@ __global__ __launch_bounds__(kThreads) void k_p2g(Params P, float* __restrict__ 
 // grid operator (mpm_solver.py:586-692)
 // ---------------------------------------------------------------------------
 template <int D>
-__global__ __launch_bounds__(kThreads) void k_grid_op(Params P, const int* __restrict__ active,
+__global__ __launch_bounds__(kThreads) void k_grid_op(const Params* __restrict__ scenes,
+                                                      const int* __restrict__ slab,
+                                                      const int* __restrict__ active,
                                                       const int* __restrict__ totals,
                                                       const int* __restrict__ pchunk_begin,
                                                       const int* __restrict__ pchunk_cnt,
@@ -720,7 +747,8 @@ __global__ __launch_bounds__(kThreads) void k_grid_op(Params P, const int* __res
     const int blk = (int)(t / NPB), node = (int)(t % NPB);
     const int bid = active[blk];
     int lb[3];
-    decode_local<D>(bid, P, lb);
+    decode_local<D>(bid, scenes[0], lb);
+    const Params& P = scenes[scene_of<D>(lb, slab)];
     const int l[3] = {node & 3, (node >> 2) & 3, node >> 4};
     float m = 0.0f, p[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
@@ -758,9 +786,10 @@ __global__ __launch_bounds__(kThreads) void k_grid_op(Params P, const int* __res
 #pragma unroll
         for (int a = 0; a < D; a++) p[a] += P.dt * P.gravity[a];
       }
-      int I[3];
+      int I[3], lbl[3];
+      to_local<D>(lb, P, lbl);
 #pragma unroll
-      for (int a = 0; a < D; a++) I[a] = (lb[a] + P.blo[a]) * kBlock + l[a];
+      for (int a = 0; a < D; a++) I[a] = (lbl[a] + P.blo[a]) * kBlock + l[a];
       apply_post<D>(P, I, p);
     }
     grid[(int64_t)bid * NPB + node] = make_float4(p[0], p[1], D == 3 ? p[2] : 0.0f, m);
@@ -771,7 +800,9 @@ __global__ __launch_bounds__(kThreads) void k_grid_op(Params P, const int* __res
 // G2P (mpm_solver.py:694-735)
 // ---------------------------------------------------------------------------
 template <int D>
-__global__ __launch_bounds__(kThreads) void k_g2p(Params P, float* __restrict__ pf,
+__global__ __launch_bounds__(kThreads) void k_g2p(const Params* __restrict__ scenes,
+                                                  const int* __restrict__ slab,
+                                                  float* __restrict__ pf,
                                                   const int* __restrict__ pi, int64_t cap,
                                                   const int* __restrict__ perm,
                                                   const int4* __restrict__ chunks,
@@ -793,8 +824,10 @@ __global__ __launch_bounds__(kThreads) void k_g2p(Params P, float* __restrict__ 
   const int nchunks = totals[kTotChunks];
   for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
     const int4 ck = chunks[c];
-    int lb[3];
-    decode_local<D>(ck.x, P, lb);
+    int lb[3], lbl[3];
+    decode_local<D>(ck.x, scenes[0], lb);
+    const Params& P = scenes[scene_of<D>(lb, slab)];
+    to_local<D>(lb, P, lbl);
     for (int i = threadIdx.x; i < (kThreads / 64) * 32; i += kThreads) (&dwcnt[0][0])[i] = 0;
     for (int i = threadIdx.x; i < TN; i += kThreads) {
       const int tl[3] = {i % kTile, (i / kTile) % kTile, i / (kTile * kTile)};
@@ -812,7 +845,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p(Params P, float* __restrict__ 
     __syncthreads();
     int b[3];
 #pragma unroll
-    for (int a = 0; a < D; a++) b[a] = (lb[a] + P.blo[a]) * kBlock;
+    for (int a = 0; a < D; a++) b[a] = (lbl[a] + P.blo[a]) * kBlock;
     // one particle per thread (chunks hold at most kThreads particles)
     const int j = ck.y + threadIdx.x;
     const bool live = j < ck.z;
@@ -905,18 +938,19 @@ __global__ __launch_bounds__(kThreads) void k_g2p(Params P, float* __restrict__ 
         int base;
         float f;
         base_fx(xn[a], P.inv_dx, base, f);
-        q[a] = (base >> 2) - P.blo[a];
-        dd[a] = q[a] - lb[a];
-        near &= (dd[a] >= -1) & (dd[a] <= 1) & (q[a] >= 0) & (q[a] < P.bn[a] - 1);
+        q[a] = (base >> 2) - P.blo[a];  // scene-local block
+        dd[a] = q[a] - lbl[a];
+        near &= (dd[a] >= -1) & (dd[a] <= 1) & (q[a] >= 0) & (q[a] < P.bnl[a] - 1);
       }
       if (near) {
         dir = (dd[2] + 1) * 9 + (dd[1] + 1) * 3 + (dd[0] + 1);
+        q[D - 1] += P.zoff;
         key_new = encode_local<D>(q, P);
       } else {
         // left the window (bit 0) or moved more than one block (bit 3): the
         // particle stays binned in its old block and the caller is told
-        atomicOr(err, (q[0] < 0 || q[0] >= P.bn[0] - 1 || q[1] < 0 || q[1] >= P.bn[1] - 1 ||
-                       (D == 3 && (q[2] < 0 || q[2] >= P.bn[2] - 1)))
+        atomicOr(err, (q[0] < 0 || q[0] >= P.bnl[0] - 1 || q[1] < 0 || q[1] >= P.bnl[1] - 1 ||
+                       (D == 3 && (q[2] < 0 || q[2] >= P.bnl[2] - 1)))
                           ? 1
                           : 8);
       }
@@ -986,13 +1020,15 @@ __global__ __launch_bounds__(kThreads) void k_g2p(Params P, float* __restrict__ 
 // ---------------------------------------------------------------------------
 template <int D>
 __global__ __launch_bounds__(kThreads) void k_offsets(
-    Params P, const int4* __restrict__ chunks, const int* __restrict__ totals,
+    const Params* __restrict__ scenes, const int4* __restrict__ chunks,
+    const int* __restrict__ totals,
     const int* __restrict__ pstart, const int* __restrict__ count,
     const int* __restrict__ pchunk_begin_prev, const int* __restrict__ pchunk_cnt_prev,
     const int* __restrict__ dcount, int* __restrict__ doff, int* __restrict__ err) {
   // one wave per new chunk; the wave of a block's first chunk does the block:
   // lane t < 3^d takes old neighbour t in ascending key order, sums its
   // chunks' counts toward this block, and a wave scan orders the neighbours
+  const Params& P = scenes[0];  // the combined window extents
   const int nch = totals[kTotChunks];
   const int lane = threadIdx.x & 63;
   const int waves = gridDim.x * (kThreads / 64);
@@ -1085,7 +1121,7 @@ template <int D>
 __global__ __launch_bounds__(kThreads) void k_append(const float* __restrict__ x,
                                                      const float* __restrict__ v,
                                                      const int* __restrict__ mat, int k,
-                                                     int n_old, float* __restrict__ pf,
+                                                     int n_old, int scene, float* __restrict__ pf,
                                                      int* __restrict__ pi, int64_t cap) {
   using L = Layout<D>;
   const int t = blockIdx.x * kThreads + threadIdx.x;
@@ -1105,6 +1141,7 @@ __global__ __launch_bounds__(kThreads) void k_append(const float* __restrict__ x
   pf[(int64_t)L::JP * cap + s] = m == kSand ? 0.0f : 1.0f;  // mpm_solver.py:831-835
   pi[(int64_t)L::MAT * cap + s] = m;
   pi[(int64_t)L::ORIG * cap + s] = s;
+  pi[(int64_t)L::SCENE * cap + s] = scene;
 }
 
 template <typename T>
@@ -1154,8 +1191,8 @@ struct TimerScope {
 };
 
 template <int D>
-void launch_substep(const Params& P, SubstepBuffers& B, int s_index, int raw, hipStream_t st,
-                    Timer* tm) {
+void launch_substep(const Params* P, const int* slab, SubstepBuffers& B, int s_index, int raw,
+                    hipStream_t st, Timer* tm) {
   const int n = B.n;
   const int G = B.persistent_groups;
   if (n == 0) return;
@@ -1170,7 +1207,7 @@ void launch_substep(const Params& P, SubstepBuffers& B, int s_index, int raw, hi
     TimerScope ts(tm, "k_keys", st);
     launch_fill_u32((unsigned*)B.count, 0u, B.total_blocks + 1, st);
     k_keys<D><<<clamp_groups(div_up(n, kThreads), 8192), kThreads, 0, st>>>(
-        P, B.pf, B.cap, n, B.total_blocks, B.keys, B.count, B.err);
+        P, B.pf, B.pi, B.cap, n, B.total_blocks, B.keys, B.count, B.err);
   }
   {
     TimerScope ts(tm, "k_bscan", st);
@@ -1217,19 +1254,19 @@ void launch_substep(const Params& P, SubstepBuffers& B, int s_index, int raw, hi
   }
   {
     TimerScope ts(tm, "k_p2g", st);
-    k_p2g<D><<<clamp_groups(chunk_bound, G), kThreads, 0, st>>>(P, B.pf, B.pi, B.cap, B.perm,
-                                                               B.chunks, B.totals, B.tiles);
+    k_p2g<D><<<clamp_groups(chunk_bound, G), kThreads, 0, st>>>(
+        P, slab, B.pf, B.pi, B.cap, B.perm, B.chunks, B.totals, B.tiles);
   }
   {
     TimerScope ts(tm, "k_grid_op", st);
     const int64_t node_bound = (int64_t)B.total_blocks * Geo<D>::kNodesPerBlock;
     k_grid_op<D><<<clamp_groups(div_up(node_bound, kThreads), G / 2), kThreads, 0, st>>>(
-        P, B.active, B.totals, B.pchunk_begin, B.pchunk_cnt, B.tiles, B.grid, raw);
+        P, slab, B.active, B.totals, B.pchunk_begin, B.pchunk_cnt, B.tiles, B.grid, raw);
   }
   {
     TimerScope ts(tm, "k_g2p", st);
     k_g2p<D><<<clamp_groups(chunk_bound, G), kThreads, 0, st>>>(
-        P, B.pf, B.pi, B.cap, B.perm, B.chunks, B.totals, B.grid,
+        P, slab, B.pf, B.pi, B.cap, B.perm, B.chunks, B.totals, B.grid,
         B.maxv + (int64_t)s_index * B.persistent_groups, B.code, B.nkey, B.dcount, B.count_next,
         B.err);
   }
@@ -1245,9 +1282,10 @@ void launch_reorder(SubstepBuffers& B, float* pf_alt, int* pi_alt, hipStream_t s
 
 template <int D>
 void launch_append(SubstepBuffers& B, const float* x, const float* v, const int* mat, int k,
-                   int n_old, hipStream_t st) {
+                   int n_old, int scene, hipStream_t st) {
   if (k <= 0) return;
-  k_append<D><<<div_up(k, kThreads), kThreads, 0, st>>>(x, v, mat, k, n_old, B.pf, B.pi, B.cap);
+  k_append<D><<<div_up(k, kThreads), kThreads, 0, st>>>(x, v, mat, k, n_old, scene, B.pf, B.pi,
+                                                        B.cap);
 }
 
 void launch_gather_f(const float* src, int64_t cap, int ncomp, const int* orig, int n,
@@ -1279,13 +1317,15 @@ void launch_fill_u32(unsigned* p, unsigned v, int64_t n, hipStream_t st) {
     k_fill_u32<<<clamp_groups(div_up(n, kThreads), 4096), kThreads, 0, st>>>(p, v, n);
 }
 
-template void launch_substep<2>(const Params&, SubstepBuffers&, int, int, hipStream_t, Timer*);
-template void launch_substep<3>(const Params&, SubstepBuffers&, int, int, hipStream_t, Timer*);
+template void launch_substep<2>(const Params*, const int*, SubstepBuffers&, int, int, hipStream_t,
+                                Timer*);
+template void launch_substep<3>(const Params*, const int*, SubstepBuffers&, int, int, hipStream_t,
+                                Timer*);
 template void launch_reorder<2>(SubstepBuffers&, float*, int*, hipStream_t, Timer*);
 template void launch_reorder<3>(SubstepBuffers&, float*, int*, hipStream_t, Timer*);
 template void launch_append<2>(SubstepBuffers&, const float*, const float*, const int*, int, int,
-                               hipStream_t);
+                               int, hipStream_t);
 template void launch_append<3>(SubstepBuffers&, const float*, const float*, const int*, int, int,
-                               hipStream_t);
+                               int, hipStream_t);
 
 }  // namespace mpm

@@ -87,19 +87,30 @@ struct EventTimer : public Timer {
   }
 };
 
+// One simulated scene: its constants, gravity, grid post-process list and
+// its slab of the combined block window.
+struct SceneHost {
+  mpm_config cfg{};
+  double gravity[3] = {0, 0, 0};
+  std::vector<PostOp> post;
+  int blo[3] = {0, 0, 0}, bnl[3] = {1, 1, 1};
+  int zoff = 0;
+};
+
 }  // namespace
 
 struct mpm_solver {
-  mpm_config cfg{};
   int D = 3;
   int device = 0;
   hipStream_t stream = nullptr;
-  Params P{};  // dt filled per call
-  double gravity[3] = {0, 0, 0};
-  std::vector<PostOp> post;
+  std::vector<SceneHost> scenes;
+  int active_scene = 0;       // target of the configuration calls and add_particles
+  Params* h_params = nullptr;  // pinned staging of the scene table
+  Params* d_params = nullptr;  // device scene table
+  int* d_slab = nullptr;       // scene of each combined block coordinate on the last axis
 
-  // block window
-  int blo[3] = {0, 0, 0}, bn[3] = {1, 1, 1};
+  // combined block window of all scenes (stacked along the last axis)
+  int bn[3] = {1, 1, 1};
   int total_blocks = 0;
   int key_bits = 0;
 
@@ -206,9 +217,8 @@ int check_device_errors(mpm_solver* s) {
   return MPM_OK;
 }
 
-void fill_params(mpm_solver* s, float dt) {
-  Params& P = s->P;
-  const mpm_config& c = s->cfg;
+void fill_params(Params& P, const mpm_solver* s, const SceneHost& sc, float dt) {
+  const mpm_config& c = sc.cfg;
   P.dx = (float)c.dx;
   P.inv_dx = (float)c.inv_dx;
   P.dt = dt;
@@ -222,14 +232,16 @@ void fill_params(mpm_solver* s, float dt) {
   P.c4 = (float)(4 * c.inv_dx);
   P.sand_coef = (float)((s->D * c.lambda_0 + 2 * c.mu_0) / (2 * c.mu_0));
   for (int a = 0; a < 3; a++) {
-    P.gravity[a] = a < s->D ? (float)s->gravity[a] : 0.0f;
+    P.gravity[a] = a < s->D ? (float)sc.gravity[a] : 0.0f;
     P.res[a] = a < s->D ? c.res[a] : 1;
-    P.blo[a] = s->blo[a];
+    P.blo[a] = sc.blo[a];
+    P.bnl[a] = sc.bnl[a];
     P.bn[a] = s->bn[a];
   }
   P.padding = c.padding;
-  P.n_post = (int)s->post.size();
-  for (size_t k = 0; k < s->post.size(); k++) P.post[k] = s->post[k];
+  P.zoff = sc.zoff;
+  P.n_post = (int)sc.post.size();
+  for (size_t k = 0; k < sc.post.size(); k++) P.post[k] = sc.post[k];
 }
 
 int free_particle_buffers(mpm_solver* s) {
@@ -260,13 +272,14 @@ int grow(mpm_solver* s, int64_t need) {
   int *pi = nullptr, *pi_alt = nullptr;
   TRY(dalloc(&pf, (int64_t)nf * ncap));
   TRY(dalloc(&pf_alt, (int64_t)nf * ncap));
-  TRY(dalloc(&pi, 2 * ncap));
-  TRY(dalloc(&pi_alt, 2 * ncap));
+  const int ni = s->D == 3 ? Layout<3>::NI : Layout<2>::NI;
+  TRY(dalloc(&pi, ni * ncap));
+  TRY(dalloc(&pi_alt, ni * ncap));
   if (s->n > 0) {
     HIP_TRY(hipMemcpy2DAsync(pf, ncap * sizeof(float), s->pf, s->cap * sizeof(float),
                              s->n * sizeof(float), nf, hipMemcpyDeviceToDevice, s->stream));
     HIP_TRY(hipMemcpy2DAsync(pi, ncap * sizeof(int), s->pi, s->cap * sizeof(int),
-                             s->n * sizeof(int), 2, hipMemcpyDeviceToDevice, s->stream));
+                             s->n * sizeof(int), ni, hipMemcpyDeviceToDevice, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
   }
   free_particle_buffers(s);
@@ -357,38 +370,50 @@ extern "C" {
 
 const char* mpm_last_error(void) { return g_last_error.c_str(); }
 
-int mpm_create(const mpm_config* cfg, mpm_solver** out) {
-  if (!cfg || !out) return fail(MPM_EINVAL, "null argument");
+static int create_impl(const mpm_config* cfgs, int nscenes, mpm_solver** out) {
+  if (!cfgs || !out || nscenes < 1) return fail(MPM_EINVAL, "null argument or no scene");
   *out = nullptr;
-  if (cfg->dim != 2 && cfg->dim != 3)
-    return fail(MPM_EINVAL, "MPM solver supports only 2D and 3D simulations.");
-  for (int a = 0; a < cfg->dim; a++)
-    if (cfg->res[a] <= 0 || cfg->res[a] > 65536) return fail(MPM_EINVAL, "bad resolution");
-  if (!(cfg->dx > 0) || !(cfg->inv_dx > 0)) return fail(MPM_EINVAL, "bad dx");
+  const int D = cfgs[0].dim;
+  if (D != 2 && D != 3) return fail(MPM_EINVAL, "MPM solver supports only 2D and 3D simulations.");
+  for (int k = 0; k < nscenes; k++) {
+    const mpm_config* cfg = &cfgs[k];
+    if (cfg->dim != D) return fail(MPM_EINVAL, "all scenes of a batch must have the same dim");
+    for (int a = 0; a < D; a++)
+      if (cfg->res[a] <= 0 || cfg->res[a] > 65536) return fail(MPM_EINVAL, "bad resolution");
+    if (!(cfg->dx > 0) || !(cfg->inv_dx > 0)) return fail(MPM_EINVAL, "bad dx");
+  }
+  const mpm_config* cfg = &cfgs[0];
   int ndev = 0;
   HIP_TRY(hipGetDeviceCount(&ndev));
   if (cfg->device < 0 || cfg->device >= ndev)
     return fail(MPM_EINVAL, "device " + std::to_string(cfg->device) + " not present (" +
                                 std::to_string(ndev) + " visible)");
   mpm_solver* s = new mpm_solver();
-  s->cfg = *cfg;
-  s->D = cfg->dim;
+  s->D = D;
   s->device = cfg->device;
   if (cfg->reorder_interval > 0) s->reorder_interval = cfg->reorder_interval;
   if (cfg->reorder_interval < 0) s->reorder_interval = 0;
   if (cfg->chunk > 0) s->chunk = std::min(cfg->chunk, kChunkMax);
-  s->gravity[1] = -9.8;  // mpm_solver.py:283, 296
-  // Block window: 2 blocks below the domain, enough above it for the wall
-  // padding plus a margin; a particle block must also have its +1 neighbour
-  // inside the window.
-  for (int a = 0; a < 3; a++) {
-    if (a < s->D) {
-      s->blo[a] = -2;
-      s->bn[a] = (cfg->res[a] + 3) / 4 + 5;
-    } else {
-      s->blo[a] = 0;
-      s->bn[a] = 1;
+  // Each scene's block window: 2 blocks below its domain, enough above it for
+  // the wall padding plus a margin; a particle block must also have its +1
+  // neighbour inside the window.  Scenes are stacked along the last axis.
+  s->scenes.resize(nscenes);
+  for (int k = 0; k < nscenes; k++) {
+    SceneHost& sc = s->scenes[k];
+    sc.cfg = cfgs[k];
+    sc.gravity[1] = -9.8;  // mpm_solver.py:283, 296
+    for (int a = 0; a < 3; a++) {
+      if (a < D) {
+        sc.blo[a] = -2;
+        sc.bnl[a] = (cfgs[k].res[a] + 3) / 4 + 5;
+      } else {
+        sc.blo[a] = 0;
+        sc.bnl[a] = 1;
+      }
     }
+    sc.zoff = s->bn[D - 1] == 1 && k == 0 ? 0 : s->bn[D - 1];
+    for (int a = 0; a < D - 1; a++) s->bn[a] = std::max(k == 0 ? 1 : s->bn[a], sc.bnl[a]);
+    s->bn[D - 1] = sc.zoff + sc.bnl[D - 1];
   }
   int64_t tb = (int64_t)s->bn[0] * s->bn[1] * s->bn[2];
   if (tb >= (1 << (8 * kMaxRadixPasses)) - 1) {
@@ -412,11 +437,23 @@ int mpm_create(const mpm_config* cfg, mpm_solver** out) {
       (r = dalloc(&s->pchunk_cnt, 2 * tb)) || (r = dalloc(&s->active, tb)) ||
       (r = dalloc(&s->tile_prefix, nparts)) ||
       (r = dalloc(&s->grid, tb * NPB)) || (r = dalloc(&s->totals, 32)) ||
-      (r = dalloc(&s->err, 1)) || (r = dalloc(&s->maxv, 16 * s->persistent_groups))) {
+      (r = dalloc(&s->err, 1)) || (r = dalloc(&s->maxv, 16 * s->persistent_groups)) ||
+      (r = dalloc(&s->d_params, nscenes)) || (r = dalloc(&s->d_slab, s->bn[D - 1]))) {
     mpm_destroy(s);
     return r;
   }
   s->maxv_cap = 16 * s->persistent_groups;
+  if (hipHostMalloc((void**)&s->h_params, sizeof(Params) * nscenes) != hipSuccess) {
+    s->h_params = nullptr;
+    mpm_destroy(s);
+    return fail(MPM_ENOMEM, "pinned host allocation failed");
+  }
+  {
+    std::vector<int> slab(s->bn[D - 1], 0);
+    for (int k = 0; k < nscenes; k++)
+      for (int z = 0; z < s->scenes[k].bnl[D - 1]; z++) slab[s->scenes[k].zoff + z] = k;
+    HIP_TRY(hipMemcpy(s->d_slab, slab.data(), sizeof(int) * slab.size(), hipMemcpyHostToDevice));
+  }
   if (hipMemsetAsync(s->count, 0, sizeof(int) * (tb + 1), s->stream) != hipSuccess ||
       hipMemsetAsync(s->count_next, 0, sizeof(int) * (tb + 1), s->stream) != hipSuccess ||
       hipMemsetAsync(s->lb_status_scan, 0, sizeof(unsigned long long) * 4 * nparts, s->stream) !=
@@ -432,6 +469,19 @@ int mpm_create(const mpm_config* cfg, mpm_solver** out) {
     return fail(MPM_EHIP, "initialising solver buffers failed");
   }
   *out = s;
+  return MPM_OK;
+}
+
+int mpm_create(const mpm_config* cfg, mpm_solver** out) { return create_impl(cfg, 1, out); }
+
+int mpm_create_batch(const mpm_config* cfgs, int32_t nscenes, mpm_solver** out) {
+  return create_impl(cfgs, nscenes, out);
+}
+
+int mpm_set_active_scene(mpm_solver* s, int32_t scene) {
+  if (!s) return fail(MPM_EINVAL, "null argument");
+  if (scene < 0 || scene >= (int)s->scenes.size()) return fail(MPM_EINVAL, "no such scene");
+  s->active_scene = scene;
   return MPM_OK;
 }
 
@@ -454,6 +504,9 @@ int mpm_destroy(mpm_solver* s) {
   dfree(s->totals);
   dfree(s->err);
   dfree(s->maxv);
+  dfree(s->d_params);
+  dfree(s->d_slab);
+  if (s->h_params) hipHostFree(s->h_params);
   if (s->stream) hipStreamDestroy(s->stream);
   delete s;
   return MPM_OK;
@@ -461,17 +514,18 @@ int mpm_destroy(mpm_solver* s) {
 
 int mpm_set_gravity(mpm_solver* s, const double* g) {
   if (!s || !g) return fail(MPM_EINVAL, "null argument");
-  for (int a = 0; a < s->D; a++) s->gravity[a] = g[a];
+  for (int a = 0; a < s->D; a++) s->scenes[s->active_scene].gravity[a] = g[a];
   return MPM_OK;
 }
 
 int mpm_add_bounding_box(mpm_solver* s, int32_t unbounded) {
   if (!s) return fail(MPM_EINVAL, "null argument");
   if (unbounded) return fail(MPM_EINVAL, "unbounded simulations are not supported");
-  if (s->post.size() >= (size_t)kMaxPost) return fail(MPM_EINVAL, "too many grid post-processes");
+  std::vector<PostOp>& post = s->scenes[s->active_scene].post;
+  if (post.size() >= (size_t)kMaxPost) return fail(MPM_EINVAL, "too many grid post-processes");
   PostOp op{};
   op.kind = kPostBBox;
-  s->post.push_back(op);
+  post.push_back(op);
   return MPM_OK;
 }
 
@@ -481,7 +535,8 @@ int mpm_add_surface_collider(mpm_solver* s, const double* point, const double* n
   if (surface < 0 || surface > 2) return fail(MPM_EINVAL, "unknown surface type");
   if (surface == MPM_STICKY && friction != 0)
     return fail(MPM_EINVAL, "friction must be 0 on sticky surfaces.");
-  if (s->post.size() >= (size_t)kMaxPost) return fail(MPM_EINVAL, "too many grid post-processes");
+  std::vector<PostOp>& post = s->scenes[s->active_scene].post;
+  if (post.size() >= (size_t)kMaxPost) return fail(MPM_EINVAL, "too many grid post-processes");
   double n2 = 0;
   for (int a = 0; a < s->D; a++) n2 += normal[a] * normal[a];
   if (!(n2 > 0)) return fail(MPM_EINVAL, "collider normal must be non-zero");
@@ -494,26 +549,27 @@ int mpm_add_surface_collider(mpm_solver* s, const double* point, const double* n
     op.point[a] = (float)point[a];
     op.normal[a] = (float)(scale * normal[a]);
   }
-  s->post.push_back(op);
+  post.push_back(op);
   return MPM_OK;
 }
 
 int mpm_add_sphere_collider(mpm_solver* s, const double* center, double radius, int32_t surface) {
   if (!s || !center) return fail(MPM_EINVAL, "null argument");
   if (surface < 0 || surface > 2) return fail(MPM_EINVAL, "unknown surface type");
-  if (s->post.size() >= (size_t)kMaxPost) return fail(MPM_EINVAL, "too many grid post-processes");
+  std::vector<PostOp>& post = s->scenes[s->active_scene].post;
+  if (post.size() >= (size_t)kMaxPost) return fail(MPM_EINVAL, "too many grid post-processes");
   PostOp op{};
   op.kind = kPostSphere;
   op.surface = surface;
   op.radius = (float)radius;
   for (int a = 0; a < s->D; a++) op.point[a] = (float)center[a];
-  s->post.push_back(op);
+  post.push_back(op);
   return MPM_OK;
 }
 
 int mpm_clear_grid_postprocess(mpm_solver* s) {
   if (!s) return fail(MPM_EINVAL, "null argument");
-  s->post.clear();
+  s->scenes[s->active_scene].post.clear();
   return MPM_OK;
 }
 
@@ -541,9 +597,9 @@ int mpm_add_particles(mpm_solver* s, int64_t k, const float* x, const float* v,
   SubstepBuffers B;
   fill_buffers(s, B);
   if (D == 3)
-    launch_append<3>(B, dx, dv, dm, (int)k, (int)s->n, s->stream);
+    launch_append<3>(B, dx, dv, dm, (int)k, (int)s->n, s->active_scene, s->stream);
   else
-    launch_append<2>(B, dx, dv, dm, (int)k, (int)s->n, s->stream);
+    launch_append<2>(B, dx, dv, dm, (int)k, (int)s->n, s->active_scene, s->stream);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(s->stream));
   dfree(dx);
@@ -562,9 +618,7 @@ int mpm_num_particles(mpm_solver* s, int64_t* n) {
   return MPM_OK;
 }
 
-int mpm_substeps(mpm_solver* s, int32_t nsub, float dt, double t0) {
-  (void)t0;  // the built-in post-processes do not depend on t
-  if (!s) return fail(MPM_EINVAL, "null argument");
+static int substeps_impl(mpm_solver* s, int32_t nsub, const float* dts) {
   if (nsub < 0) return fail(MPM_EINVAL, "negative substep count");
   HIP_TRY(hipSetDevice(s->device));
   if ((int64_t)nsub * s->persistent_groups > s->maxv_cap) {
@@ -576,7 +630,12 @@ int mpm_substeps(mpm_solver* s, int32_t nsub, float dt, double t0) {
   launch_fill_u32(s->maxv, 0u, (int64_t)std::max(nsub, 1) * s->persistent_groups, s->stream);
   s->last_batch = nsub;
   if (s->n == 0) return MPM_OK;
-  fill_params(s, dt);
+  // scene table: the staging buffer is reused only once the last upload is done
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  for (size_t k = 0; k < s->scenes.size(); k++)
+    fill_params(s->h_params[k], s, s->scenes[k], dts[k]);
+  HIP_TRY(hipMemcpyAsync(s->d_params, s->h_params, sizeof(Params) * s->scenes.size(),
+                         hipMemcpyHostToDevice, s->stream));
   Timer* tm = s->timing ? &s->timer : nullptr;
   for (int k = 0; k < nsub; k++) {
     SubstepBuffers B;
@@ -593,9 +652,9 @@ int mpm_substeps(mpm_solver* s, int32_t nsub, float dt, double t0) {
       fill_buffers(s, B);
     }
     if (s->D == 3)
-      launch_substep<3>(s->P, B, k, s->debug & 1, s->stream, tm);
+      launch_substep<3>(s->d_params, s->d_slab, B, k, s->debug & 1, s->stream, tm);
     else
-      launch_substep<2>(s->P, B, k, s->debug & 1, s->stream, tm);
+      launch_substep<2>(s->d_params, s->d_slab, B, k, s->debug & 1, s->stream, tm);
     take_buffers(s, B);
     s->substep_counter++;
     s->have_perm = true;
@@ -604,6 +663,18 @@ int mpm_substeps(mpm_solver* s, int32_t nsub, float dt, double t0) {
   }
   HIP_TRY(hipGetLastError());
   return MPM_OK;
+}
+
+int mpm_substeps(mpm_solver* s, int32_t nsub, float dt, double t0) {
+  (void)t0;  // the built-in post-processes do not depend on t
+  if (!s) return fail(MPM_EINVAL, "null argument");
+  std::vector<float> dts(s->scenes.size(), dt);
+  return substeps_impl(s, nsub, dts.data());
+}
+
+int mpm_substeps_batch(mpm_solver* s, int32_t nsub, const float* dts) {
+  if (!s || !dts) return fail(MPM_EINVAL, "null argument");
+  return substeps_impl(s, nsub, dts);
 }
 
 int mpm_read_max_velocity(mpm_solver* s, int32_t nsub, float* out) {
@@ -739,12 +810,14 @@ int mpm_read_grid(mpm_solver* s, const int32_t* lo, const int32_t* nn, float* m,
         float mv = 0, vv[3] = {0, 0, 0};
         int lb[3], l[3];
         bool ok = true;
+        const SceneHost& sc = s->scenes[s->active_scene];
         for (int a = 0; a < D; a++) {
           int b = I[a] >> 2;
           l[a] = I[a] - 4 * b;
-          lb[a] = b - s->blo[a];
-          if (lb[a] < 0 || lb[a] >= s->bn[a]) ok = false;
+          lb[a] = b - sc.blo[a];
+          if (lb[a] < 0 || lb[a] >= sc.bnl[a]) ok = false;
         }
+        lb[D - 1] += sc.zoff;
         if (ok) {
           const int64_t bid = D == 3 ? ((int64_t)lb[2] * s->bn[1] + lb[1]) * s->bn[0] + lb[0]
                                      : (int64_t)lb[1] * s->bn[0] + lb[0];
