@@ -127,7 +127,11 @@ int mpm_num_particles(mpm_solver* s, int64_t* n);
  * the post-process of the first substep.  Asynchronous. */
 int mpm_substeps(mpm_solver* s, int32_t n, float dt, double t0);
 /* Batched form of mpm_substeps: one substep of every scene per iteration,
- * scene k with time step dts[k]. */
+ * scene k with time step dts[k].  dts[k] == 0 means scene k sits these
+ * substeps out: its particles and grid are left untouched, which lets scenes
+ * with different substep counts per frame share one batch.  The maximum
+ * velocity of mpm_read_max_velocity is then taken over the scenes that
+ * moved. */
 int mpm_substeps_batch(mpm_solver* s, int32_t n, const float* dts);
 /* Per-substep max over particles of max_i |v_i| (compute_max_velocity,
  * mpm_solver.py:726-735) for the substeps of the last mpm_substeps call;

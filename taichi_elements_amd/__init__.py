@@ -5,5 +5,6 @@
 kernels of ``libmpm_hip.so`` (see include/mpm_abi.h).
 """
 from .mpm_solver import MPMSolver  # noqa: F401
+from .batch import MPMSolverBatch  # noqa: F401
 
-__all__ = ["MPMSolver"]
+__all__ = ["MPMSolver", "MPMSolverBatch"]

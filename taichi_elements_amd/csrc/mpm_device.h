@@ -54,6 +54,7 @@ struct Params {
   int bnl[3];           // extent of the scene's block window (in blocks)
   int bn[3];            // extent of the combined window of all scenes (in blocks)
   int zoff;             // the scene's first block along the stacking axis (last axis)
+  int skip;             // 1: the scene sits this substep out (state and grid untouched)
   int n_post;
   PostOp post[kMaxPost];
 };

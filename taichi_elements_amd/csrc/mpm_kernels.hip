@@ -520,6 +520,7 @@ __global__ __launch_bounds__(kThreads) void k_p2g(const Params* __restrict__ sce
     int lbc[3], b[3];
     decode_local<D>(ck.x, scenes[0], lbc);
     const Params& P = scenes[scene_of<D>(lbc, slab)];
+    if (P.skip) continue;  // workgroup-uniform: the whole chunk belongs to one scene
     to_local<D>(lbc, P, b);
 #pragma unroll
     for (int a = 0; a < D; a++) b[a] = (b[a] + P.blo[a]) * kBlock;  // first node of the block
@@ -749,6 +750,7 @@ __global__ __launch_bounds__(kThreads) void k_grid_op(const Params* __restrict__
     int lb[3];
     decode_local<D>(bid, scenes[0], lb);
     const Params& P = scenes[scene_of<D>(lb, slab)];
+    if (P.skip) continue;
     const int l[3] = {node & 3, (node >> 2) & 3, node >> 4};
     float m = 0.0f, p[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
@@ -828,8 +830,11 @@ __global__ __launch_bounds__(kThreads) void k_g2p(const Params* __restrict__ sce
     decode_local<D>(ck.x, scenes[0], lb);
     const Params& P = scenes[scene_of<D>(lb, slab)];
     to_local<D>(lb, P, lbl);
+    // a skipped scene keeps its particles (and so their blocks) unchanged; its
+    // grid was not rebuilt and is not read
+    const bool skip = P.skip != 0;
     for (int i = threadIdx.x; i < (kThreads / 64) * 32; i += kThreads) (&dwcnt[0][0])[i] = 0;
-    for (int i = threadIdx.x; i < TN; i += kThreads) {
+    for (int i = skip ? TN : threadIdx.x; i < TN; i += kThreads) {
       const int tl[3] = {i % kTile, (i / kTile) % kTile, i / (kTile * kTile)};
       int q[3], ln[3];
 #pragma unroll
@@ -872,7 +877,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p(const Params* __restrict__ sce
       // C = 4 inv_dx (P - v fx^T).  One pass over the 27 nodes, per-axis
       // partial sums instead of a 3x3 outer product per node.
       float nv[3] = {0.0f, 0.0f, 0.0f}, pe[3][3] = {{0.0f}}, nC[3][3];
-      if (inside) {
+      if (inside && !skip) {
 #pragma unroll
         for (int oz = 0; oz < (D == 3 ? 3 : 1); oz++) {
           float b0[3] = {0.0f, 0.0f, 0.0f}, bx[3] = {0.0f, 0.0f, 0.0f}, by[3] = {0.0f, 0.0f, 0.0f};
@@ -914,7 +919,8 @@ __global__ __launch_bounds__(kThreads) void k_g2p(const Params* __restrict__ sce
 #pragma unroll
         for (int e = 0; e < D; e++) nC[c][e] = P.c4 * (pe[c][e] - nv[c] * fx[e]);
       float xn[3] = {xp[0], xp[1], xp[2]};
-      if (mat != kStationary) {  // mpm_solver.py:722-724
+      if (skip) {
+      } else if (mat != kStationary) {  // mpm_solver.py:722-724
 #pragma unroll
         for (int a = 0; a < D; a++) {
           xn[a] = xp[a] + P.dt * nv[a];

@@ -217,7 +217,7 @@ int check_device_errors(mpm_solver* s) {
   return MPM_OK;
 }
 
-void fill_params(Params& P, const mpm_solver* s, const SceneHost& sc, float dt) {
+void fill_params(Params& P, const mpm_solver* s, const SceneHost& sc, float dt, int skip) {
   const mpm_config& c = sc.cfg;
   P.dx = (float)c.dx;
   P.inv_dx = (float)c.inv_dx;
@@ -240,6 +240,7 @@ void fill_params(Params& P, const mpm_solver* s, const SceneHost& sc, float dt) 
   }
   P.padding = c.padding;
   P.zoff = sc.zoff;
+  P.skip = skip;
   P.n_post = (int)sc.post.size();
   for (size_t k = 0; k < sc.post.size(); k++) P.post[k] = sc.post[k];
 }
@@ -618,8 +619,13 @@ int mpm_num_particles(mpm_solver* s, int64_t* n) {
   return MPM_OK;
 }
 
-static int substeps_impl(mpm_solver* s, int32_t nsub, const float* dts) {
+// dts[k] is scene k's time step; with batch set, a zero time step means the
+// scene sits the substeps out.
+static int substeps_impl(mpm_solver* s, int32_t nsub, const float* dts, bool batch) {
   if (nsub < 0) return fail(MPM_EINVAL, "negative substep count");
+  for (size_t k = 0; k < s->scenes.size(); k++)
+    if (batch && !(std::isfinite(dts[k]) && dts[k] >= 0.0f))
+      return fail(MPM_EINVAL, "time steps of a batch must be finite and >= 0");
   HIP_TRY(hipSetDevice(s->device));
   if ((int64_t)nsub * s->persistent_groups > s->maxv_cap) {
     HIP_TRY(hipStreamSynchronize(s->stream));
@@ -633,7 +639,7 @@ static int substeps_impl(mpm_solver* s, int32_t nsub, const float* dts) {
   // scene table: the staging buffer is reused only once the last upload is done
   HIP_TRY(hipStreamSynchronize(s->stream));
   for (size_t k = 0; k < s->scenes.size(); k++)
-    fill_params(s->h_params[k], s, s->scenes[k], dts[k]);
+    fill_params(s->h_params[k], s, s->scenes[k], dts[k], batch && dts[k] == 0.0f);
   HIP_TRY(hipMemcpyAsync(s->d_params, s->h_params, sizeof(Params) * s->scenes.size(),
                          hipMemcpyHostToDevice, s->stream));
   Timer* tm = s->timing ? &s->timer : nullptr;
@@ -669,12 +675,12 @@ int mpm_substeps(mpm_solver* s, int32_t nsub, float dt, double t0) {
   (void)t0;  // the built-in post-processes do not depend on t
   if (!s) return fail(MPM_EINVAL, "null argument");
   std::vector<float> dts(s->scenes.size(), dt);
-  return substeps_impl(s, nsub, dts.data());
+  return substeps_impl(s, nsub, dts.data(), false);
 }
 
 int mpm_substeps_batch(mpm_solver* s, int32_t nsub, const float* dts) {
   if (!s || !dts) return fail(MPM_EINVAL, "null argument");
-  return substeps_impl(s, nsub, dts);
+  return substeps_impl(s, nsub, dts, true);
 }
 
 int mpm_read_max_velocity(mpm_solver* s, int32_t nsub, float* out) {

@@ -158,23 +158,33 @@ class MPMSolver:
         self._source_velocity = [0.0] * self.dim
         self._gravity = [0.0] * self.dim
         self._handle = None
-        self._create_handle()
-
         self.n_particles = _Field0D(self._num_particles)
         self.gravity = _Field0D(lambda: tuple(self._gravity), self.set_gravity)
         self.grid_postprocess = []
+        self.writers = []
+        if self._owns_handle:
+            self._create_handle()
+            self._init_scene()
+
+    # A solver that shares the handle of a batch (taichi_elements_amd.batch)
+    # sets this to False and is initialised by the batch.
+    _owns_handle = True
+
+    def _init_scene(self):
+        """Gravity and bounding box of a new solver (mpm_solver.py:283-302)."""
         if self.dim == 2:
             self.set_gravity((0, -9.8))
         else:
             self.set_gravity((0, -9.8, 0))
         self.add_bounding_box(self.unbounded)
-        self.writers = []
 
     # ------------------------------------------------------------------ #
     # handle management
     # ------------------------------------------------------------------ #
-    def _create_handle(self):
-        L = _lib.lib()
+    def _h(self):
+        return self._handle
+
+    def _make_config(self):
         cfg = _lib.MpmConfig()
         cfg.dim = self.dim
         for a in range(3):
@@ -186,21 +196,26 @@ class MPMSolver:
         cfg.dx, cfg.inv_dx, cfg.p_vol, cfg.p_mass = self.dx, self.inv_dx, self.p_vol, self.p_mass
         cfg.mu_0, cfg.lambda_0, cfg.alpha = self.mu_0, self.lambda_0, self.alpha
         cfg.water_density = self.water_density
+        return cfg
+
+    def _create_handle(self):
+        L = _lib.lib()
+        cfg = self._make_config()
         h = _lib._P()
         check(L.mpm_create(cfg, h))
         self._handle = h
         if self._kernel_profiler:
-            check(L.mpm_set_timing(self._handle, 1))
+            check(L.mpm_set_timing(self._h(), 1))
 
     def __del__(self):
         h = getattr(self, "_handle", None)
-        if h is not None and _lib._lib is not None:
+        if h is not None and self._owns_handle and _lib._lib is not None:
             _lib._lib.mpm_destroy(h)
             self._handle = None
 
     def _num_particles(self):
         n = _lib.ctypes.c_int64()
-        check(_lib.lib().mpm_num_particles(self._handle, n))
+        check(_lib.lib().mpm_num_particles(self._h(), n))
         return int(n.value)
 
     # ------------------------------------------------------------------ #
@@ -210,17 +225,17 @@ class MPMSolver:
         assert isinstance(g, (tuple, list))
         assert len(g) == self.dim
         self._gravity = [float(v) for v in g]
-        check(_lib.lib().mpm_set_gravity(self._handle, _lib.dvec(self._gravity)))
+        check(_lib.lib().mpm_set_gravity(self._h(), _lib.dvec(self._gravity)))
 
     def add_bounding_box(self, unbounded):
         if unbounded:
             raise NotImplementedError("unbounded simulations are not supported")
-        check(_lib.lib().mpm_add_bounding_box(self._handle, 0))
+        check(_lib.lib().mpm_add_bounding_box(self._h(), 0))
         self.grid_postprocess.append(("bounding_box", {}))
 
     def add_sphere_collider(self, center, radius, surface=surface_sticky):
         center = list(center)
-        check(_lib.lib().mpm_add_sphere_collider(self._handle, _lib.dvec(center), float(radius),
+        check(_lib.lib().mpm_add_sphere_collider(self._h(), _lib.dvec(center), float(radius),
                                                  int(surface)))
         self.grid_postprocess.append(("sphere", dict(center=center, radius=radius,
                                                      surface=surface)))
@@ -231,14 +246,14 @@ class MPMSolver:
         normal = list(normal_scale * x for x in normal)
         if surface == self.surface_sticky and friction != 0:
             raise ValueError('friction must be 0 on sticky surfaces.')
-        check(_lib.lib().mpm_add_surface_collider(self._handle, _lib.dvec(point),
+        check(_lib.lib().mpm_add_surface_collider(self._h(), _lib.dvec(point),
                                                   _lib.dvec(normal), int(surface),
                                                   float(friction)))
         self.grid_postprocess.append(("surface", dict(point=point, normal=normal,
                                                       surface=surface, friction=friction)))
 
     def clear_grid_postprocess(self):
-        check(_lib.lib().mpm_clear_grid_postprocess(self._handle))
+        check(_lib.lib().mpm_clear_grid_postprocess(self._h()))
         self.grid_postprocess.clear()
 
     # ------------------------------------------------------------------ #
@@ -261,7 +276,7 @@ class MPMSolver:
         mat = _lib.as_i32(np.broadcast_to(np.asarray(material, np.int32), (n,)), (n,))
         col = _lib.as_i32(np.broadcast_to(np.asarray(color, np.int64).astype(np.int32), (n,)),
                           (n,))
-        check(_lib.lib().mpm_add_particles(self._handle, n, _lib.fptr(x), _lib.fptr(v),
+        check(_lib.lib().mpm_add_particles(self._h(), n, _lib.fptr(x), _lib.fptr(v),
                                            _lib.iptr(mat), _lib.iptr(col)))
         self._emitter_ids = np.concatenate(
             [self._emitter_ids, np.full(n, emitter_id, np.int32)])
@@ -344,10 +359,10 @@ class MPMSolver:
     def clear_particles(self):
         """mpm_solver.py:1193-1196: drop every particle (keeps the settings)."""
         post = list(self.grid_postprocess)
-        _lib.lib().mpm_destroy(self._handle)
+        _lib.lib().mpm_destroy(self._h())
         self._handle = None
         self._create_handle()
-        check(_lib.lib().mpm_set_gravity(self._handle, _lib.dvec(self._gravity)))
+        check(_lib.lib().mpm_set_gravity(self._h(), _lib.dvec(self._gravity)))
         self.grid_postprocess = []
         for kind, kw in post:
             if kind == "bounding_box":
@@ -411,9 +426,9 @@ class MPMSolver:
 
     def _run_substeps(self, count, dt):
         L = _lib.lib()
-        check(L.mpm_substeps(self._handle, int(count), float(dt), float(self.t)))
+        check(L.mpm_substeps(self._h(), int(count), float(dt), float(self.t)))
         out = np.zeros(max(count, 1), np.float32)
-        check(L.mpm_read_max_velocity(self._handle, int(count), _lib.fptr(out)))
+        check(L.mpm_read_max_velocity(self._h(), int(count), _lib.fptr(out)))
         return out[:count]
 
     def substep(self, dt):
@@ -442,7 +457,7 @@ class MPMSolver:
         counts = (_lib.ctypes.c_int32 * cap)()
         ms = (_lib.ctypes.c_double * cap)()
         nk = _lib.ctypes.c_int32()
-        check(L.mpm_kernel_times(self._handle, names, 4096, counts, ms, cap, nk))
+        check(L.mpm_kernel_times(self._h(), names, 4096, counts, ms, cap, nk))
         nm = names.value.decode().split("\n")[:nk.value]
         return nm, list(counts[:nk.value]), list(ms[:nk.value])
 
@@ -456,7 +471,7 @@ class MPMSolver:
         np_material = np.empty((n, ), dtype=np.int32)
         np_color = np.empty((n, ), dtype=np.int32)
         if n:
-            check(_lib.lib().mpm_particle_info(self._handle, _lib.fptr(np_x), _lib.fptr(np_v),
+            check(_lib.lib().mpm_particle_info(self._h(), _lib.fptr(np_x), _lib.fptr(np_v),
                                                _lib.iptr(np_material), _lib.iptr(np_color)))
         particles_data = {
             'position': np_x,
@@ -483,7 +498,7 @@ class MPMSolver:
             dt = np.int32 if name in ("material", "color") else np.float32
             a = np.empty(self._shape(rank, n), dt)
             if n:
-                check(_lib.lib().mpm_get_state(self._handle, fid, a.ctypes.data))
+                check(_lib.lib().mpm_get_state(self._h(), fid, a.ctypes.data))
             out[name] = a
         return out
 
@@ -495,7 +510,7 @@ class MPMSolver:
             dt = np.int32 if name in ("material", "color") else np.float32
             a = np.ascontiguousarray(np.asarray(value, dt).reshape(self._shape(rank, n)))
             if n:
-                check(_lib.lib().mpm_set_state(self._handle, fid, a.ctypes.data))
+                check(_lib.lib().mpm_set_state(self._h(), fid, a.ctypes.data))
 
     def read_grid(self, lo=None, n=None):
         """Grid mass and velocity after the last grid operator over the node
@@ -510,7 +525,7 @@ class MPMSolver:
         shape = tuple(int(v) for v in reversed(list(n)))
         m = np.zeros(shape, np.float32)
         v = np.zeros(shape + (self.dim, ), np.float32)
-        check(_lib.lib().mpm_read_grid(self._handle, lo3, n3, _lib.fptr(m), _lib.fptr(v)))
+        check(_lib.lib().mpm_read_grid(self._h(), lo3, n3, _lib.fptr(m), _lib.fptr(v)))
         return m, v
 
     def read_binning(self):
@@ -519,16 +534,16 @@ class MPMSolver:
         keys = np.zeros(n, np.int32)
         perm = np.zeros(n, np.int32)
         orig = np.zeros(n, np.int32)
-        check(_lib.lib().mpm_read_binning(self._handle, _lib.iptr(keys), _lib.iptr(perm),
+        check(_lib.lib().mpm_read_binning(self._h(), _lib.iptr(keys), _lib.iptr(perm),
                                           _lib.iptr(orig)))
         return keys, perm, orig
 
     def set_debug(self, flags):
-        check(_lib.lib().mpm_set_debug(self._handle, int(flags)))
+        check(_lib.lib().mpm_set_debug(self._h(), int(flags)))
 
     def last_counts(self):
         out = np.zeros(4, np.int64)
-        check(_lib.lib().mpm_last_counts(self._handle, out.ctypes.data_as(_lib._I64P)))
+        check(_lib.lib().mpm_last_counts(self._h(), out.ctypes.data_as(_lib._I64P)))
         return dict(valid=int(out[0]), chunks=int(out[1]), active_blocks=int(out[2]),
                     particle_blocks=int(out[3]))
 

@@ -1,0 +1,162 @@
+"""Batched scenes (taichi_elements_amd.MPMSolverBatch) on the GPU.
+
+A scene of a batch must evolve exactly as the same scene in its own
+MPMSolver: the batch shares launches, not arithmetic.  Every per-particle and
+per-node computation of a scene reads only that scene's constants, and the
+order in which a block's particles are summed is the scene's own insertion
+order in both cases, so the comparison is bit-exact (np.array_equal) on x, v,
+C, F, Jp and on the grid.  The single-scene path itself is checked against
+the oracle in test_gpu_parity.py; one scene of a batch is also compared with
+the oracle directly here.
+"""
+import numpy as np
+import pytest
+
+from tests.helpers import cube_particles, deformed_state, oracle_cfg_for, rel_err
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("x", "v", "C", "F", "Jp", "material", "color")
+
+
+@pytest.fixture(autouse=True)
+def _quiet():
+    from taichi_elements_amd import mpm_solver
+    mpm_solver.PRINT_PROGRESS = False
+
+
+def _scene_specs(dim):
+    """Uneven scenes: resolution, stiffness, particle count, colliders."""
+    return [
+        dict(res=(32, ) * dim, n=1500, seed=1, E_scale=1.0, gravity=-9.8, collider=None),
+        dict(res=(48, ) * dim, n=4000, seed=2, E_scale=0.5, gravity=-4.0, collider="sphere"),
+        dict(res=(64, ) * dim, n=6000, seed=3, E_scale=2.0, gravity=-9.8, collider="plane"),
+        dict(res=(40, ) * dim, n=700, seed=4, E_scale=1.0, gravity=0.0, collider=None),
+    ]
+
+
+def _particles(spec, dim):
+    rng = np.random.default_rng(spec["seed"])
+    n = spec["n"]
+    x = cube_particles(rng, n, [0.3] * dim, [0.35] * dim, dim)
+    mat = rng.integers(0, 4, n).astype(np.int32)
+    st = deformed_state(rng, x, mat, dim)
+    st["color"] = rng.integers(0, 1 << 24, n).astype(np.int32)
+    return st
+
+
+def _configure(s, spec, st):
+    dim = s.dim
+    g = [0.0] * dim
+    g[1] = spec["gravity"]
+    s.set_gravity(tuple(g))
+    if spec["collider"] == "sphere":
+        s.add_sphere_collider(center=[0.5] * dim, radius=0.12, surface=s.surface_slip)
+    elif spec["collider"] == "plane":
+        s.add_surface_collider(point=[0.0, 0.36] + [0.0] * (dim - 2),
+                               normal=[0.0, 1.0] + [0.0] * (dim - 2),
+                               surface=s.surface_separate, friction=0.3)
+    n = len(st["x"])
+    s.read_restart(n, st["x"], st["v"], st["material"], st["color"])
+    s.set_state(F=st["F"], C=st["C"], Jp=st["Jp"])
+
+
+def _solo(spec, st, dim):
+    from taichi_elements_amd import MPMSolver
+    s = MPMSolver(res=spec["res"], E_scale=spec["E_scale"])
+    _configure(s, spec, st)
+    return s
+
+
+def _batch(specs, states, dim, **kw):
+    from taichi_elements_amd import MPMSolverBatch
+    b = MPMSolverBatch([dict(res=sp["res"], E_scale=sp["E_scale"]) for sp in specs], **kw)
+    # fill the scenes out of order so that insertion indices interleave
+    for k in (2, 0, 3, 1):
+        _configure(b[k], specs[k], states[k])
+    return b
+
+
+def _assert_same(got, ref, label):
+    for f in FIELDS:
+        same = np.array_equal(got[f], ref[f])
+        if not same:
+            print(label, f, "rel err", rel_err(got[f], ref[f]), flush=True)
+        assert same, (label, f)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_batch_matches_separate_solvers_bit_exactly(dim):
+    specs = _scene_specs(dim)
+    states = [_particles(sp, dim) for sp in specs]
+    b = _batch(specs, states, dim)
+    assert b.n_particles == sum(sp["n"] for sp in specs)
+    nsub = 40  # crosses the periodic physical reorder of the particles
+    b.substeps(nsub)
+    got = [s.get_state() for s in b]
+    for k, sp in enumerate(specs):
+        solo = _solo(sp, states[k], dim)
+        for _ in range(nsub):
+            solo.substep(solo.default_dt)
+        _assert_same(got[k], solo.get_state(), f"dim={dim} scene={k}")
+        assert b[k].t == pytest.approx(solo.t, rel=1e-12)
+        m_b, v_b = b[k].read_grid()
+        m_s, v_s = solo.read_grid()
+        assert np.array_equal(m_b, m_s) and np.array_equal(v_b, v_s), f"grid of scene {k}"
+        info = b.particle_info()[k]
+        np.testing.assert_array_equal(info["position"], got[k]["x"])
+        np.testing.assert_array_equal(info["color"], states[k]["color"])
+
+
+def test_batch_step_lets_short_scenes_sit_out():
+    """step(frame_dt) applies each scene's own substep count; a scene that
+    needs fewer substeps must end exactly as its own solver's step()."""
+    dim = 3
+    specs = _scene_specs(dim)
+    states = [_particles(sp, dim) for sp in specs]
+    b = _batch(specs, states, dim)
+    frame_dt = 1 / 240
+    b.step(frame_dt)
+    counts = set()
+    for k, sp in enumerate(specs):
+        solo = _solo(sp, states[k], dim)
+        solo.step(frame_dt)
+        counts.add(solo.total_substeps)
+        assert b[k].total_substeps == solo.total_substeps
+        _assert_same(b[k].get_state(), solo.get_state(), f"step scene={k}")
+    assert len(counts) > 1  # the scenes really needed different substep counts
+
+
+def test_batch_scene_against_oracle(oracle):
+    """One substep of a batch from pre-deformed states, each scene against
+    the oracle (same tolerances as the single-scene parity tests)."""
+    dim = 3
+    specs = _scene_specs(dim)
+    states = [_particles(sp, dim) for sp in specs]
+    b = _batch(specs, states, dim)
+    b.substep()
+    for k, sp in enumerate(specs):
+        s = b[k]
+        cfg = oracle_cfg_for(oracle, s)
+        st = {f: states[k][f] for f in ("x", "v", "C", "F", "Jp", "material")}
+        ref, *_ = oracle.substep(cfg, st, s.default_dt, 0.0, np.float32)
+        got = s.get_state()
+        for f in ("x", "v", "F", "Jp"):
+            e = rel_err(got[f], ref[f])
+            assert e <= 1e-5, (k, f, e)
+        assert rel_err(got["C"], ref["C"]) <= 3e-5
+
+
+def test_batch_rejects_bad_time_steps():
+    from taichi_elements_amd import MPMSolverBatch
+    b = MPMSolverBatch([dict(res=(32, 32)), dict(res=(32, 32))])
+    b[0].add_cube((0.4, 0.4), (0.1, 0.1), b[0].material_water)
+    with pytest.raises(ValueError):
+        b.substeps(1, [1e-4, -1e-4])
+    with pytest.raises(ValueError):
+        b.substeps(1, [1e-4])
+    with pytest.raises(NotImplementedError):
+        b[0].substep(1e-4)
+    # a scene with no particles and a zero time step are both fine
+    b.substeps(2, [1e-4, 0.0])
+    assert b[0].total_substeps == 2 and b[1].total_substeps == 0
