@@ -76,6 +76,107 @@ def make_sparse_scene(seed=2, total=16_000_000, res=512):
     return x, np.zeros_like(x), m
 
 
+def make_batched_scenes(seed=3, nscenes=64):
+    """64 independent 3D scenes: N_k = round(exp(U[ln 1e4, ln 5e5])) particles,
+    res_k uniform in {64, 96, 128}, 1-4 cubes of side U[0.1, 0.3] with lower
+    corners U[0.05, 0.95 - side]^3 and one random material each (water /
+    elastic / snow / sand), v ~ U[-1, 1]^3."""
+    rng = np.random.default_rng(seed)
+    scenes = []
+    for _ in range(nscenes):
+        n = int(round(np.exp(rng.uniform(np.log(1e4), np.log(5e5)))))
+        res = int(rng.choice([64, 96, 128]))
+        ncubes = int(rng.integers(1, 5))
+        split = rng.multinomial(n, [1.0 / ncubes] * ncubes)
+        xs, ms = [], []
+        for c in range(ncubes):
+            side = rng.uniform(0.1, 0.3)
+            lo = rng.uniform(0.05, 0.95 - side, 3)
+            xs.append((lo + rng.random((split[c], 3)) * side).astype(np.float32))
+            ms.append(np.full(split[c], rng.integers(0, 4), np.int32))
+        v = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
+        scenes.append(dict(res=res, x=np.concatenate(xs), v=v, m=np.concatenate(ms)))
+    return scenes
+
+
+def batched_bench(steps, warmup):
+    """The 64-scene batch advanced by one engine (every stage one launch over
+    all scenes) against a loop of plain solves of the same scenes, one
+    MPMSolver per scene.  Both advance every scene by `steps` substeps of its
+    own default_dt from the same state."""
+    from taichi_elements_amd import MPMSolver, MPMSolverBatch
+    from taichi_elements_amd._lib import check, fptr
+    from taichi_elements_amd._lib import lib as _l
+    L = _l()
+    specs = make_batched_scenes()
+    total = sum(len(sp["x"]) for sp in specs)
+    b = MPMSolverBatch([dict(res=(sp["res"], ) * 3) for sp in specs])
+    for sc, sp in zip(b, specs):
+        sc.read_restart(len(sp["x"]), sp["x"], sp["v"], sp["m"], np.zeros(len(sp["x"]), np.int32))
+    dts = np.ascontiguousarray(np.asarray(b.default_dts(), np.float32))
+    solos = []
+    for sp in specs:
+        s = MPMSolver(res=(sp["res"], ) * 3)
+        s.read_restart(len(sp["x"]), sp["x"], sp["v"], sp["m"], np.zeros(len(sp["x"]), np.int32))
+        solos.append(s)
+    print(f"[bench] batched: {len(specs)} scenes, {total} particles", file=sys.stderr, flush=True)
+
+    def run_batch(k):
+        check(L.mpm_substeps_batch(b._handle, k, fptr(dts)))
+        check(L.mpm_sync(b._handle))
+
+    def run_loop(k):
+        # one scene after the other, each solve finished before the next starts
+        for s in solos:
+            check(L.mpm_substeps(s._handle, k, s.default_dt, 0.0))
+            check(L.mpm_sync(s._handle))
+
+    def run_loop_async(k):
+        # every solve enqueued on its own stream, one wait at the end
+        for s in solos:
+            check(L.mpm_substeps(s._handle, k, s.default_dt, 0.0))
+        for s in solos:
+            check(L.mpm_sync(s._handle))
+
+    res = {}
+    identical = None
+    for name, fn in (("batch", run_batch), ("loop", run_loop), ("loop_async", run_loop_async)):
+        fn(warmup)
+        t0 = time.perf_counter()
+        fn(steps)
+        res[name] = time.perf_counter() - t0
+        if name == "loop":
+            # batch and loop have now run the same substeps from the same state
+            got = b.particle_info()
+            identical = all(np.array_equal(g["position"], s.particle_info()["position"]) and
+                            np.array_equal(g["velocity"], s.particle_info()["velocity"])
+                            for g, s in zip(got, solos))
+    check(L.mpm_set_timing(b._handle, 1))
+    run_batch(steps)
+    names, calls, ms = b.kernel_times()
+    check(L.mpm_set_timing(b._handle, 0))
+    kern = {nm: round(1000 * t / max(c, 1), 2) for nm, c, t in zip(names, calls, ms)}
+    counts = b.last_counts()
+    out = dict(scenes=len(specs), particles=total, substeps=steps,
+               kernels_us=kern, active_grid_blocks=counts["active_blocks"],
+               chunks=counts["chunks"],
+               value=total * steps / res["batch"],
+               ms_per_substep=1000 * res["batch"] / steps,
+               loop_value=total * steps / res["loop"],
+               loop_ms_per_substep=1000 * res["loop"] / steps,
+               loop_async_value=total * steps / res["loop_async"],
+               speedup_vs_loop=res["loop"] / res["batch"],
+               speedup_vs_loop_async=res["loop_async"] / res["batch"],
+               identical_to_loop=identical,
+               unit="particle-substeps/s",
+               workload="64 independent 3D scenes, N log-uniform in [1e4, 5e5], res in "
+                        "{64, 96, 128}, 1-4 random cubes of random material, per-scene "
+                        "default_dt")
+    b.close()
+    del solos
+    return out
+
+
 def algorithmic_bytes(n_particles, n_nodes):
     """Bytes a substep must move in the reference's field layout (float32):
     per particle 16 (binning) + 148 (p2g) + 80 (g2p), per active grid node
@@ -141,6 +242,8 @@ def main():
                     help="dense1m: the 3D four-material 128^3 workload (default, the "
                          "headline line); sparse16m: 512^3 window, 16M particles in 64 "
                          "ellipsoids with a plane collider")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the 64-scene batched measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device-to-device copy (hipMemcpy) so that "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE can be scaled by a kernel of "
@@ -236,6 +339,9 @@ def main():
     }
     if args.copy_calibration:
         out["copy_calibration_bytes"] = copy_calibration()
+    if not args.no_batched and args.config == "dense1m":
+        del s
+        out["batched"] = batched_bench(args.steps, args.warmup)
     if not args.no_cpu_baseline and args.config == "dense1m":
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(x, v, m, dt, args.cpu_seconds)
