@@ -187,6 +187,15 @@ int mpm_host_svd(int32_t dim, const float* A, float* U, float* sig, float* V);
  * work items (chunks), active grid blocks, and particle blocks. */
 int mpm_last_counts(mpm_solver* s, int64_t* out4);
 
+/* The quantised particle file of ParticleIO.write_particles
+ * (engine/particle_io.py:12-76), computed on the GPU: ranges (2, dim, 2)
+ * float32 = [x | v][axis][min, max] with too-narrow ranges widened to
+ * min + 1e-5; x_and_v (n, dim) uint32 = (x quantised to 24 bits << 8) +
+ * (v quantised to 8 bits); color (n, 3) uint8 = R, G, B of the 0xRRGGBB
+ * colour.  Insertion order, every particle of the handle (all scenes of a
+ * batch).  No particles is MPM_EINVAL (the reference's np.min raises). */
+int mpm_quantize_particles(mpm_solver* s, float* ranges, uint32_t* x_and_v, uint8_t* color);
+
 #ifdef __cplusplus
 }
 #endif

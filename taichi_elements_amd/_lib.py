@@ -60,6 +60,8 @@ SIGNATURES = [
                                         ctypes.POINTER(ctypes.c_double), ctypes.c_int32, _I32P]),
     ("mpm_last_counts", ctypes.c_int, [_P, _I64P]),
     ("mpm_host_svd", ctypes.c_int, [ctypes.c_int32, _FP, _FP, _FP, _FP]),
+    ("mpm_quantize_particles", ctypes.c_int, [_P, _FP, ctypes.POINTER(ctypes.c_uint32),
+                                              ctypes.POINTER(ctypes.c_uint8)]),
 ]
 
 _lib = None

@@ -111,6 +111,16 @@ void launch_scatter_i(const int* src, int64_t cap, int ncomp, const int* orig, i
                       hipStream_t st);
 void launch_fill_u32(unsigned* p, unsigned v, int64_t n, hipStream_t st);
 
+// Quantised particle output (mpm_io.hip).  Components 0..D-1 are x, D..2D-1 v.
+struct QuantRanges {
+  float lo[6];
+  float inv[6];  // 1 / (hi - lo), rounded to float32
+};
+void launch_minmax(int D, const float* pf, int64_t cap, int n, float* partial, int groups,
+                   hipStream_t st);
+void launch_quantize(int D, const float* pf, const int* orig, int64_t cap, int n,
+                     const QuantRanges& q, unsigned* out, hipStream_t st);
+
 // Event-bracketed kernel timing (enabled on demand).
 class Timer {
  public:

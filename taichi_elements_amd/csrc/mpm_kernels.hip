@@ -741,7 +741,7 @@ __global__ __launch_bounds__(kThreads) void k_grid_op(const Params* __restrict__
                                                       const int* __restrict__ pchunk_cnt,
                                                       const float4* __restrict__ tiles,
                                                       float4* __restrict__ grid, int raw) {
-  constexpr int NPB = Geo<D>::kNodesPerBlock, TN = Geo<D>::kTileNodes, NC = D + 1;
+  constexpr int NPB = Geo<D>::kNodesPerBlock, TN = Geo<D>::kTileNodes;
   const int64_t total = (int64_t)totals[kTotActive] * NPB;
   for (int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * kThreads) {
@@ -816,7 +816,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p(const Params* __restrict__ sce
                                                   int* __restrict__ count_next,
                                                   int* __restrict__ err) {
   using L = Layout<D>;
-  constexpr int NPB = Geo<D>::kNodesPerBlock, TN = Geo<D>::kTileNodes, NC = D + 1;
+  constexpr int NPB = Geo<D>::kNodesPerBlock, TN = Geo<D>::kTileNodes;
   __shared__ float4 tile[TN];
   __shared__ float red[kThreads / 64];
   __shared__ int dwcnt[kThreads / 64][32];

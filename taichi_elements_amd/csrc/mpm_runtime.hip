@@ -924,4 +924,61 @@ int mpm_last_counts(mpm_solver* s, int64_t* out4) {
   return MPM_OK;
 }
 
+int mpm_quantize_particles(mpm_solver* s, float* ranges, uint32_t* x_and_v, uint8_t* color) {
+  if (!s || !ranges || !x_and_v || !color) return fail(MPM_EINVAL, "null argument");
+  // np.min over no particles raises ValueError (particle_io.py:42)
+  if (s->n == 0) return fail(MPM_EINVAL, "no particles to write");
+  TRY(check_device_errors(s));
+  HIP_TRY(hipSetDevice(s->device));
+  const int D = s->D, NC = 2 * D, n = (int)s->n;
+  const int groups = (int)std::min<int64_t>((s->n + kThreads - 1) / kThreads, 1024);
+  float* partial = nullptr;
+  unsigned* out = nullptr;
+  HIP_TRY(hipMalloc((void**)&partial, sizeof(float) * groups * NC * 2));
+  if (hipMalloc((void**)&out, sizeof(unsigned) * s->n * D) != hipSuccess) {
+    hipFree(partial);
+    return fail(MPM_ENOMEM, "device allocation for the quantised particles failed");
+  }
+  std::vector<float> part((size_t)groups * NC * 2);
+  launch_minmax(D, s->pf, s->cap, n, partial, groups, s->stream);
+  hipError_t e = hipMemcpyAsync(part.data(), partial, sizeof(float) * part.size(),
+                                hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  QuantRanges q{};
+  if (e == hipSuccess) {
+    for (int c = 0; c < NC; c++) {
+      float lo = part[c * 2], hi = part[c * 2 + 1];
+      for (int g = 1; g < groups; g++) {
+        lo = std::min(lo, part[((size_t)g * NC + c) * 2]);
+        hi = std::max(hi, part[((size_t)g * NC + c) * 2 + 1]);
+      }
+      // max(lo + 1e-5, hi) with the float32 lo promoted to a float64 sum and
+      // the result stored back into the float32 array (particle_io.py:45-47)
+      const double widened = (double)lo + 1e-5;
+      if (widened > (double)hi) hi = (float)widened;
+      const int kind = c < D ? 0 : 1, d = c < D ? c : c - D;
+      ranges[(kind * D + d) * 2 + 0] = lo;
+      ranges[(kind * D + d) * 2 + 1] = hi;
+      q.lo[c] = lo;
+      const float width = hi - lo;  // float32 subtraction
+      q.inv[c] = (float)(1.0 / (double)width);  // correctly rounded 1 / width
+    }
+    launch_quantize(D, s->pf, s->pi + s->cap /* Layout ORIG == 1 */, s->cap, n, q, out, s->stream);
+    e = hipMemcpyAsync(x_and_v, out, sizeof(unsigned) * s->n * D, hipMemcpyDeviceToHost,
+                       s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  }
+  hipFree(partial);
+  hipFree(out);
+  if (e != hipSuccess) return fail(MPM_EHIP, std::string("quantising particles: ") + hipGetErrorString(e));
+  // colours live on the host: int32 -> float32 -> uint32 as the reference's
+  // float32 staging array does (particle_io.py:62-74), then R, G, B bytes
+  for (int64_t i = 0; i < s->n; i++) {
+    const float f = (float)s->color[i];
+    const uint32_t u = (uint32_t)(int64_t)f;
+    for (int c = 0; c < 3; c++) color[i * 3 + c] = (uint8_t)((u >> (8 * (2 - c))) & 255u);
+  }
+  return check_device_errors(s);
+}
+
 }  // extern "C"

@@ -16,7 +16,7 @@ Differences from the reference, all explicit:
     4096^d sparse grid; particles that leave it raise ``DomainError``;
   * ``unbounded``, ``quant``, ``use_g2p2g``, ``use_adaptive_dt`` and
     ``support_plasticity=False`` raise ``NotImplementedError``, as do the
-    mesh/texture/polygon seeders and the particle file writers;
+    mesh/texture/polygon seeders;
   * ``grid_postprocess`` lists descriptions of the built-in post-processes
     (bounding box and colliders); arbitrary Taichi callables cannot be added.
 """
@@ -382,11 +382,19 @@ class MPMSolver:
     def add_texture_2d(self, *args, **kwargs):
         raise NotImplementedError("add_texture_2d is not part of this engine")
 
-    def write_particles(self, *args, **kwargs):
-        raise NotImplementedError("particle file writers are not part of this engine")
+    def write_particles(self, fn, slice_size=1000000):
+        """Quantised particle file (mpm_solver.py:1198-1200); the
+        quantisation runs on the GPU (taichi_elements_amd.particle_io)."""
+        from .particle_io import ParticleIO
+        ParticleIO.write_particles(self, fn, slice_size)
 
-    def write_particles_ply(self, *args, **kwargs):
-        raise NotImplementedError("particle file writers are not part of this engine")
+    def write_particles_ply(self, fn):
+        """PLY point cloud of positions and 0xRRGGBB colours as four
+        little-endian bytes (mpm_solver.py:1202-1209)."""
+        from .particle_io import write_point_cloud
+        info = self.particle_info()
+        write_point_cloud(fn, np.hstack([info['position'],
+                                         info['color'].view(np.uint32)[:, None].view(np.float32)]))
 
     # ------------------------------------------------------------------ #
     # time stepping (mpm_solver.py:748-821)
