@@ -187,6 +187,33 @@ int mpm_host_svd(int32_t dim, const float* A, float* U, float* sig, float* V);
  * work items (chunks), active grid blocks, and particle blocks. */
 int mpm_last_counts(mpm_solver* s, int64_t* out4);
 
+/* Differentiable mode (no reference counterpart: the reference's p2g is not
+ * differentiable -- it mutates F and Jp in place -- and it has no per-particle
+ * stiffness).  One substep is a function of a caller-owned device state:
+ * state arrays are float32 SoA of NF = 2d + 2d^2 + 1 rows of n entries
+ * (x[d], v[d], C[d*d] row-major, F[d*d], Jp), in particle order; youngs is
+ * the per-particle Young's modulus E_p (device, n floats) and nu the global
+ * Poisson ratio: mu_0 = E_p / (2 (1 + nu)), lambda_0 = E_p nu / ((1 + nu)(1 -
+ * 2 nu)), scaled by h = 0.3 for elastic particles as the reference does.
+ * Materials are water and elastic; the post-processes must be the bounding
+ * box only.  stream: the hipStream_t to enqueue on (NULL: the null stream,
+ * e.g. torch's default stream).  The handle must hold no particles of its
+ * own.
+ * mpm_diff_setup: n particles with these host materials.
+ * mpm_diff_substep: state_out = substep(state_in) (may alias).
+ * mpm_diff_substep_backward: given grad_out = dL/d state_out, writes
+ *   grad_in = dL/d state_in, grad_youngs = dL/dE (may be NULL) and
+ *   grad_nu[0] = dL/dnu (device).  The binning and the grid are recomputed
+ *   from state_in.
+ * mpm_diff_sync: waits for stream and reports domain / CFL errors. */
+int mpm_diff_setup(mpm_solver* s, int64_t n, const int32_t* material);
+int mpm_diff_substep(mpm_solver* s, float dt, const float* state_in, float* state_out,
+                     const float* youngs, double nu, void* stream);
+int mpm_diff_substep_backward(mpm_solver* s, float dt, const float* state_in,
+                              const float* youngs, double nu, const float* grad_out,
+                              float* grad_in, float* grad_youngs, float* grad_nu, void* stream);
+int mpm_diff_sync(mpm_solver* s, void* stream);
+
 /* The quantised particle file of ParticleIO.write_particles
  * (engine/particle_io.py:12-76), computed on the GPU: ranges (2, dim, 2)
  * float32 = [x | v][axis][min, max] with too-narrow ranges widened to

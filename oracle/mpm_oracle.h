@@ -58,6 +58,12 @@ typedef struct {
   int32_t grid_lo[3];
   int32_t grid_n[3];
   oracle_post_op post[ORACLE_MAX_POST];
+  /* Optional per-particle Young's modulus (NULL: mu_0 / lambda_0 above):
+   * mu_0 = E_p / (2 (1 + nu)), lambda_0 = E_p nu / ((1 + nu)(1 - 2 nu)),
+   * the constants of mpm_solver.py:201-206 with E made per particle (the
+   * engine's differentiable mode; the reference has one global E). */
+  const double* youngs;
+  double nu;
 } oracle_cfg;
 
 /* ---- float instantiation ---- */

@@ -39,7 +39,8 @@ class Cfg(ctypes.Structure):
                 ("alpha", ctypes.c_double), ("water_density", ctypes.c_double),
                 ("gravity", ctypes.c_double * 3),
                 ("grid_lo", ctypes.c_int32 * 3), ("grid_n", ctypes.c_int32 * 3),
-                ("post", PostOp * MAX_POST)]
+                ("post", PostOp * MAX_POST),
+                ("youngs", ctypes.POINTER(ctypes.c_double)), ("nu", ctypes.c_double)]
 
 
 def build():
@@ -76,6 +77,18 @@ def reference_constants(res, size=1, dt_scale=1, E_scale=1):
     alpha = math.sqrt(2 / 3) * 2 * sin_phi / (3 - sin_phi)
     return dict(dim=dim, dx=dx, inv_dx=inv_dx, default_dt=default_dt, p_vol=p_vol,
                 p_mass=p_mass, E=E, nu=nu, mu_0=mu_0, lambda_0=lambda_0, alpha=alpha)
+
+
+def with_youngs(cfg, youngs, nu):
+    """A copy of cfg with a per-particle Young's modulus (float64, one per
+    particle) and Poisson ratio nu."""
+    c = Cfg()
+    ctypes.pointer(c)[0] = cfg
+    arr = np.ascontiguousarray(np.asarray(youngs, np.float64))
+    c.youngs = arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    c.nu = float(nu)
+    c._youngs_ref = arr  # keeps the array alive with the struct
+    return c
 
 
 def make_cfg(res, size=1, padding=3, gravity=None, water_density=1.0, post=None,

@@ -55,6 +55,12 @@ struct Params {
   int bn[3];            // extent of the combined window of all scenes (in blocks)
   int zoff;             // the scene's first block along the stacking axis (last axis)
   int skip;             // 1: the scene sits this substep out (state and grid untouched)
+  // Differentiable mode: per-particle Young's modulus E_p (indexed by slot)
+  // and a global Poisson ratio nu.  mu_0 = E_p kmu, lambda_0 = E_p kla with
+  // kmu = 1 / (2 (1 + nu)), kla = nu / ((1 + nu)(1 - 2 nu)); dkmu, dkla are
+  // their derivatives in nu.  youngs == nullptr: the global mu_0, lambda_0.
+  const float* youngs;
+  float kmu, kla, dkmu, dkla;
   int n_post;
   PostOp post[kMaxPost];
 };
@@ -298,8 +304,9 @@ __device__ __forceinline__ void sand_projection(float sig[3], float& Jp, const P
 // Per-particle F / Jp update and Kirchhoff stress, mpm_solver.py:507-568.
 // F is updated in place; stress receives tau (before the p2g scale).
 template <int D>
-__device__ __forceinline__ void constitutive(int mat, const Params& P, float F[3][3],
-                                             const float C[3][3], float& Jp, float stress[3][3]) {
+__device__ __forceinline__ void constitutive(int mat, const Params& P, float mu_0, float lambda_0,
+                                             float F[3][3], const float C[3][3], float& Jp,
+                                             float stress[3][3]) {
   float G[3][3];
   if (mat == kWater) {  // mpm_solver.py:508-511
 #pragma unroll
@@ -339,7 +346,7 @@ __device__ __forceinline__ void constitutive(int mat, const Params& P, float F[3
       }
     F[0][0] = J;
     Jp = J;
-    const float iso = P.lambda_0 * J * (J - 1.0f);
+    const float iso = lambda_0 * J * (J - 1.0f);
 #pragma unroll
     for (int i = 0; i < D; i++) stress[i][i] = iso;
     return;
@@ -348,7 +355,7 @@ __device__ __forceinline__ void constitutive(int mat, const Params& P, float F[3
   float h = 1.0f;  // mpm_solver.py:515-524 (sand ignores h: its stress uses mu_0, lambda_0)
   if (mat == kSnow) h = __expf(10.0f * (1.0f - Jp));
   if (mat == kElastic) h = 0.3f;
-  float mu = P.mu_0 * h, la = P.lambda_0 * h;
+  float mu = mu_0 * h, la = lambda_0 * h;
 
   float U[3][3], V[3][3], sig[3];
   if (D == 3)
@@ -415,10 +422,10 @@ __device__ __forceinline__ void constitutive(int mat, const Params& P, float F[3
     for (int i = 0; i < D; i++) {
       float l = logf(sig[i]);
       lsum += l;
-      center[i] = 2.0f * P.mu_0 * l * (1.0f / sig[i]);
+      center[i] = 2.0f * mu_0 * l * (1.0f / sig[i]);
     }
 #pragma unroll
-    for (int i = 0; i < D; i++) center[i] += P.lambda_0 * lsum * (1.0f / sig[i]);
+    for (int i = 0; i < D; i++) center[i] += lambda_0 * lsum * (1.0f / sig[i]);
     float T[3][3];
 #pragma unroll
     for (int i = 0; i < D; i++)
@@ -503,6 +510,43 @@ __device__ __forceinline__ void apply_post(const Params& P, const int I[3], floa
       }
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// combined block window: block ids <-> coordinates, scenes
+// ---------------------------------------------------------------------------
+template <int D>
+__device__ __forceinline__ void decode_local(int bid, const Params& P, int lb[3]) {
+  lb[0] = bid % P.bn[0];
+  int r = bid / P.bn[0];
+  if (D == 3) {
+    lb[1] = r % P.bn[1];
+    lb[2] = r / P.bn[1];
+  } else {
+    lb[1] = r;
+    lb[2] = 0;
+  }
+}
+
+template <int D>
+__device__ __forceinline__ int encode_local(const int lb[3], const Params& P) {
+  return D == 3 ? (lb[2] * P.bn[1] + lb[1]) * P.bn[0] + lb[0] : lb[1] * P.bn[0] + lb[0];
+}
+
+// Scenes are stacked along the last axis of one combined block window; the
+// slab table maps a combined block coordinate on that axis to its scene.
+template <int D>
+__device__ __forceinline__ int scene_of(const int lb[3], const int* __restrict__ slab) {
+  return slab[lb[D - 1]];
+}
+
+// scene-local block coordinates of a combined block
+template <int D>
+__device__ __forceinline__ void to_local(const int lb[3], const Params& P, int out[3]) {
+  out[0] = lb[0];
+  out[1] = lb[1];
+  out[2] = lb[2];
+  out[D - 1] -= P.zoff;
 }
 
 }  // namespace mpm

@@ -94,6 +94,21 @@ template <int D>
 void launch_substep(const Params* P, const int* slab, SubstepBuffers& B, int substep_index,
                     int raw_grid, hipStream_t st, Timer* timer);
 
+// binning + p2g + grid operator only (raw_grid: keep the unnormalised sums)
+template <int D>
+void launch_front(const Params* P, const int* slab, SubstepBuffers& B, int raw_grid,
+                  hipStream_t st, Timer* timer);
+
+// Adjoint of one substep after launch_front(raw_grid = 1) on the same old
+// state pf_in: gout = adjoints of the new state, gin = adjoints of the old
+// state (both laid out like pf), gE (may be null) = adjoint of the
+// per-particle Young's modulus, gnu[0] = adjoint of nu.  gbar holds
+// total_blocks * nodes float4, nu_part persistent_groups floats.
+template <int D>
+void launch_adjoint(const Params* P, const int* slab, SubstepBuffers& B, const float* pf_in,
+                    const float* gout, float* gin, float* gE, float4* gbar, float* nu_part,
+                    float* gnu, hipStream_t st, Timer* timer);
+
 template <int D>
 void launch_reorder(SubstepBuffers& B, float* pf_alt, int* pi_alt, hipStream_t st, Timer* timer);
 

@@ -13,6 +13,7 @@
 //   k_g2p             per chunk: LDS tile of grid velocity, gather,
 //                     advection, max |v|                                (g2p, mpm:694-735)
 #include "mpm_kernels.h"
+#include "mpm_adjoint.h"
 
 namespace mpm {
 
@@ -53,40 +54,6 @@ __device__ __forceinline__ int4 block_exscan4(int4 v, int4* total, int4* wsum) {
   *total = tot;
   return make_int4(pre.x + inc.x - v.x, pre.y + inc.y - v.y, pre.z + inc.z - v.z,
                    pre.w + inc.w - v.w);
-}
-
-template <int D>
-__device__ __forceinline__ void decode_local(int bid, const Params& P, int lb[3]) {
-  lb[0] = bid % P.bn[0];
-  int r = bid / P.bn[0];
-  if (D == 3) {
-    lb[1] = r % P.bn[1];
-    lb[2] = r / P.bn[1];
-  } else {
-    lb[1] = r;
-    lb[2] = 0;
-  }
-}
-
-template <int D>
-__device__ __forceinline__ int encode_local(const int lb[3], const Params& P) {
-  return D == 3 ? (lb[2] * P.bn[1] + lb[1]) * P.bn[0] + lb[0] : lb[1] * P.bn[0] + lb[0];
-}
-
-// Scenes are stacked along the last axis of one combined block window; the
-// slab table maps a combined block coordinate on that axis to its scene.
-template <int D>
-__device__ __forceinline__ int scene_of(const int lb[3], const int* __restrict__ slab) {
-  return slab[lb[D - 1]];
-}
-
-// scene-local block coordinates of a combined block
-template <int D>
-__device__ __forceinline__ void to_local(const int lb[3], const Params& P, int out[3]) {
-  out[0] = lb[0];
-  out[1] = lb[1];
-  out[2] = lb[2];
-  out[D - 1] -= P.zoff;
 }
 
 // ---------------------------------------------------------------------------
@@ -489,7 +456,12 @@ __global__ __launch_bounds__(kThreads) void k_onesweep(
 //           node tile (momentum, mass) is written to HBM for the grid pass.
 // The accumulation order is fixed, so p2g is bitwise reproducible.
 // ---------------------------------------------------------------------------
-template <int D>
+// MODE 0: the forward transfer.  MODE 1: the scatter half of the g2p adjoint
+// -- node o of particle p receives w_o (a + c4 Cbar (o - fx)) with
+// a = vbar' + dt xbar' and Cbar the adjoint of the new C (gf holds the
+// adjoints of the new particle state, same layout as pf); the mass channel
+// stays zero.  Both scatter "q + Ad o" per node through the same LDS path.
+template <int D, int MODE>
 __global__ __launch_bounds__(kThreads) void k_p2g(const Params* __restrict__ scenes,
                                                   const int* __restrict__ slab,
                                                   float* __restrict__ pf,
@@ -497,7 +469,8 @@ __global__ __launch_bounds__(kThreads) void k_p2g(const Params* __restrict__ sce
                                                   const int* __restrict__ perm,
                                                   const int4* __restrict__ chunks,
                                                   const int* __restrict__ totals,
-                                                  float4* __restrict__ tiles) {
+                                                  float4* __restrict__ tiles,
+                                                  const float* __restrict__ gf) {
   using L = Layout<D>;
   constexpr int TN = Geo<D>::kTileNodes;
   constexpr int CELLS = Geo<D>::kNodesPerBlock;  // cells of a block (64 / 16)
@@ -532,7 +505,42 @@ __global__ __launch_bounds__(kThreads) void k_p2g(const Params* __restrict__ sce
     const int t = threadIdx.x;
     const bool valid = t < nc;
     int cell = 0;
-    if (valid) {
+    if (MODE == 1 && valid) {
+      const int p = perm[ck.y + t];
+      const int mat = pi[(int64_t)L::MAT * cap + p];
+      float fx[3] = {0.0f, 0.0f, 0.0f};
+      int l[3] = {0, 0, 0};
+#pragma unroll
+      for (int a = 0; a < D; a++) {
+        int base;
+        base_fx(pf[(int64_t)(L::X + a) * cap + p], P.inv_dx, base, fx[a]);
+        l[a] = min(max(base - b[a], 0), kBlock - 1);
+      }
+      cell = D == 3 ? l[0] + 4 * (l[1] + 4 * l[2]) : l[0] + 4 * l[1];
+      // a stationary particle keeps x, v, C in g2p: no grid adjoint
+      const float on = mat == kStationary ? 0.0f : 1.0f;
+      float Ad[3][3] = {{0.0f}}, q[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int a = 0; a < D; a++) {
+        float s = on * (gf[(int64_t)(L::V + a) * cap + p] + P.dt * gf[(int64_t)(L::X + a) * cap + p]);
+#pragma unroll
+        for (int e = 0; e < D; e++) {
+          Ad[a][e] = on * P.c4 * gf[(int64_t)(L::C + a * D + e) * cap + p];
+          s -= Ad[a][e] * fx[e];
+        }
+        q[a] = s;
+      }
+      if (D == 3) {
+        stage[t][0] = make_float4(Ad[0][0], Ad[0][1], Ad[0][2], Ad[1][0]);
+        stage[t][1] = make_float4(Ad[1][1], Ad[1][2], Ad[2][0], Ad[2][1]);
+        stage[t][2] = make_float4(Ad[2][2], q[0], q[1], q[2]);
+        stage[t][3] = make_float4(fx[0], fx[1], fx[2], 0.0f);
+      } else {
+        stage[t][0] = make_float4(Ad[0][0], Ad[0][1], Ad[1][0], Ad[1][1]);
+        stage[t][1] = make_float4(q[0], q[1], fx[0], fx[1]);
+        stage[t][2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      }
+    } else if (MODE == 0 && valid) {
       const int p = perm[ck.y + t];
       const int mat = pi[(int64_t)L::MAT * cap + p];
       float xp[3], vp[3], C[3][3], F[3][3], stress[3][3];
@@ -559,7 +567,13 @@ __global__ __launch_bounds__(kThreads) void k_p2g(const Params* __restrict__ sce
         l[a] = min(max(base - b[a], 0), kBlock - 1);
       }
       cell = D == 3 ? l[0] + 4 * (l[1] + 4 * l[2]) : l[0] + 4 * l[1];
-      constitutive<D>(mat, P, F, C, Jp, stress);
+      float mu_0 = P.mu_0, lambda_0 = P.lambda_0;
+      if (P.youngs) {
+        const float E = P.youngs[p];
+        mu_0 = E * P.kmu;
+        lambda_0 = E * P.kla;
+      }
+      constitutive<D>(mat, P, mu_0, lambda_0, F, C, Jp, stress);
 #pragma unroll
       for (int a = 0; a < D; a++)
 #pragma unroll
@@ -732,7 +746,30 @@ __global__ __launch_bounds__(kThreads) void k_p2g(const Params* __restrict__ sce
 // ---------------------------------------------------------------------------
 // grid operator (mpm_solver.py:586-692)
 // ---------------------------------------------------------------------------
+// Bounding-box walls zero the outward velocity components near the domain
+// faces (mpm_solver.py:600-616); bit a of the result is set when component a
+// of v (before the walls) passes unchanged.
 template <int D>
+__device__ __forceinline__ int wall_keep_mask(const Params& P, const int I[3], const float v[3]) {
+  int keep = (1 << D) - 1;
+  bool bbox = false;
+  for (int k = 0; k < P.n_post; k++) bbox |= P.post[k].kind == kPostBBox;
+  if (bbox) {
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+      if ((I[a] < P.padding && v[a] < 0.0f) || (I[a] >= P.res[a] - P.padding && v[a] > 0.0f))
+        keep &= ~(1 << a);
+    }
+  }
+  return keep;
+}
+
+// MODE 0: the grid operator (raw != 0: store the p2g sums unnormalised).
+// MODE 1: its adjoint -- the tiles hold the g2p-adjoint scatter vbar; with
+// the forward's raw sums (p, m) in grid, write (pbar, mbar) to gbar:
+// v = keep * (p / m + dt g) gives pbar = keep * vbar / m and
+// mbar = -sum_a pbar_a p_a / m (walls are the only post-process allowed).
+template <int D, int MODE>
 __global__ __launch_bounds__(kThreads) void k_grid_op(const Params* __restrict__ scenes,
                                                       const int* __restrict__ slab,
                                                       const int* __restrict__ active,
@@ -740,7 +777,8 @@ __global__ __launch_bounds__(kThreads) void k_grid_op(const Params* __restrict__
                                                       const int* __restrict__ pchunk_begin,
                                                       const int* __restrict__ pchunk_cnt,
                                                       const float4* __restrict__ tiles,
-                                                      float4* __restrict__ grid, int raw) {
+                                                      float4* __restrict__ grid, int raw,
+                                                      float4* __restrict__ gbar) {
   constexpr int NPB = Geo<D>::kNodesPerBlock, TN = Geo<D>::kTileNodes;
   const int64_t total = (int64_t)totals[kTotActive] * NPB;
   for (int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x; t < total;
@@ -780,6 +818,29 @@ __global__ __launch_bounds__(kThreads) void k_grid_op(const Params* __restrict__
             m += tv.w;
           }
         }
+    if (MODE == 1) {
+      const float4 f = grid[(int64_t)bid * NPB + node];
+      float pb[3] = {0.0f, 0.0f, 0.0f}, mb = 0.0f;
+      if (f.w > 0.0f) {
+        const float inv = 1.0f / f.w;
+        const float pf4[3] = {f.x, f.y, f.z};
+        float v[3];
+#pragma unroll
+        for (int a = 0; a < D; a++) v[a] = pf4[a] * inv + P.dt * P.gravity[a];
+        int I[3], lbl[3];
+        to_local<D>(lb, P, lbl);
+#pragma unroll
+        for (int a = 0; a < D; a++) I[a] = (lbl[a] + P.blo[a]) * kBlock + l[a];
+        const int keep = wall_keep_mask<D>(P, I, v);
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+          pb[a] = ((keep >> a) & 1) ? p[a] * inv : 0.0f;
+          mb -= pb[a] * (pf4[a] * inv);
+        }
+      }
+      gbar[(int64_t)bid * NPB + node] = make_float4(pb[0], pb[1], D == 3 ? pb[2] : 0.0f, mb);
+      continue;
+    }
     if (!raw) {
       if (m > 0.0f) {  // mpm_solver.py:591-593
         const float inv = 1.0f / m;
@@ -1196,9 +1257,10 @@ struct TimerScope {
   }
 };
 
+// binning, p2g and the grid operator of one substep
 template <int D>
-void launch_substep(const Params* P, const int* slab, SubstepBuffers& B, int s_index, int raw,
-                    hipStream_t st, Timer* tm) {
+void launch_front(const Params* P, const int* slab, SubstepBuffers& B, int raw, hipStream_t st,
+                  Timer* tm) {
   const int n = B.n;
   const int G = B.persistent_groups;
   if (n == 0) return;
@@ -1260,15 +1322,25 @@ void launch_substep(const Params* P, const int* slab, SubstepBuffers& B, int s_i
   }
   {
     TimerScope ts(tm, "k_p2g", st);
-    k_p2g<D><<<clamp_groups(chunk_bound, G), kThreads, 0, st>>>(
-        P, slab, B.pf, B.pi, B.cap, B.perm, B.chunks, B.totals, B.tiles);
+    k_p2g<D, 0><<<clamp_groups(chunk_bound, G), kThreads, 0, st>>>(
+        P, slab, B.pf, B.pi, B.cap, B.perm, B.chunks, B.totals, B.tiles, nullptr);
   }
   {
     TimerScope ts(tm, "k_grid_op", st);
     const int64_t node_bound = (int64_t)B.total_blocks * Geo<D>::kNodesPerBlock;
-    k_grid_op<D><<<clamp_groups(div_up(node_bound, kThreads), G / 2), kThreads, 0, st>>>(
-        P, slab, B.active, B.totals, B.pchunk_begin, B.pchunk_cnt, B.tiles, B.grid, raw);
+    k_grid_op<D, 0><<<clamp_groups(div_up(node_bound, kThreads), G / 2), kThreads, 0, st>>>(
+        P, slab, B.active, B.totals, B.pchunk_begin, B.pchunk_cnt, B.tiles, B.grid, raw, nullptr);
   }
+}
+
+// g2p of one substep (after launch_front); s_index selects the max-velocity slot
+template <int D>
+void launch_g2p(const Params* P, const int* slab, SubstepBuffers& B, int s_index, hipStream_t st,
+                Timer* tm) {
+  const int n = B.n;
+  const int G = B.persistent_groups;
+  if (n == 0) return;
+  const int chunk_bound = std::min(B.total_blocks, n) + div_up(n, B.chunk);
   {
     TimerScope ts(tm, "k_g2p", st);
     k_g2p<D><<<clamp_groups(chunk_bound, G), kThreads, 0, st>>>(
@@ -1277,6 +1349,47 @@ void launch_substep(const Params* P, const int* slab, SubstepBuffers& B, int s_i
         B.err);
   }
   std::swap(B.count, B.count_next);
+}
+
+template <int D>
+void launch_substep(const Params* P, const int* slab, SubstepBuffers& B, int s_index, int raw,
+                    hipStream_t st, Timer* tm) {
+  launch_front<D>(P, slab, B, raw, st, tm);
+  launch_g2p<D>(P, slab, B, s_index, st, tm);
+}
+
+template <int D>
+void launch_adjoint(const Params* P, const int* slab, SubstepBuffers& B, const float* pf_in,
+                    const float* gout, float* gin, float* gE, float4* gbar, float* nu_part,
+                    float* gnu, hipStream_t st, Timer* tm) {
+  const int n = B.n;
+  const int G = B.persistent_groups;
+  const int chunk_bound = n > 0 ? std::min(B.total_blocks, n) + div_up(n, B.chunk) : 1;
+  // particles outside every chunk (none unless one escaped) keep zero adjoints
+  launch_fill_u32((unsigned*)gin, 0u, (int64_t)Layout<D>::NF * B.cap, st);
+  if (gE) launch_fill_u32((unsigned*)gE, 0u, B.cap, st);
+  const int groups = clamp_groups(chunk_bound, G);
+  if (n > 0) {
+    {
+      TimerScope ts(tm, "k_g2p_adj_scatter", st);
+      k_p2g<D, 1><<<groups, kThreads, 0, st>>>(P, slab, const_cast<float*>(pf_in), B.pi, B.cap,
+                                               B.perm, B.chunks, B.totals, B.tiles, gout);
+    }
+    {
+      TimerScope ts(tm, "k_grid_op_adj", st);
+      const int64_t node_bound = (int64_t)B.total_blocks * Geo<D>::kNodesPerBlock;
+      k_grid_op<D, 1><<<clamp_groups(div_up(node_bound, kThreads), G / 2), kThreads, 0, st>>>(
+          P, slab, B.active, B.totals, B.pchunk_begin, B.pchunk_cnt, B.tiles, B.grid, 0, gbar);
+    }
+    {
+      TimerScope ts(tm, "k_p2g_adj", st);
+      k_p2g_adj<D><<<groups, kThreads, 0, st>>>(P, slab, pf_in, B.pi, B.cap, B.perm, B.chunks,
+                                                B.totals, B.grid, gbar, gout, gin, gE, nu_part);
+    }
+  } else {
+    launch_fill_u32((unsigned*)nu_part, 0u, groups, st);
+  }
+  k_sum_partials<<<1, kThreads, 0, st>>>(nu_part, groups, gnu);
 }
 
 template <int D>
@@ -1323,6 +1436,16 @@ void launch_fill_u32(unsigned* p, unsigned v, int64_t n, hipStream_t st) {
     k_fill_u32<<<clamp_groups(div_up(n, kThreads), 4096), kThreads, 0, st>>>(p, v, n);
 }
 
+template void launch_front<2>(const Params*, const int*, SubstepBuffers&, int, hipStream_t,
+                              Timer*);
+template void launch_front<3>(const Params*, const int*, SubstepBuffers&, int, hipStream_t,
+                              Timer*);
+template void launch_adjoint<2>(const Params*, const int*, SubstepBuffers&, const float*,
+                                const float*, float*, float*, float4*, float*, float*, hipStream_t,
+                                Timer*);
+template void launch_adjoint<3>(const Params*, const int*, SubstepBuffers&, const float*,
+                                const float*, float*, float*, float4*, float*, float*, hipStream_t,
+                                Timer*);
 template void launch_substep<2>(const Params*, const int*, SubstepBuffers&, int, int, hipStream_t,
                                 Timer*);
 template void launch_substep<3>(const Params*, const int*, SubstepBuffers&, int, int, hipStream_t,

@@ -166,6 +166,18 @@ struct mpm_solver {
   bool timing = false;
   EventTimer timer;
 
+  // differentiable mode (mpm_diff_*): the particle state lives in caller
+  // arrays; the handle supplies the block window, workspace and grid
+  int64_t diff_n = 0;
+  int* diff_pi = nullptr;          // 3 x diff_n: material, insertion index, scene 0
+  float* diff_scratch = nullptr;   // NF x diff_n: the state p2g rewrites in backward
+  float4* diff_gbar = nullptr;     // grid adjoint
+  float* diff_nu_part = nullptr;   // per-workgroup nu adjoints
+  bool diff_params_valid = false;  // the device scene table holds these values:
+  float diff_dt = 0.0f;
+  const float* diff_youngs = nullptr;
+  double diff_nu = 0.0;
+
   int nf() const { return D == 3 ? Layout<3>::NF : Layout<2>::NF; }
 };
 
@@ -241,6 +253,8 @@ void fill_params(Params& P, const mpm_solver* s, const SceneHost& sc, float dt, 
   P.padding = c.padding;
   P.zoff = sc.zoff;
   P.skip = skip;
+  P.youngs = nullptr;
+  P.kmu = P.kla = P.dkmu = P.dkla = 0.0f;
   P.n_post = (int)sc.post.size();
   for (size_t k = 0; k < sc.post.size(); k++) P.post[k] = sc.post[k];
 }
@@ -507,6 +521,10 @@ int mpm_destroy(mpm_solver* s) {
   dfree(s->maxv);
   dfree(s->d_params);
   dfree(s->d_slab);
+  dfree(s->diff_pi);
+  dfree(s->diff_scratch);
+  dfree(s->diff_gbar);
+  dfree(s->diff_nu_part);
   if (s->h_params) hipHostFree(s->h_params);
   if (s->stream) hipStreamDestroy(s->stream);
   delete s;
@@ -922,6 +940,143 @@ int mpm_last_counts(mpm_solver* s, int64_t* out4) {
   out4[2] = tot[kTotActive];
   out4[3] = tot[kTotPBlocks];
   return MPM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// differentiable mode
+// ---------------------------------------------------------------------------
+int mpm_diff_setup(mpm_solver* s, int64_t n, const int32_t* material) {
+  if (!s || (n > 0 && !material)) return fail(MPM_EINVAL, "null argument");
+  if (s->n != 0) return fail(MPM_EINVAL, "a differentiable handle holds no particles of its own");
+  if (s->scenes.size() != 1) return fail(MPM_EINVAL, "differentiable mode runs one scene");
+  if (n <= 0 || n > (int64_t)INT32_MAX / 2) return fail(MPM_EINVAL, "bad particle count");
+  for (int64_t i = 0; i < n; i++)
+    if (material[i] != MPM_WATER && material[i] != MPM_ELASTIC)
+      return fail(MPM_EINVAL, "differentiable mode supports water and elastic particles");
+  for (const PostOp& op : s->scenes[0].post)
+    if (op.kind != kPostBBox)
+      return fail(MPM_EINVAL, "differentiable mode supports the bounding box only");
+  HIP_TRY(hipSetDevice(s->device));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  TRY(grow(s, n));
+  dfree(s->diff_pi);
+  dfree(s->diff_scratch);
+  dfree(s->diff_gbar);
+  dfree(s->diff_nu_part);
+  const int NPB = s->D == 3 ? Geo<3>::kNodesPerBlock : Geo<2>::kNodesPerBlock;
+  TRY(dalloc(&s->diff_pi, 3 * n));
+  TRY(dalloc(&s->diff_scratch, (int64_t)s->nf() * n));
+  TRY(dalloc(&s->diff_gbar, (int64_t)s->total_blocks * NPB));
+  TRY(dalloc(&s->diff_nu_part, s->persistent_groups));
+  std::vector<int32_t> host(3 * n);
+  for (int64_t i = 0; i < n; i++) {
+    host[i] = material[i];
+    host[n + i] = (int32_t)i;
+    host[2 * n + i] = 0;
+  }
+  HIP_TRY(hipMemcpy(s->diff_pi, host.data(), sizeof(int32_t) * host.size(),
+                    hipMemcpyHostToDevice));
+  s->diff_n = n;
+  s->diff_params_valid = false;
+  return MPM_OK;
+}
+
+namespace {
+
+// scene table for a differentiable call, uploaded on st only when it changed
+int diff_params(mpm_solver* s, float dt, const float* youngs, double nu, hipStream_t st) {
+  if (!(dt > 0.0f) || !std::isfinite(dt)) return fail(MPM_EINVAL, "dt must be positive");
+  if (!youngs) return fail(MPM_EINVAL, "null Young's modulus array");
+  if (!(nu > -1.0 && nu < 0.5)) return fail(MPM_EINVAL, "nu must lie in (-1, 0.5)");
+  if (s->diff_params_valid && s->diff_dt == dt && s->diff_youngs == youngs && s->diff_nu == nu)
+    return MPM_OK;
+  HIP_TRY(hipStreamSynchronize(st));  // the pinned staging copy may still be in flight
+  Params& P = s->h_params[0];
+  fill_params(P, s, s->scenes[0], dt, 0);
+  P.youngs = youngs;
+  P.kmu = (float)(1.0 / (2.0 * (1.0 + nu)));
+  P.kla = (float)(nu / ((1.0 + nu) * (1.0 - 2.0 * nu)));
+  P.dkmu = (float)(-1.0 / (2.0 * (1.0 + nu) * (1.0 + nu)));
+  const double g = (1.0 + nu) * (1.0 - 2.0 * nu);
+  P.dkla = (float)((1.0 + 2.0 * nu * nu) / (g * g));
+  HIP_TRY(hipMemcpyAsync(s->d_params, s->h_params, sizeof(Params), hipMemcpyHostToDevice, st));
+  s->diff_params_valid = true;
+  s->diff_dt = dt;
+  s->diff_youngs = youngs;
+  s->diff_nu = nu;
+  return MPM_OK;
+}
+
+void diff_buffers(mpm_solver* s, SubstepBuffers& B, float* pf) {
+  fill_buffers(s, B);
+  B.pf = pf;
+  B.pi = s->diff_pi;
+  B.cap = s->diff_n;
+  B.n = (int)s->diff_n;
+  B.full_sort = true;  // every call is a function of its input state alone
+  B.perm_identity = false;
+}
+
+}  // namespace
+
+int mpm_diff_substep(mpm_solver* s, float dt, const float* state_in, float* state_out,
+                     const float* youngs, double nu, void* stream) {
+  if (!s || !state_in || !state_out) return fail(MPM_EINVAL, "null argument");
+  if (s->diff_n == 0) return fail(MPM_EINVAL, "mpm_diff_setup first");
+  hipStream_t st = (hipStream_t)stream;  // NULL: the null stream
+  HIP_TRY(hipSetDevice(s->device));
+  TRY(diff_params(s, dt, youngs, nu, st));
+  const int64_t n = s->diff_n;
+  if (state_out != state_in)
+    HIP_TRY(hipMemcpyAsync(state_out, state_in, sizeof(float) * s->nf() * n,
+                           hipMemcpyDeviceToDevice, st));
+  SubstepBuffers B;
+  diff_buffers(s, B, state_out);
+  if (s->D == 3)
+    launch_substep<3>(s->d_params, s->d_slab, B, 0, 0, st, nullptr);
+  else
+    launch_substep<2>(s->d_params, s->d_slab, B, 0, 0, st, nullptr);
+  take_buffers(s, B);
+  s->substep_counter++;
+  HIP_TRY(hipGetLastError());
+  return MPM_OK;
+}
+
+int mpm_diff_substep_backward(mpm_solver* s, float dt, const float* state_in,
+                              const float* youngs, double nu, const float* grad_out,
+                              float* grad_in, float* grad_youngs, float* grad_nu, void* stream) {
+  if (!s || !state_in || !grad_out || !grad_in || !grad_nu)
+    return fail(MPM_EINVAL, "null argument");
+  if (s->diff_n == 0) return fail(MPM_EINVAL, "mpm_diff_setup first");
+  hipStream_t st = (hipStream_t)stream;  // NULL: the null stream
+  HIP_TRY(hipSetDevice(s->device));
+  TRY(diff_params(s, dt, youngs, nu, st));
+  const int64_t n = s->diff_n;
+  // the forward's binning and raw grid sums, recomputed from the old state
+  HIP_TRY(hipMemcpyAsync(s->diff_scratch, state_in, sizeof(float) * s->nf() * n,
+                         hipMemcpyDeviceToDevice, st));
+  SubstepBuffers B;
+  diff_buffers(s, B, s->diff_scratch);
+  if (s->D == 3) {
+    launch_front<3>(s->d_params, s->d_slab, B, 1, st, nullptr);
+    launch_adjoint<3>(s->d_params, s->d_slab, B, state_in, grad_out, grad_in, grad_youngs,
+                      s->diff_gbar, s->diff_nu_part, grad_nu, st, nullptr);
+  } else {
+    launch_front<2>(s->d_params, s->d_slab, B, 1, st, nullptr);
+    launch_adjoint<2>(s->d_params, s->d_slab, B, state_in, grad_out, grad_in, grad_youngs,
+                      s->diff_gbar, s->diff_nu_part, grad_nu, st, nullptr);
+  }
+  take_buffers(s, B);
+  s->substep_counter++;
+  HIP_TRY(hipGetLastError());
+  return MPM_OK;
+}
+
+int mpm_diff_sync(mpm_solver* s, void* stream) {
+  if (!s) return fail(MPM_EINVAL, "null argument");
+  HIP_TRY(hipSetDevice(s->device));
+  HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+  return check_device_errors(s);
 }
 
 int mpm_quantize_particles(mpm_solver* s, float* ranges, uint32_t* x_and_v, uint8_t* color) {

@@ -1,0 +1,168 @@
+"""Differentiable MLS-MPM substeps on the GPU through torch.autograd.
+
+The reference's substep (engine/mpm_solver.py:487-735) is not
+differentiable: its p2g overwrites F and Jp in place and its Young's modulus
+is one global constant (mpm_solver.py:201).  Here one substep is a pure
+function of a particle state held in a torch tensor, with a per-particle
+Young's modulus E_p and a global Poisson ratio nu as differentiable inputs:
+
+    state' = substep(state, E, nu)        (water and fixed-corotated elastic)
+
+The forward runs the engine's HIP kernels (binning, p2g, grid operator,
+g2p) on the caller's tensors; the backward runs hand-written adjoint kernels
+(scatter of the g2p adjoint into LDS node tiles, the grid-operator adjoint,
+and a per-particle kernel with the p2g/g2p gathers and the constitutive
+adjoint, including the derivative of the polar rotation R = U V^T), after
+recomputing the binning and the grid from the substep's input.  Each
+substep is one ``torch.autograd.Function``; ``run`` checkpoints the state
+every ``checkpoint_every`` substeps (torch.utils.checkpoint) so only those
+states are stored and each segment is recomputed during the backward pass.
+
+State layout: float32 tensor of shape (NF, n), NF = 2d + 2d^2 + 1, rows
+x[d], v[d], C[d*d] (row-major), F[d*d], Jp.
+
+torch is imported before the engine library is loaded, so that the process
+runs a single HIP runtime (torch's).
+"""
+import numpy as np
+import torch
+import torch.utils.checkpoint
+
+from . import _lib
+from ._lib import check
+from .mpm_solver import MPMSolver
+
+
+def n_fields(dim):
+    return 2 * dim + 2 * dim * dim + 1
+
+
+def _ptr(t):
+    return _lib.ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream():
+    return _lib.ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class _Substep(torch.autograd.Function):
+
+    @staticmethod
+    def forward(ctx, state, youngs, nu, solver, nu_value, dt):
+        out = torch.empty_like(state)
+        check(_lib.lib().mpm_diff_substep(solver._handle, float(dt), _ptr(state), _ptr(out),
+                                          _ptr(youngs), float(nu_value), _stream()))
+        ctx.save_for_backward(state, youngs)
+        ctx.solver, ctx.nu_value, ctx.dt = solver, nu_value, dt
+        ctx.nu_shape, ctx.nu_dtype = nu.shape, nu.dtype
+        return out
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        state, youngs = ctx.saved_tensors
+        grad_out = grad_out.contiguous()
+        grad_in = torch.empty_like(state)
+        grad_e = torch.empty_like(youngs)
+        grad_nu = torch.empty(1, dtype=torch.float32, device=state.device)
+        check(_lib.lib().mpm_diff_substep_backward(
+            ctx.solver._handle, float(ctx.dt), _ptr(state), _ptr(youngs), float(ctx.nu_value),
+            _ptr(grad_out), _ptr(grad_in), _ptr(grad_e), _ptr(grad_nu), _stream()))
+        return (grad_in, grad_e, grad_nu.reshape(ctx.nu_shape).to(ctx.nu_dtype), None, None,
+                None)
+
+
+class DiffMPMSolver:
+    """A differentiable scene: the constants, gravity and walls of
+    ``MPMSolver(res, size, padding, dt_scale, E_scale, water_density)``,
+    particles in a torch state tensor.
+
+    ``setup(material)`` fixes the particle count and materials (water /
+    elastic); ``state(x, v)`` builds an initial state (F = I, C = 0,
+    Jp = 1); ``substep`` / ``run`` advance it differentiably.
+    """
+
+    def __init__(self, res, size=1, padding=3, dt_scale=1, E_scale=1, water_density=1.0,
+                 device=0, gravity=None):
+        from . import mpm_solver
+        mpm_solver.PRINT_PROGRESS = False
+        self._solver = MPMSolver(res, size=size, padding=padding, dt_scale=dt_scale,
+                                 E_scale=E_scale, water_density=water_density, device=device,
+                                 reorder_interval=-1)
+        if gravity is not None:
+            self._solver.set_gravity(tuple(gravity))
+        self.dim = self._solver.dim
+        self.default_dt = self._solver.default_dt
+        self.E = self._solver.E
+        self.nu = self._solver.nu
+        self.device = torch.device("cuda", device)
+        self.n = 0
+
+    @property
+    def _handle(self):
+        return self._solver._handle
+
+    @property
+    def solver(self):
+        """The underlying MPMSolver (constants, set_gravity)."""
+        return self._solver
+
+    def setup(self, material):
+        material = np.ascontiguousarray(np.asarray(material, np.int32).reshape(-1))
+        check(_lib.lib().mpm_diff_setup(self._handle, len(material), _lib.iptr(material)))
+        self.n = len(material)
+        self.material = material
+
+    def state(self, x, v=None):
+        """Initial state tensor from positions (and velocities): F = I,
+        C = 0, Jp = 1 (seed_particle, mpm_solver.py:823-838)."""
+        d, n = self.dim, self.n
+        x = torch.as_tensor(np.asarray(x, np.float32).reshape(n, d))
+        s = torch.zeros(n_fields(d), n, dtype=torch.float32)
+        s[0:d] = x.T
+        if v is not None:
+            s[d:2 * d] = torch.as_tensor(np.asarray(v, np.float32).reshape(n, d)).T
+        f0 = 2 * d + d * d
+        for a in range(d):
+            s[f0 + a * d + a] = 1.0
+        s[-1] = 1.0
+        return s.to(self.device)
+
+    def positions(self, state):
+        return state[0:self.dim].T
+
+    def substep(self, state, youngs, nu, dt=None, nu_value=None):
+        """One differentiable substep.  ``youngs``: (n,) float32 tensor;
+        ``nu``: 0-d tensor (its value is read on the host unless
+        ``nu_value`` is given)."""
+        if dt is None:
+            dt = self.default_dt
+        if nu_value is None:
+            nu_value = float(nu.detach()) if torch.is_tensor(nu) else float(nu)
+        return _Substep.apply(state, youngs, nu, self, nu_value, dt)
+
+    def _segment(self, state, youngs, nu, nu_value, dt, k):
+        for _ in range(k):
+            state = _Substep.apply(state, youngs, nu, self, nu_value, dt)
+        return state
+
+    def run(self, state, youngs, nu, nsub, dt=None, checkpoint_every=8):
+        """``nsub`` substeps; with gradients enabled and checkpoint_every > 1
+        only every checkpoint_every-th state is kept and each segment is
+        recomputed in the backward pass."""
+        if dt is None:
+            dt = self.default_dt
+        nu_value = float(nu.detach()) if torch.is_tensor(nu) else float(nu)
+        k = max(1, int(checkpoint_every))
+        for t0 in range(0, nsub, k):
+            m = min(k, nsub - t0)
+            if k > 1 and torch.is_grad_enabled():
+                state = torch.utils.checkpoint.checkpoint(self._segment, state, youngs, nu,
+                                                          nu_value, dt, m, use_reentrant=False)
+            else:
+                state = self._segment(state, youngs, nu, nu_value, dt, m)
+        return state
+
+    def check(self):
+        """Waits for the work queued so far; raises DomainError if particles
+        left the grid window or moved more than a block in one substep."""
+        check(_lib.lib().mpm_diff_sync(self._handle, _stream()))
