@@ -177,6 +177,76 @@ def batched_bench(steps, warmup):
     return out
 
 
+def fwd_bwd_bench(nsub=64, checkpoint_every=8, repeats=3, seed=4):
+    """Differentiable workload: 128^3 grid, an elastic cube of 100k particles
+    with per-particle E ~ U[5e5, 2e6] and a water cube of 100k particles
+    pressed against one of its faces (so that the elastic body's motion
+    depends on its stiffness through the contact), v ~ U[-0.5, 0.5]^3,
+    64 substeps at default_dt with checkpoints every 8; loss = squared
+    distance of the elastic body's centre of mass from a seeded target;
+    gradients with respect to the initial velocities, E and nu.  Times the
+    forward alone (no autograd) and forward + backward."""
+    import torch  # before the engine library loads: one HIP runtime
+    from taichi_elements_amd.diff import DiffMPMSolver
+    rng = np.random.default_rng(seed)
+    n_el = n_wa = 100_000
+    x_el = (np.array([0.25, 0.2, 0.25], np.float32)
+            + rng.random((n_el, 3), dtype=np.float32) * np.float32(0.23))
+    x_wa = (np.array([0.48, 0.2, 0.25], np.float32)
+            + rng.random((n_wa, 3), dtype=np.float32) * np.float32(0.23))
+    x = np.concatenate([x_el, x_wa])
+    mat = np.concatenate([np.full(n_el, 1, np.int32), np.zeros(n_wa, np.int32)])
+    v = rng.uniform(-0.5, 0.5, x.shape).astype(np.float32)
+    E = np.concatenate([rng.uniform(5e5, 2e6, n_el), np.full(n_wa, 1e6)]).astype(np.float32)
+    target = torch.tensor(x_el.mean(0) + rng.uniform(-0.05, 0.05, 3).astype(np.float32),
+                          device="cuda")
+    ds = DiffMPMSolver((128, 128, 128))
+    ds.setup(mat, x)
+    s0 = ds.state(x, v)
+    E_t = torch.tensor(E, device="cuda")
+    nu_t = torch.tensor(0.2, device="cuda")
+
+    def forward_only():
+        with torch.no_grad():
+            s = ds.run(s0, E_t, nu_t, nsub, checkpoint_every=checkpoint_every)
+            return ((ds.positions(s)[:n_el].mean(0) - target) ** 2).sum()
+
+    def forward_backward():
+        v_t = s0[3:6].clone().requires_grad_(True)
+        e_t = E_t.clone().requires_grad_(True)
+        n_t = nu_t.clone().requires_grad_(True)
+        s = torch.cat([s0[:3], v_t, s0[6:]], 0)
+        s = ds.run(s, e_t, n_t, nsub, checkpoint_every=checkpoint_every)
+        loss = ((ds.positions(s)[:n_el].mean(0) - target) ** 2).sum()
+        loss.backward()
+        return loss, v_t.grad, e_t.grad, n_t.grad
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        best = None
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            out = fn()
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+            best = el if best is None else min(best, el)
+        return best, out
+
+    t_f, loss = timed(forward_only)
+    t_fb, (loss2, gv, ge, gn) = timed(forward_backward)
+    ds.check()
+    n = n_el + n_wa
+    return dict(particles=n, substeps=nsub, checkpoint_every=checkpoint_every,
+                fwd_ms=1000 * t_f, fwd_bwd_ms=1000 * t_fb, ratio=t_fb / t_f,
+                fwd_value=n * nsub / t_f, fwd_bwd_value=n * nsub / t_fb,
+                unit="particle-substeps/s", loss=float(loss),
+                grad_norms=dict(v0=float(gv.norm()), E=float(ge.norm()), nu=float(gn)),
+                workload="128^3 grid, elastic cube 100k (E ~ U[5e5, 2e6]) + water cube 100k, "
+                         "64 substeps, checkpoints every 8, loss = |COM_elastic - target|^2, "
+                         "gradients w.r.t. v0, E, nu")
+
+
 def algorithmic_bytes(n_particles, n_nodes):
     """Bytes a substep must move in the reference's field layout (float32):
     per particle 16 (binning) + 148 (p2g) + 80 (g2p), per active grid node
@@ -238,12 +308,15 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--reorder-interval", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
-    ap.add_argument("--config", choices=["dense1m", "sparse16m"], default="dense1m",
+    ap.add_argument("--config", choices=["dense1m", "sparse16m", "fwdbwd200k"], default="dense1m",
                     help="dense1m: the 3D four-material 128^3 workload (default, the "
                          "headline line); sparse16m: 512^3 window, 16M particles in 64 "
-                         "ellipsoids with a plane collider")
+                         "ellipsoids with a plane collider; fwdbwd200k: only the "
+                         "differentiable 200k-particle forward + backward measurement")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the 64-scene batched measurement")
+    ap.add_argument("--no-fwd-bwd", action="store_true",
+                    help="skip the differentiable (forward + backward) measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device-to-device copy (hipMemcpy) so that "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE can be scaled by a kernel of "
@@ -255,6 +328,21 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.config == "fwdbwd200k":
+        import torch  # noqa: F401  (before the engine library loads)
+        fb = fwd_bwd_bench()
+        print(json.dumps({
+            "metric": "particle-substeps/s (3D 1M, 4-material): plain/batched/fwd+bwd; % HBM peak",
+            "value": fb["fwd_bwd_value"], "unit": "particle-substeps/s", "n_gpus": 1,
+            "steps": fb["substeps"], "warmup": 1, "ms_per_step": fb["fwd_bwd_ms"] / fb["substeps"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (seeded numpy: elastic + water cube, 100k particles each)",
+            "config": {"workload": fb["workload"]}, "fwd_bwd": fb}), flush=True)
+        return
+    if not args.no_fwd_bwd and args.config == "dense1m":
+        # torch (used by the differentiable leg) must be imported before the
+        # engine library loads, so that the process runs one HIP runtime
+        import torch  # noqa: F401
     from taichi_elements_amd import MPMSolver, mpm_solver
     mpm_solver.PRINT_PROGRESS = False
     res = RES
@@ -342,6 +430,8 @@ def main():
     if not args.no_batched and args.config == "dense1m":
         del s
         out["batched"] = batched_bench(args.steps, args.warmup)
+    if not args.no_fwd_bwd and args.config == "dense1m":
+        out["fwd_bwd"] = fwd_bwd_bench()
     if not args.no_cpu_baseline and args.config == "dense1m":
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(x, v, m, dt, args.cpu_seconds)

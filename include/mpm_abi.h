@@ -200,18 +200,22 @@ int mpm_last_counts(mpm_solver* s, int64_t* out4);
  * e.g. torch's default stream).  The handle must hold no particles of its
  * own.
  * mpm_diff_setup: n particles with these host materials.
- * mpm_diff_substep: state_out = substep(state_in) (may alias).
+ * mpm_diff_binning_bytes: size of the binning record of one substep.
+ * mpm_diff_substep: state_out = substep(state_in) (may alias); binning
+ *   (device, may be NULL) receives the substep's binning record.
  * mpm_diff_substep_backward: given grad_out = dL/d state_out, writes
  *   grad_in = dL/d state_in, grad_youngs = dL/dE (may be NULL) and
- *   grad_nu[0] = dL/dnu (device).  The binning and the grid are recomputed
- *   from state_in.
+ *   grad_nu[0] = dL/dnu (device).  The grid is recomputed from state_in, on
+ *   the forward's binning record when given (else the binning is redone).
  * mpm_diff_sync: waits for stream and reports domain / CFL errors. */
 int mpm_diff_setup(mpm_solver* s, int64_t n, const int32_t* material);
+int mpm_diff_binning_bytes(mpm_solver* s, int64_t* bytes);
 int mpm_diff_substep(mpm_solver* s, float dt, const float* state_in, float* state_out,
-                     const float* youngs, double nu, void* stream);
+                     const float* youngs, double nu, void* binning, void* stream);
 int mpm_diff_substep_backward(mpm_solver* s, float dt, const float* state_in,
-                              const float* youngs, double nu, const float* grad_out,
-                              float* grad_in, float* grad_youngs, float* grad_nu, void* stream);
+                              const float* youngs, double nu, const void* binning,
+                              const float* grad_out, float* grad_in, float* grad_youngs,
+                              float* grad_nu, void* stream);
 int mpm_diff_sync(mpm_solver* s, void* stream);
 
 /* The quantised particle file of ParticleIO.write_particles

@@ -61,9 +61,11 @@ SIGNATURES = [
     ("mpm_last_counts", ctypes.c_int, [_P, _I64P]),
     ("mpm_host_svd", ctypes.c_int, [ctypes.c_int32, _FP, _FP, _FP, _FP]),
     ("mpm_diff_setup", ctypes.c_int, [_P, ctypes.c_int64, _I32P]),
-    ("mpm_diff_substep", ctypes.c_int, [_P, ctypes.c_float, _P, _P, _P, ctypes.c_double, _P]),
+    ("mpm_diff_binning_bytes", ctypes.c_int, [_P, _I64P]),
+    ("mpm_diff_substep", ctypes.c_int, [_P, ctypes.c_float, _P, _P, _P, ctypes.c_double, _P,
+                                        _P]),
     ("mpm_diff_substep_backward", ctypes.c_int, [_P, ctypes.c_float, _P, _P, ctypes.c_double, _P,
-                                                 _P, _P, _P, _P]),
+                                                 _P, _P, _P, _P, _P]),
     ("mpm_diff_sync", ctypes.c_int, [_P, _P]),
     ("mpm_quantize_particles", ctypes.c_int, [_P, _FP, ctypes.POINTER(ctypes.c_uint32),
                                               ctypes.POINTER(ctypes.c_uint8)]),

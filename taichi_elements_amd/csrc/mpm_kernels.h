@@ -40,6 +40,7 @@ enum Total : int { kTotValid = 0, kTotChunks = 1, kTotActive = 2, kTotPBlocks = 
 
 struct SubstepBuffers {
   float* pf;  // particle float components
+  float* pf_out;  // where p2g / g2p write the new state (== pf: in place)
   int* pi;    // particle int components
   int64_t cap;
   int n;
@@ -94,6 +95,11 @@ template <int D>
 void launch_substep(const Params* P, const int* slab, SubstepBuffers& B, int substep_index,
                     int raw_grid, hipStream_t st, Timer* timer);
 
+// p2g + grid operator on the binning already in B
+template <int D>
+void launch_transfer(const Params* P, const int* slab, SubstepBuffers& B, int raw_grid,
+                     hipStream_t st, Timer* timer);
+
 // binning + p2g + grid operator only (raw_grid: keep the unnormalised sums)
 template <int D>
 void launch_front(const Params* P, const int* slab, SubstepBuffers& B, int raw_grid,
@@ -103,11 +109,12 @@ void launch_front(const Params* P, const int* slab, SubstepBuffers& B, int raw_g
 // state pf_in: gout = adjoints of the new state, gin = adjoints of the old
 // state (both laid out like pf), gE (may be null) = adjoint of the
 // per-particle Young's modulus, gnu[0] = adjoint of nu.  gbar holds
-// total_blocks * nodes float4, nu_part persistent_groups floats.
+// total_blocks * nodes float4, abuf D*D * cap floats, nu_part
+// persistent_groups floats.
 template <int D>
 void launch_adjoint(const Params* P, const int* slab, SubstepBuffers& B, const float* pf_in,
-                    const float* gout, float* gin, float* gE, float4* gbar, float* nu_part,
-                    float* gnu, hipStream_t st, Timer* timer);
+                    const float* gout, float* gin, float* gE, float4* gbar, float* abuf,
+                    float* nu_part, float* gnu, hipStream_t st, Timer* timer);
 
 template <int D>
 void launch_reorder(SubstepBuffers& B, float* pf_alt, int* pi_alt, hipStream_t st, Timer* timer);
@@ -125,6 +132,15 @@ void launch_scatter_f(const float* src, int64_t cap, int ncomp, const int* orig,
 void launch_scatter_i(const int* src, int64_t cap, int ncomp, const int* orig, int n, int* dst,
                       hipStream_t st);
 void launch_fill_u32(unsigned* p, unsigned v, int64_t n, hipStream_t st);
+
+// Up to 8 int array copies in one launch.
+struct CopySegments {
+  const int* src[8];
+  int* dst[8];
+  int64_t n[8];
+  int count;
+};
+void launch_copy_segments(const CopySegments& S, hipStream_t st);
 
 // Quantised particle output (mpm_io.hip).  Components 0..D-1 are x, D..2D-1 v.
 struct QuantRanges {

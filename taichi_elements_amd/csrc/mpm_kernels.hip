@@ -470,7 +470,8 @@ __global__ __launch_bounds__(kThreads) void k_p2g(const Params* __restrict__ sce
                                                   const int4* __restrict__ chunks,
                                                   const int* __restrict__ totals,
                                                   float4* __restrict__ tiles,
-                                                  const float* __restrict__ gf) {
+                                                  const float* __restrict__ gf,
+                                                  float* __restrict__ pf_out) {
   using L = Layout<D>;
   constexpr int TN = Geo<D>::kTileNodes;
   constexpr int CELLS = Geo<D>::kNodesPerBlock;  // cells of a block (64 / 16)
@@ -577,8 +578,8 @@ __global__ __launch_bounds__(kThreads) void k_p2g(const Params* __restrict__ sce
 #pragma unroll
       for (int a = 0; a < D; a++)
 #pragma unroll
-        for (int e = 0; e < D; e++) pf[(int64_t)(L::F + a * D + e) * cap + p] = F[a][e];
-      pf[(int64_t)L::JP * cap + p] = Jp;
+        for (int e = 0; e < D; e++) pf_out[(int64_t)(L::F + a * D + e) * cap + p] = F[a][e];
+      pf_out[(int64_t)L::JP * cap + p] = Jp;
       float mass = P.p_mass;  // mpm_solver.py:571-573
       if (mat == kWater) mass *= P.water_density;
       // contribution to node o: w_o * (mass v + A (o - fx) dx) = w_o * (q + Adx o)
@@ -875,7 +876,8 @@ __global__ __launch_bounds__(kThreads) void k_g2p(const Params* __restrict__ sce
                                                   int* __restrict__ code, int* __restrict__ nkey,
                                                   int* __restrict__ dcount,
                                                   int* __restrict__ count_next,
-                                                  int* __restrict__ err) {
+                                                  int* __restrict__ err,
+                                                  float* __restrict__ pf_out, int n) {
   using L = Layout<D>;
   constexpr int NPB = Geo<D>::kNodesPerBlock, TN = Geo<D>::kTileNodes;
   __shared__ float4 tile[TN];
@@ -985,15 +987,25 @@ __global__ __launch_bounds__(kThreads) void k_g2p(const Params* __restrict__ sce
 #pragma unroll
         for (int a = 0; a < D; a++) {
           xn[a] = xp[a] + P.dt * nv[a];
-          pf[(int64_t)(L::V + a) * cap + p] = nv[a];
-          pf[(int64_t)(L::X + a) * cap + p] = xn[a];
+          pf_out[(int64_t)(L::V + a) * cap + p] = nv[a];
+          pf_out[(int64_t)(L::X + a) * cap + p] = xn[a];
           vmax = fmaxf(vmax, fabsf(nv[a]));
 #pragma unroll
-          for (int e = 0; e < D; e++) pf[(int64_t)(L::C + a * D + e) * cap + p] = nC[a][e];
+          for (int e = 0; e < D; e++) pf_out[(int64_t)(L::C + a * D + e) * cap + p] = nC[a][e];
         }
       } else {
 #pragma unroll
-        for (int a = 0; a < D; a++) vmax = fmaxf(vmax, fabsf(pf[(int64_t)(L::V + a) * cap + p]));
+        for (int a = 0; a < D; a++) {
+          const float va = pf[(int64_t)(L::V + a) * cap + p];
+          vmax = fmaxf(vmax, fabsf(va));
+          if (pf_out != pf) {  // out of place: a stationary particle keeps x, v, C
+            pf_out[(int64_t)(L::V + a) * cap + p] = va;
+            pf_out[(int64_t)(L::X + a) * cap + p] = xp[a];
+#pragma unroll
+            for (int e = 0; e < D; e++)
+              pf_out[(int64_t)(L::C + a * D + e) * cap + p] = pf[(int64_t)(L::C + a * D + e) * cap + p];
+          }
+        }
       }
       // block of the new position for the next substep's binning
       // (build_pid, mpm_solver.py:357-361); particles move less than one block
@@ -1055,6 +1067,15 @@ __global__ __launch_bounds__(kThreads) void k_g2p(const Params* __restrict__ sce
       nkey[j] = key_new;
     }
     __syncthreads();
+  }
+  // out of place: particles outside every chunk (escaped the window; sorted
+  // to the end) keep their whole state
+  if (pf_out != pf) {
+    for (int j = totals[kTotValid] + blockIdx.x * kThreads + threadIdx.x; j < n;
+         j += gridDim.x * kThreads) {
+      const int p = perm[j];
+      for (int c = 0; c < L::NF; c++) pf_out[(int64_t)c * cap + p] = pf[(int64_t)c * cap + p];
+    }
   }
   // compute_max_velocity (mpm_solver.py:726-735): non-negative floats order
   // like their bit patterns.
@@ -1231,6 +1252,13 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const T* __restrict__ src,
   }
 }
 
+__global__ __launch_bounds__(kThreads) void k_copy_segments(CopySegments S) {
+  for (int s = 0; s < S.count; s++)
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < S.n[s];
+         i += (int64_t)gridDim.x * kThreads)
+      S.dst[s][i] = S.src[s][i];
+}
+
 __global__ __launch_bounds__(kThreads) void k_fill_u32(unsigned* __restrict__ p, unsigned v,
                                                        int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
@@ -1320,10 +1348,21 @@ void launch_front(const Params* P, const int* slab, SubstepBuffers& B, int raw, 
     std::swap(B.perm, B.perm_alt);
     B.keys_sorted = B.keys_alt;
   }
+  launch_transfer<D>(P, slab, B, raw, st, tm);
+}
+
+// p2g and the grid operator on an existing binning
+template <int D>
+void launch_transfer(const Params* P, const int* slab, SubstepBuffers& B, int raw, hipStream_t st,
+                     Timer* tm) {
+  const int n = B.n;
+  const int G = B.persistent_groups;
+  if (n == 0) return;
+  const int chunk_bound = std::min(B.total_blocks, n) + div_up(n, B.chunk);
   {
     TimerScope ts(tm, "k_p2g", st);
     k_p2g<D, 0><<<clamp_groups(chunk_bound, G), kThreads, 0, st>>>(
-        P, slab, B.pf, B.pi, B.cap, B.perm, B.chunks, B.totals, B.tiles, nullptr);
+        P, slab, B.pf, B.pi, B.cap, B.perm, B.chunks, B.totals, B.tiles, nullptr, B.pf_out);
   }
   {
     TimerScope ts(tm, "k_grid_op", st);
@@ -1346,7 +1385,7 @@ void launch_g2p(const Params* P, const int* slab, SubstepBuffers& B, int s_index
     k_g2p<D><<<clamp_groups(chunk_bound, G), kThreads, 0, st>>>(
         P, slab, B.pf, B.pi, B.cap, B.perm, B.chunks, B.totals, B.grid,
         B.maxv + (int64_t)s_index * B.persistent_groups, B.code, B.nkey, B.dcount, B.count_next,
-        B.err);
+        B.err, B.pf_out, n);
   }
   std::swap(B.count, B.count_next);
 }
@@ -1360,20 +1399,24 @@ void launch_substep(const Params* P, const int* slab, SubstepBuffers& B, int s_i
 
 template <int D>
 void launch_adjoint(const Params* P, const int* slab, SubstepBuffers& B, const float* pf_in,
-                    const float* gout, float* gin, float* gE, float4* gbar, float* nu_part,
-                    float* gnu, hipStream_t st, Timer* tm) {
+                    const float* gout, float* gin, float* gE, float4* gbar, float* abuf,
+                    float* nu_part, float* gnu, hipStream_t st, Timer* tm) {
   const int n = B.n;
   const int G = B.persistent_groups;
   const int chunk_bound = n > 0 ? std::min(B.total_blocks, n) + div_up(n, B.chunk) : 1;
-  // particles outside every chunk (none unless one escaped) keep zero adjoints
-  launch_fill_u32((unsigned*)gin, 0u, (int64_t)Layout<D>::NF * B.cap, st);
-  if (gE) launch_fill_u32((unsigned*)gE, 0u, B.cap, st);
   const int groups = clamp_groups(chunk_bound, G);
+  const int pgroups = clamp_groups(div_up(n, kThreads), G);
   if (n > 0) {
+    {
+      TimerScope ts(tm, "k_adj_affine", st);
+      k_adj_affine<D><<<pgroups, kThreads, 0, st>>>(P, slab, pf_in, B.pi, B.cap, B.perm,
+                                                    B.totals, abuf);
+    }
     {
       TimerScope ts(tm, "k_g2p_adj_scatter", st);
       k_p2g<D, 1><<<groups, kThreads, 0, st>>>(P, slab, const_cast<float*>(pf_in), B.pi, B.cap,
-                                               B.perm, B.chunks, B.totals, B.tiles, gout);
+                                               B.perm, B.chunks, B.totals, B.tiles, gout,
+                                               nullptr);
     }
     {
       TimerScope ts(tm, "k_grid_op_adj", st);
@@ -1384,12 +1427,17 @@ void launch_adjoint(const Params* P, const int* slab, SubstepBuffers& B, const f
     {
       TimerScope ts(tm, "k_p2g_adj", st);
       k_p2g_adj<D><<<groups, kThreads, 0, st>>>(P, slab, pf_in, B.pi, B.cap, B.perm, B.chunks,
-                                                B.totals, B.grid, gbar, gout, gin, gE, nu_part);
+                                                B.totals, B.grid, gbar, gout, abuf, gin, gE, n);
+    }
+    {
+      TimerScope ts(tm, "k_adj_constitutive", st);
+      k_adj_constitutive<D><<<pgroups, kThreads, 0, st>>>(P, slab, pf_in, B.pi, B.cap, B.perm,
+                                                          B.totals, gout, gin, gE, nu_part);
     }
   } else {
-    launch_fill_u32((unsigned*)nu_part, 0u, groups, st);
+    launch_fill_u32((unsigned*)nu_part, 0u, pgroups, st);
   }
-  k_sum_partials<<<1, kThreads, 0, st>>>(nu_part, groups, gnu);
+  k_sum_partials<<<1, kThreads, 0, st>>>(nu_part, pgroups, gnu);
 }
 
 template <int D>
@@ -1431,21 +1479,32 @@ void launch_scatter_i(const int* src, int64_t cap, int ncomp, const int* orig, i
     k_scatter<int><<<clamp_groups(div_up(n, kThreads), 8192), kThreads, 0, st>>>(src, cap, ncomp,
                                                                               orig, n, dst);
 }
+void launch_copy_segments(const CopySegments& S, hipStream_t st) {
+  int64_t most = 0;
+  for (int s = 0; s < S.count; s++) most = std::max(most, S.n[s]);
+  if (most > 0)
+    k_copy_segments<<<clamp_groups(div_up(most, kThreads), 2048), kThreads, 0, st>>>(S);
+}
+
 void launch_fill_u32(unsigned* p, unsigned v, int64_t n, hipStream_t st) {
   if (n > 0)
     k_fill_u32<<<clamp_groups(div_up(n, kThreads), 4096), kThreads, 0, st>>>(p, v, n);
 }
 
+template void launch_transfer<2>(const Params*, const int*, SubstepBuffers&, int, hipStream_t,
+                                 Timer*);
+template void launch_transfer<3>(const Params*, const int*, SubstepBuffers&, int, hipStream_t,
+                                 Timer*);
 template void launch_front<2>(const Params*, const int*, SubstepBuffers&, int, hipStream_t,
                               Timer*);
 template void launch_front<3>(const Params*, const int*, SubstepBuffers&, int, hipStream_t,
                               Timer*);
 template void launch_adjoint<2>(const Params*, const int*, SubstepBuffers&, const float*,
-                                const float*, float*, float*, float4*, float*, float*, hipStream_t,
-                                Timer*);
+                                const float*, float*, float*, float4*, float*, float*, float*,
+                                hipStream_t, Timer*);
 template void launch_adjoint<3>(const Params*, const int*, SubstepBuffers&, const float*,
-                                const float*, float*, float*, float4*, float*, float*, hipStream_t,
-                                Timer*);
+                                const float*, float*, float*, float4*, float*, float*, float*,
+                                hipStream_t, Timer*);
 template void launch_substep<2>(const Params*, const int*, SubstepBuffers&, int, int, hipStream_t,
                                 Timer*);
 template void launch_substep<3>(const Params*, const int*, SubstepBuffers&, int, int, hipStream_t,

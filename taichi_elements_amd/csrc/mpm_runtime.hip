@@ -172,6 +172,7 @@ struct mpm_solver {
   int* diff_pi = nullptr;          // 3 x diff_n: material, insertion index, scene 0
   float* diff_scratch = nullptr;   // NF x diff_n: the state p2g rewrites in backward
   float4* diff_gbar = nullptr;     // grid adjoint
+  float* diff_abuf = nullptr;      // D*D x diff_n: affine matrices of the backward
   float* diff_nu_part = nullptr;   // per-workgroup nu adjoints
   bool diff_params_valid = false;  // the device scene table holds these values:
   float diff_dt = 0.0f;
@@ -326,6 +327,7 @@ int grow(mpm_solver* s, int64_t need) {
 
 void fill_buffers(mpm_solver* s, SubstepBuffers& B) {
   B.pf = s->pf;
+  B.pf_out = s->pf;
   B.pi = s->pi;
   B.cap = s->cap;
   B.n = (int)s->n;
@@ -524,6 +526,7 @@ int mpm_destroy(mpm_solver* s) {
   dfree(s->diff_pi);
   dfree(s->diff_scratch);
   dfree(s->diff_gbar);
+  dfree(s->diff_abuf);
   dfree(s->diff_nu_part);
   if (s->h_params) hipHostFree(s->h_params);
   if (s->stream) hipStreamDestroy(s->stream);
@@ -962,12 +965,14 @@ int mpm_diff_setup(mpm_solver* s, int64_t n, const int32_t* material) {
   dfree(s->diff_pi);
   dfree(s->diff_scratch);
   dfree(s->diff_gbar);
+  dfree(s->diff_abuf);
   dfree(s->diff_nu_part);
   const int NPB = s->D == 3 ? Geo<3>::kNodesPerBlock : Geo<2>::kNodesPerBlock;
   TRY(dalloc(&s->diff_pi, 3 * n));
   TRY(dalloc(&s->diff_scratch, (int64_t)s->nf() * n));
   TRY(dalloc(&s->diff_gbar, (int64_t)s->total_blocks * NPB));
   TRY(dalloc(&s->diff_nu_part, s->persistent_groups));
+  TRY(dalloc(&s->diff_abuf, (int64_t)s->D * s->D * n));
   std::vector<int32_t> host(3 * n);
   for (int64_t i = 0; i < n; i++) {
     host[i] = material[i];
@@ -1007,9 +1012,30 @@ int diff_params(mpm_solver* s, float dt, const float* youngs, double nu, hipStre
   return MPM_OK;
 }
 
+// Saved binning of a differentiable substep: perm, chunks, per-block chunk
+// ranges, active blocks and totals, each 256-byte aligned in one blob.
+struct BinLayout {
+  int64_t perm, chunks, pbeg, pcnt, active, totals, bytes;
+};
+
+BinLayout bin_layout(const mpm_solver* s) {
+  auto al = [](int64_t b) { return (b + 255) & ~(int64_t)255; };
+  const int64_t n = s->diff_n, tb = s->total_blocks;
+  BinLayout L;
+  L.perm = 0;
+  L.chunks = al(4 * n);
+  L.pbeg = L.chunks + al(16 * s->chunk_cap);
+  L.pcnt = L.pbeg + al(4 * tb);
+  L.active = L.pcnt + al(4 * tb);
+  L.totals = L.active + al(4 * tb);
+  L.bytes = L.totals + 256;
+  return L;
+}
+
 void diff_buffers(mpm_solver* s, SubstepBuffers& B, float* pf) {
   fill_buffers(s, B);
   B.pf = pf;
+  B.pf_out = pf;
   B.pi = s->diff_pi;
   B.cap = s->diff_n;
   B.n = (int)s->diff_n;
@@ -1019,23 +1045,44 @@ void diff_buffers(mpm_solver* s, SubstepBuffers& B, float* pf) {
 
 }  // namespace
 
+int mpm_diff_binning_bytes(mpm_solver* s, int64_t* bytes) {
+  if (!s || !bytes) return fail(MPM_EINVAL, "null argument");
+  if (s->diff_n == 0) return fail(MPM_EINVAL, "mpm_diff_setup first");
+  *bytes = bin_layout(s).bytes;
+  return MPM_OK;
+}
+
 int mpm_diff_substep(mpm_solver* s, float dt, const float* state_in, float* state_out,
-                     const float* youngs, double nu, void* stream) {
+                     const float* youngs, double nu, void* binning, void* stream) {
   if (!s || !state_in || !state_out) return fail(MPM_EINVAL, "null argument");
   if (s->diff_n == 0) return fail(MPM_EINVAL, "mpm_diff_setup first");
   hipStream_t st = (hipStream_t)stream;  // NULL: the null stream
   HIP_TRY(hipSetDevice(s->device));
   TRY(diff_params(s, dt, youngs, nu, st));
-  const int64_t n = s->diff_n;
-  if (state_out != state_in)
-    HIP_TRY(hipMemcpyAsync(state_out, state_in, sizeof(float) * s->nf() * n,
-                           hipMemcpyDeviceToDevice, st));
   SubstepBuffers B;
-  diff_buffers(s, B, state_out);
+  diff_buffers(s, B, const_cast<float*>(state_in));
+  B.pf_out = state_out;  // p2g / g2p write the new state out of place
   if (s->D == 3)
     launch_substep<3>(s->d_params, s->d_slab, B, 0, 0, st, nullptr);
   else
     launch_substep<2>(s->d_params, s->d_slab, B, 0, 0, st, nullptr);
+  if (binning) {
+    const BinLayout L = bin_layout(s);
+    char* b = (char*)binning;
+    const int64_t tb = s->total_blocks;
+    CopySegments S{};
+    const int* src[6] = {B.perm, (const int*)B.chunks, B.pchunk_begin, B.pchunk_cnt, B.active,
+                         B.totals};
+    const int64_t off[6] = {L.perm, L.chunks, L.pbeg, L.pcnt, L.active, L.totals};
+    const int64_t cnt[6] = {s->diff_n, 4 * s->chunk_cap, tb, tb, tb, 16};
+    for (int i = 0; i < 6; i++) {
+      S.src[i] = src[i];
+      S.dst[i] = (int*)(b + off[i]);
+      S.n[i] = cnt[i];
+    }
+    S.count = 6;
+    launch_copy_segments(S, st);
+  }
   take_buffers(s, B);
   s->substep_counter++;
   HIP_TRY(hipGetLastError());
@@ -1043,31 +1090,49 @@ int mpm_diff_substep(mpm_solver* s, float dt, const float* state_in, float* stat
 }
 
 int mpm_diff_substep_backward(mpm_solver* s, float dt, const float* state_in,
-                              const float* youngs, double nu, const float* grad_out,
-                              float* grad_in, float* grad_youngs, float* grad_nu, void* stream) {
+                              const float* youngs, double nu, const void* binning,
+                              const float* grad_out, float* grad_in, float* grad_youngs,
+                              float* grad_nu, void* stream) {
   if (!s || !state_in || !grad_out || !grad_in || !grad_nu)
     return fail(MPM_EINVAL, "null argument");
   if (s->diff_n == 0) return fail(MPM_EINVAL, "mpm_diff_setup first");
   hipStream_t st = (hipStream_t)stream;  // NULL: the null stream
   HIP_TRY(hipSetDevice(s->device));
   TRY(diff_params(s, dt, youngs, nu, st));
-  const int64_t n = s->diff_n;
-  // the forward's binning and raw grid sums, recomputed from the old state
-  HIP_TRY(hipMemcpyAsync(s->diff_scratch, state_in, sizeof(float) * s->nf() * n,
-                         hipMemcpyDeviceToDevice, st));
+  // the forward's raw grid sums recomputed from the old state (p2g writes
+  // its F / Jp into scratch), on the saved binning or a recomputed one
   SubstepBuffers B;
-  diff_buffers(s, B, s->diff_scratch);
-  if (s->D == 3) {
-    launch_front<3>(s->d_params, s->d_slab, B, 1, st, nullptr);
-    launch_adjoint<3>(s->d_params, s->d_slab, B, state_in, grad_out, grad_in, grad_youngs,
-                      s->diff_gbar, s->diff_nu_part, grad_nu, st, nullptr);
+  diff_buffers(s, B, const_cast<float*>(state_in));
+  B.pf_out = s->diff_scratch;
+  if (binning) {
+    const BinLayout L = bin_layout(s);
+    char* b = (char*)binning;
+    B.perm = (int*)(b + L.perm);
+    B.chunks = (int4*)(b + L.chunks);
+    B.pchunk_begin = (int*)(b + L.pbeg);
+    B.pchunk_cnt = (int*)(b + L.pcnt);
+    B.active = (int*)(b + L.active);
+    B.totals = (int*)(b + L.totals);
+    if (s->D == 3)
+      launch_transfer<3>(s->d_params, s->d_slab, B, 1, st, nullptr);
+    else
+      launch_transfer<2>(s->d_params, s->d_slab, B, 1, st, nullptr);
   } else {
-    launch_front<2>(s->d_params, s->d_slab, B, 1, st, nullptr);
-    launch_adjoint<2>(s->d_params, s->d_slab, B, state_in, grad_out, grad_in, grad_youngs,
-                      s->diff_gbar, s->diff_nu_part, grad_nu, st, nullptr);
+    if (s->D == 3)
+      launch_front<3>(s->d_params, s->d_slab, B, 1, st, nullptr);
+    else
+      launch_front<2>(s->d_params, s->d_slab, B, 1, st, nullptr);
   }
-  take_buffers(s, B);
-  s->substep_counter++;
+  if (s->D == 3)
+    launch_adjoint<3>(s->d_params, s->d_slab, B, state_in, grad_out, grad_in, grad_youngs,
+                      s->diff_gbar, s->diff_abuf, s->diff_nu_part, grad_nu, st, nullptr);
+  else
+    launch_adjoint<2>(s->d_params, s->d_slab, B, state_in, grad_out, grad_in, grad_youngs,
+                      s->diff_gbar, s->diff_abuf, s->diff_nu_part, grad_nu, st, nullptr);
+  if (!binning) {  // the block scan ran: keep the substep-parity bookkeeping
+    take_buffers(s, B);
+    s->substep_counter++;
+  }
   HIP_TRY(hipGetLastError());
   return MPM_OK;
 }

@@ -50,23 +50,32 @@ class _Substep(torch.autograd.Function):
     @staticmethod
     def forward(ctx, state, youngs, nu, solver, nu_value, dt):
         out = torch.empty_like(state)
+        # the substep's binning, kept for the backward pass (which then only
+        # recomputes the grid)
+        binning = torch.empty(solver._binning_bytes, dtype=torch.uint8, device=state.device)
         check(_lib.lib().mpm_diff_substep(solver._handle, float(dt), _ptr(state), _ptr(out),
-                                          _ptr(youngs), float(nu_value), _stream()))
-        ctx.save_for_backward(state, youngs)
+                                          _ptr(youngs), float(nu_value), _ptr(binning),
+                                          _stream()))
+        ctx.mark_non_differentiable(binning)
+        ctx.set_materialize_grads(False)
+        ctx.save_for_backward(state, youngs, binning)
         ctx.solver, ctx.nu_value, ctx.dt = solver, nu_value, dt
         ctx.nu_shape, ctx.nu_dtype = nu.shape, nu.dtype
-        return out
+        return out, binning
 
     @staticmethod
-    def backward(ctx, grad_out):
-        state, youngs = ctx.saved_tensors
+    def backward(ctx, grad_out, _grad_binning):
+        state, youngs, binning = ctx.saved_tensors
+        if grad_out is None:
+            grad_out = torch.zeros_like(state)
         grad_out = grad_out.contiguous()
         grad_in = torch.empty_like(state)
         grad_e = torch.empty_like(youngs)
         grad_nu = torch.empty(1, dtype=torch.float32, device=state.device)
         check(_lib.lib().mpm_diff_substep_backward(
             ctx.solver._handle, float(ctx.dt), _ptr(state), _ptr(youngs), float(ctx.nu_value),
-            _ptr(grad_out), _ptr(grad_in), _ptr(grad_e), _ptr(grad_nu), _stream()))
+            _ptr(binning), _ptr(grad_out), _ptr(grad_in), _ptr(grad_e), _ptr(grad_nu),
+            _stream()))
         return (grad_in, grad_e, grad_nu.reshape(ctx.nu_shape).to(ctx.nu_dtype), None, None,
                 None)
 
@@ -96,6 +105,7 @@ class DiffMPMSolver:
         self.nu = self._solver.nu
         self.device = torch.device("cuda", device)
         self.n = 0
+        self._order = self._inverse = None
 
     @property
     def _handle(self):
@@ -106,10 +116,32 @@ class DiffMPMSolver:
         """The underlying MPMSolver (constants, set_gravity)."""
         return self._solver
 
-    def setup(self, material):
+    def setup(self, material, positions=None):
+        """Fix the particles' materials (water / elastic).  With the initial
+        ``positions`` given, the substeps work on the particles in a
+        spatially sorted order (by 4^d grid block) so that the transfers
+        read coherent memory; ``run`` maps to and from the caller's order
+        with differentiable gathers."""
         material = np.ascontiguousarray(np.asarray(material, np.int32).reshape(-1))
-        check(_lib.lib().mpm_diff_setup(self._handle, len(material), _lib.iptr(material)))
-        self.n = len(material)
+        n = len(material)
+        self._order = self._inverse = None
+        if positions is not None:
+            x = np.asarray(positions, np.float32).reshape(n, self.dim)
+            base = np.floor(x * np.float32(self._solver.inv_dx) - np.float32(0.5)).astype(np.int64)
+            blk = (base >> 2) + 1024
+            key = np.zeros(n, np.int64)
+            for a in reversed(range(self.dim)):
+                key = key * 4096 + blk[:, a]
+            order = np.argsort(key, kind="stable")
+            self._order = torch.as_tensor(order, device=self.device)
+            self._inverse = torch.empty_like(self._order)
+            self._inverse[self._order] = torch.arange(n, device=self.device)
+            material = np.ascontiguousarray(material[order])
+        check(_lib.lib().mpm_diff_setup(self._handle, n, _lib.iptr(material)))
+        nbytes = _lib.ctypes.c_int64()
+        check(_lib.lib().mpm_diff_binning_bytes(self._handle, nbytes))
+        self._binning_bytes = int(nbytes.value)
+        self.n = n
         self.material = material
 
     def state(self, x, v=None):
@@ -134,24 +166,24 @@ class DiffMPMSolver:
         """One differentiable substep.  ``youngs``: (n,) float32 tensor;
         ``nu``: 0-d tensor (its value is read on the host unless
         ``nu_value`` is given)."""
-        if dt is None:
-            dt = self.default_dt
-        if nu_value is None:
-            nu_value = float(nu.detach()) if torch.is_tensor(nu) else float(nu)
-        return _Substep.apply(state, youngs, nu, self, nu_value, dt)
+        return self.run(state, youngs, nu, 1, dt=dt, checkpoint_every=1, nu_value=nu_value)
 
     def _segment(self, state, youngs, nu, nu_value, dt, k):
         for _ in range(k):
-            state = _Substep.apply(state, youngs, nu, self, nu_value, dt)
+            state, _ = _Substep.apply(state, youngs, nu, self, nu_value, dt)
         return state
 
-    def run(self, state, youngs, nu, nsub, dt=None, checkpoint_every=8):
+    def run(self, state, youngs, nu, nsub, dt=None, checkpoint_every=8, nu_value=None):
         """``nsub`` substeps; with gradients enabled and checkpoint_every > 1
         only every checkpoint_every-th state is kept and each segment is
         recomputed in the backward pass."""
         if dt is None:
             dt = self.default_dt
-        nu_value = float(nu.detach()) if torch.is_tensor(nu) else float(nu)
+        if nu_value is None:
+            nu_value = float(nu.detach()) if torch.is_tensor(nu) else float(nu)
+        if self._order is not None:
+            state = state[:, self._order]
+            youngs = youngs[self._order]
         k = max(1, int(checkpoint_every))
         for t0 in range(0, nsub, k):
             m = min(k, nsub - t0)
@@ -160,6 +192,8 @@ class DiffMPMSolver:
                                                           nu_value, dt, m, use_reentrant=False)
             else:
                 state = self._segment(state, youngs, nu, nu_value, dt, m)
+        if self._inverse is not None:
+            state = state[:, self._inverse]
         return state
 
     def check(self):

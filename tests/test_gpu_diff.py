@@ -49,7 +49,7 @@ def test_gradients_against_float64_finite_differences(oracle, dim):
     x, v, mat, E, target = _scene(dim, n_el, n_wa, 7 + dim, res)
     nu0 = 0.2
     ds = DiffMPMSolver((res, ) * dim)
-    ds.setup(mat)
+    ds.setup(mat, x if dim == 3 else None)  # 3D: spatially sorted order
     dt = ds.default_dt
     dev = ds.device
     x_t = torch.tensor(x, device=dev, requires_grad=True)
@@ -104,7 +104,7 @@ def test_checkpointing_does_not_change_gradients():
     res, n_el, n_wa = 32, 300, 300
     x, v, mat, E, target = _scene(3, n_el, n_wa, 3, res)
     ds = DiffMPMSolver((res, ) * 3)
-    ds.setup(mat)
+    ds.setup(mat, x)
     grads = []
     for k in (1, 3, 8):
         v_t = torch.tensor(v, device=ds.device, requires_grad=True)
@@ -132,7 +132,7 @@ def test_forward_matches_the_engine_substep():
     plain.substep(plain.default_dt)
     ref = plain.get_state()
     ds = DiffMPMSolver((res, ) * 3)
-    ds.setup(mat)
+    ds.setup(mat, x)
     E = torch.full((len(x), ), float(ds.E), device=ds.device)
     nu = torch.tensor(float(ds.nu), device=ds.device)
     with torch.no_grad():
