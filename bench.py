@@ -99,7 +99,7 @@ def make_batched_scenes(seed=3, nscenes=64):
     return scenes
 
 
-def batched_bench(steps, warmup):
+def batched_bench(steps, warmup, autotune=True):
     """The 64-scene batch advanced by one engine (every stage one launch over
     all scenes) against a loop of plain solves of the same scenes, one
     MPMSolver per scene.  Both advance every scene by `steps` substeps of its
@@ -114,6 +114,10 @@ def batched_bench(steps, warmup):
     for sc, sp in zip(b, specs):
         sc.read_restart(len(sp["x"]), sp["x"], sp["v"], sp["m"], np.zeros(len(sp["x"]), np.int32))
     dts = np.ascontiguousarray(np.asarray(b.default_dts(), np.float32))
+    tuning = None
+    if autotune:
+        b.autotune(substeps=3)
+        tuning = dict(zip(("chunk", "groups"), b.tuning()))
     solos = []
     for sp in specs:
         s = MPMSolver(res=(sp["res"], ) * 3)
@@ -157,7 +161,7 @@ def batched_bench(steps, warmup):
     check(L.mpm_set_timing(b._handle, 0))
     kern = {nm: round(1000 * t / max(c, 1), 2) for nm, c, t in zip(names, calls, ms)}
     counts = b.last_counts()
-    out = dict(scenes=len(specs), particles=total, substeps=steps,
+    out = dict(scenes=len(specs), particles=total, substeps=steps, tuning=tuning,
                kernels_us=kern, active_grid_blocks=counts["active_blocks"],
                chunks=counts["chunks"],
                value=total * steps / res["batch"],
@@ -247,6 +251,37 @@ def fwd_bwd_bench(nsub=64, checkpoint_every=8, repeats=3, seed=4):
                          "gradients w.r.t. v0, E, nu")
 
 
+def g2p2g_bench(x, v, m, steps, warmup, autotune=True):
+    """The same 1M-particle workload with use_g2p2g=True (one fused transfer
+    per substep, grid velocity clamp at 0.9 cells)."""
+    from taichi_elements_amd import MPMSolver
+    from taichi_elements_amd._lib import check
+    from taichi_elements_amd._lib import lib as _l
+    L = _l()
+    s = MPMSolver(res=(RES, RES, RES), use_g2p2g=True)
+    s.read_restart(len(x), x, v, m, np.zeros(len(x), np.int32))
+    if autotune:
+        s.autotune(substeps=3)
+    dt = s.default_dt
+
+    def run(k):
+        check(L.mpm_substeps(s._handle, k, dt, 0.0))
+        check(L.mpm_sync(s._handle))
+
+    run(warmup)
+    t0 = time.perf_counter()
+    run(steps)
+    el = time.perf_counter() - t0
+    check(L.mpm_set_timing(s._handle, 1))
+    run(steps)
+    names, calls, ms = s.kernel_times()
+    check(L.mpm_set_timing(s._handle, 0))
+    return dict(value=len(x) * steps / el, ms_per_substep=1000 * el / steps,
+                unit="particle-substeps/s", tuning=dict(zip(("chunk", "groups"), s.tuning())),
+                kernels_us={nm: round(1000 * t / max(c, 1), 2) for nm, c, t in zip(names, calls, ms)},
+                workload="3D four-material 1M particles, 128^3, use_g2p2g=True")
+
+
 def algorithmic_bytes(n_particles, n_nodes):
     """Bytes a substep must move in the reference's field layout (float32):
     per particle 16 (binning) + 148 (p2g) + 80 (g2p), per active grid node
@@ -313,8 +348,12 @@ def main():
                          "headline line); sparse16m: 512^3 window, 16M particles in 64 "
                          "ellipsoids with a plane collider; fwdbwd200k: only the "
                          "differentiable 200k-particle forward + backward measurement")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shape instead of autotuning it on the scene")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the 64-scene batched measurement")
+    ap.add_argument("--no-g2p2g", action="store_true",
+                    help="skip the use_g2p2g measurement")
     ap.add_argument("--no-fwd-bwd", action="store_true",
                     help="skip the differentiable (forward + backward) measurement")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -358,6 +397,12 @@ def main():
     s.read_restart(n, x, v, m, np.zeros(n, np.int32))
     dt = s.default_dt
     L = __import__("taichi_elements_amd._lib", fromlist=["lib"]).lib()
+    tuning = None
+    if not args.no_autotune and not args.chunk:
+        cand = s.autotune(substeps=3)
+        tuning = dict(chosen=dict(zip(("chunk", "groups"), s.tuning())),
+                      ms_per_substep={f"{c}/{g}": round(v, 4) for (c, g), v in cand.items()})
+        print(f"[bench] autotuned launch shape: {tuning['chosen']}", file=sys.stderr, flush=True)
 
     def run(k):
         from taichi_elements_amd._lib import check
@@ -422,14 +467,18 @@ def main():
                         "frac": substep_gbs / HBM_PEAK_GBS},
         "kernels_us": {k: round(v_["avg_us"], 2) for k, v_ in kern.items()},
         "timed_with_events_ms_per_step": 1000 * (t3 - t2) / args.steps,
+        "tuning": tuning,
+        "g2p2g": None,
         "batched": None,
         "fwd_bwd": None,
     }
     if args.copy_calibration:
         out["copy_calibration_bytes"] = copy_calibration()
-    if not args.no_batched and args.config == "dense1m":
+    if not args.no_g2p2g and args.config == "dense1m":
         del s
-        out["batched"] = batched_bench(args.steps, args.warmup)
+        out["g2p2g"] = g2p2g_bench(x, v, m, args.steps, args.warmup, not args.no_autotune)
+    if not args.no_batched and args.config == "dense1m":
+        out["batched"] = batched_bench(args.steps, args.warmup, not args.no_autotune)
     if not args.no_fwd_bwd and args.config == "dense1m":
         out["fwd_bwd"] = fwd_bwd_bench()
     if not args.no_cpu_baseline and args.config == "dense1m":

@@ -183,6 +183,26 @@ int mpm_kernel_times(mpm_solver* s, char* names, int32_t names_cap, int32_t* cou
  * A = U diag(sig) V^T, row-major dim x dim. */
 int mpm_host_svd(int32_t dim, const float* A, float* U, float* sig, float* V);
 
+/* use_g2p2g mode (MPMSolver(use_g2p2g=True, v_clamp_g2p2g, g2p2g_allowed_cfl),
+ * mpm_solver.py:363-485, 586-598, 773-787): every substep is one fused
+ * g2p2g transfer -- G2P from the previous grid, advection, p2g at the new
+ * positions into the other grid -- followed by the grid operator with each
+ * velocity component clamped to dx * allowed_cfl / dt when clamp != 0 and
+ * allowed_cfl > 0.  Particles added since the previous substep skip the G2P
+ * once (they keep their velocity).  Must be chosen before the first substep.
+ * A particle that moves more than one cell in a substep (only possible
+ * without the clamp) is reported as MPM_EDOMAIN. */
+int mpm_set_g2p2g(mpm_solver* s, int32_t enabled, double allowed_cfl, int32_t clamp);
+
+/* Launch tuning (no reference counterpart; the reference leaves launch
+ * shapes to Taichi).  chunk: particles per transfer work item (16-256);
+ * groups: workgroups of the persistent transfer kernels (64-16384); 0 keeps
+ * a value.  The next substep re-bins from scratch; results change only by
+ * float32 summation order.  MPMSolver.autotune() times candidates on the
+ * live scene and keeps the fastest. */
+int mpm_set_tuning(mpm_solver* s, int32_t chunk, int32_t groups);
+int mpm_get_tuning(mpm_solver* s, int32_t* chunk, int32_t* groups);
+
 /* Bookkeeping of the last substep: number of valid particles, transfer
  * work items (chunks), active grid blocks, and particle blocks. */
 int mpm_last_counts(mpm_solver* s, int64_t* out4);

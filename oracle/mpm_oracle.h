@@ -64,6 +64,9 @@ typedef struct {
    * engine's differentiable mode; the reference has one global E). */
   const double* youngs;
   double nu;
+  /* g2p2g mode's grid velocity clamp (mpm:589, 596-598): each component
+   * limited to dx * v_clamp_cfl / dt; 0 = no clamp (the p2g/g2p path). */
+  double v_clamp_cfl;
 } oracle_cfg;
 
 /* ---- float instantiation ---- */
@@ -90,6 +93,11 @@ void oracle_substep_f32(const oracle_cfg* cfg, int64_t n, float* x, float* v,
                         const int32_t* material, float dt, double t,
                         float* grid_m, float* grid_v, float* max_v,
                         int nthreads);
+/* g2p2g (mpm:363-485): G2P from grid_v_in, advection, p2g into the zeroed
+ * output grid; particles with index >= n_final skip the G2P. */
+void oracle_g2p2g_f32(const oracle_cfg* cfg, int64_t n, float* x, float* v, float* F,
+                      float* Jp, const int32_t* material, float dt, int64_t n_final,
+                      const float* grid_v_in, float* grid_m_out, float* grid_v_out);
 
 /* ---- double instantiation ---- */
 void oracle_svd_f64(int dim, const double* A, double* U, double* sig,
@@ -113,6 +121,9 @@ void oracle_substep_f64(const oracle_cfg* cfg, int64_t n, double* x,
                         const int32_t* material, double dt, double t,
                         double* grid_m, double* grid_v, double* max_v,
                         int nthreads);
+void oracle_g2p2g_f64(const oracle_cfg* cfg, int64_t n, double* x, double* v, double* F,
+                      double* Jp, const int32_t* material, double dt, int64_t n_final,
+                      const double* grid_v_in, double* grid_m_out, double* grid_v_out);
 
 /* ---- binning (integer work; f32 positions as in the reference) ----
  * Block key of a particle: base = floor(x*inv_dx - 0.5) computed in f32

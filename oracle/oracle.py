@@ -40,7 +40,8 @@ class Cfg(ctypes.Structure):
                 ("gravity", ctypes.c_double * 3),
                 ("grid_lo", ctypes.c_int32 * 3), ("grid_n", ctypes.c_int32 * 3),
                 ("post", PostOp * MAX_POST),
-                ("youngs", ctypes.POINTER(ctypes.c_double)), ("nu", ctypes.c_double)]
+                ("youngs", ctypes.POINTER(ctypes.c_double)), ("nu", ctypes.c_double),
+                ("v_clamp_cfl", ctypes.c_double)]
 
 
 def build():
@@ -240,6 +241,22 @@ def g2p(cfg, state, grid_v, dt, dtype=np.float32, nthreads=1):
     return s, float(mv[0])
 
 
+def g2p2g(cfg, state, grid_v_in, dt, n_final=0, dtype=np.float32):
+    """One g2p2g transfer (mpm_solver.py:363-485): returns (grid_m_out,
+    grid_momentum_out, new_state); the output grid is not normalised."""
+    npt, ct, suf = _types(dtype)
+    s = _state(state, npt)
+    n = len(s["Jp"])
+    gm = np.zeros(grid_shape(cfg), npt)
+    gv = np.zeros(grid_shape(cfg) + (cfg.dim, ), npt)
+    gin = np.ascontiguousarray(grid_v_in, dtype=npt)
+    fn = getattr(lib(), "oracle_g2p2g_" + suf)
+    fn(ctypes.byref(cfg), ctypes.c_int64(n), _ptr(s["x"], ct), _ptr(s["v"], ct), _ptr(s["F"], ct),
+       _ptr(s["Jp"], ct), _ptr(s["material"], ctypes.c_int32), ct(dt), ctypes.c_int64(n_final),
+       _ptr(gin, ct), _ptr(gm, ct), _ptr(gv, ct))
+    return gm, gv, s
+
+
 class Substepper:
     """Runs whole substeps in place on one state (grid buffers reused)."""
 
@@ -262,6 +279,51 @@ class Substepper:
                 _ptr(self.gm, ct), _ptr(self.gv, ct), _ptr(self.mv, ct),
                 ctypes.c_int(self.nthreads))
         return float(self.mv[0])
+
+
+def with_clamp(cfg, cfl):
+    """A copy of cfg with the g2p2g grid velocity clamp (mpm_solver.py:589,
+    596-598) at ``cfl`` cells per substep (0: none)."""
+    c = Cfg()
+    ctypes.pointer(c)[0] = cfg
+    c.v_clamp_cfl = float(cfl)
+    if hasattr(cfg, "_youngs_ref"):
+        c._youngs_ref = cfg._youngs_ref
+    return c
+
+
+class G2P2GStepper:
+    """use_g2p2g substeps (mpm_solver.py:773-787) on one state: g2p2g from
+    the input grid into the output grid, grid operator on the output, swap.
+    ``n_final`` = particles that existed at the previous g2p2g substep
+    (0: every particle is new, as on the first substep)."""
+
+    def __init__(self, cfg, state, dtype=np.float32, n_final=0):
+        self.npt, self.ct, suf = _types(dtype)
+        self.cfg = cfg
+        self.s = _state(state, self.npt)
+        self.n = len(self.s["Jp"])
+        shape = grid_shape(cfg)
+        self.gm = np.zeros(shape, self.npt)
+        self.gv = np.zeros(shape + (cfg.dim, ), self.npt)
+        self.gv_in = np.zeros(shape + (cfg.dim, ), self.npt)
+        self.n_final = n_final
+        self.fn = getattr(lib(), "oracle_g2p2g_" + suf)
+        self.gop = getattr(lib(), "oracle_grid_op_" + suf)
+
+    def substep(self, dt, t=0.0):
+        s, ct = self.s, self.ct
+        self.gm.fill(0)
+        self.gv.fill(0)
+        self.fn(ctypes.byref(self.cfg), ctypes.c_int64(self.n), _ptr(s["x"], ct), _ptr(s["v"], ct),
+                _ptr(s["F"], ct), _ptr(s["Jp"], ct), _ptr(s["material"], ctypes.c_int32), ct(dt),
+                ctypes.c_int64(self.n_final), _ptr(self.gv_in, ct), _ptr(self.gm, ct),
+                _ptr(self.gv, ct))
+        self.gop(ctypes.byref(self.cfg), ct(dt), ctypes.c_double(t), _ptr(self.gm, ct),
+                 _ptr(self.gv, ct), ctypes.c_int(1))
+        self.gv_in, self.gv = self.gv, self.gv_in
+        self.n_final = self.n
+        return float(np.abs(s["v"]).max()) if self.n else 0.0
 
 
 def substep(cfg, state, dt, t=0.0, dtype=np.float32, nthreads=1):

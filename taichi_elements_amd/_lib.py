@@ -60,6 +60,9 @@ SIGNATURES = [
                                         ctypes.POINTER(ctypes.c_double), ctypes.c_int32, _I32P]),
     ("mpm_last_counts", ctypes.c_int, [_P, _I64P]),
     ("mpm_host_svd", ctypes.c_int, [ctypes.c_int32, _FP, _FP, _FP, _FP]),
+    ("mpm_set_g2p2g", ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_double, ctypes.c_int32]),
+    ("mpm_set_tuning", ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),
+    ("mpm_get_tuning", ctypes.c_int, [_P, _I32P, _I32P]),
     ("mpm_diff_setup", ctypes.c_int, [_P, ctypes.c_int64, _I32P]),
     ("mpm_diff_binning_bytes", ctypes.c_int, [_P, _I64P]),
     ("mpm_diff_substep", ctypes.c_int, [_P, ctypes.c_float, _P, _P, _P, ctypes.c_double, _P,

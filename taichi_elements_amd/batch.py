@@ -266,6 +266,28 @@ class MPMSolverBatch:
     def sync(self):
         check(_lib.lib().mpm_sync(self._handle))
 
+    def n_particles_total(self):
+        return self._total_particles()
+
+    def set_tuning(self, chunk=0, groups=0):
+        check(_lib.lib().mpm_set_tuning(self._handle, int(chunk), int(groups)))
+
+    def tuning(self):
+        return MPMSolver.tuning(self)
+
+    def autotune(self, substeps=4, chunks=(64, 128, 256), groups=(1024, 2048, 4096),
+                 verbose=False):
+        """MPMSolver.autotune for the whole batch (each scene at its
+        default_dt); the state of every scene is restored afterwards."""
+        from .mpm_solver import _autotune
+
+        def restore(st):
+            for name, a in st.items():
+                self._set_state_all(name, a)
+
+        return _autotune(self, self._handle, None, self._get_state_all, restore, substeps,
+                         chunks, groups, verbose, self.default_dts)
+
     def kernel_times(self):
         return MPMSolver.kernel_times(self)
 

@@ -61,6 +61,11 @@ struct Params {
   // their derivatives in nu.  youngs == nullptr: the global mu_0, lambda_0.
   const float* youngs;
   float kmu, kla, dkmu, dkla;
+  // use_g2p2g mode (mpm_solver.py:363-485, 773-787): 1 selects the fused
+  // transfer; v_allowed = dx * g2p2g_allowed_cfl / dt clamps every grid
+  // velocity component (mpm:589, 596-598; <= 0: no clamp).
+  int g2p2g;
+  float v_allowed;
   int n_post;
   PostOp post[kMaxPost];
 };
@@ -303,12 +308,16 @@ __device__ __forceinline__ void sand_projection(float sig[3], float& Jp, const P
 
 // Per-particle F / Jp update and Kirchhoff stress, mpm_solver.py:507-568.
 // F is updated in place; stress receives tau (before the p2g scale).
-template <int D>
+// G2P2G = true: the update inside g2p2g (mpm_solver.py:416-473), which
+// differs for water: F is not rebuilt from Jp first, the hardening
+// exp(10 (1 - Jp)) applies, and Jp is left unchanged (the volume is kept
+// in F[0][0] only).
+template <int D, bool G2P2G = false>
 __device__ __forceinline__ void constitutive(int mat, const Params& P, float mu_0, float lambda_0,
                                              float F[3][3], const float C[3][3], float& Jp,
                                              float stress[3][3]) {
   float G[3][3];
-  if (mat == kWater) {  // mpm_solver.py:508-511
+  if (mat == kWater && !G2P2G) {  // mpm_solver.py:508-511
 #pragma unroll
     for (int i = 0; i < D; i++)
 #pragma unroll
@@ -345,8 +354,12 @@ __device__ __forceinline__ void constitutive(int mat, const Params& P, float mu_
         stress[i][j] = 0.0f;
       }
     F[0][0] = J;
-    Jp = J;
-    const float iso = lambda_0 * J * (J - 1.0f);
+    float la = lambda_0;
+    if (G2P2G)
+      la = lambda_0 * __expf(10.0f * (1.0f - Jp));  // mpm:422-424 (Jp stays 1 unless set)
+    else
+      Jp = J;
+    const float iso = la * J * (J - 1.0f);
 #pragma unroll
     for (int i = 0; i < D; i++) stress[i][i] = iso;
     return;

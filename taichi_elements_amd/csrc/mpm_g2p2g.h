@@ -1,0 +1,539 @@
+// use_g2p2g mode: the fused transfer of mpm_solver.py:363-485, device code
+// included by mpm_kernels.hip.
+//
+// One work item is a chunk of the particles binned (by their current x) in
+// block b.  The workgroup gathers the input grid's (4+2)^d node tile around
+// b (G2P: v', C'; particles added since the last substep skip it and keep v
+// with C' = 0), advects the particles (not the stationary ones), applies the
+// constitutive update with C' (the particle's C field is not written, as in
+// the reference), and scatters the new momenta around the NEW positions into
+// an (4+4)^d node tile whose origin is one node below the block: a particle
+// moves less than one cell per substep under the grid velocity clamp, so its
+// new stencil stays inside.  Particles whose new cell lies in the block's own
+// 4^d cells go through the cell-major conflict-free LDS scatter of p2g; the
+// few that crossed into a neighbour's cells are added with LDS float atomics
+// afterwards.  The tile goes to HBM, and the grid operator (MODE 2) sums the
+// tiles of the 3^d neighbouring blocks into the output grid.  The same pass
+// writes the next substep's binning hints (direction codes and ranks).
+#pragma once
+
+#include "mpm_device.h"
+
+namespace mpm {
+
+template <int D>
+struct G2P2GGeo {
+  static constexpr int kEdge = kBlock + 4;  // output tile nodes per axis
+  static constexpr int kNodes = D == 3 ? kEdge * kEdge * kEdge : kEdge * kEdge;
+};
+
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_g2p2g(
+    const Params* __restrict__ scenes, const int* __restrict__ slab, float* __restrict__ pf,
+    const int* __restrict__ pi, int64_t cap, const int* __restrict__ perm,
+    const int4* __restrict__ chunks, const int* __restrict__ totals,
+    const float4* __restrict__ grid_in, float4* __restrict__ tiles, int n_final,
+    unsigned* __restrict__ maxv, int* __restrict__ code, int* __restrict__ nkey,
+    int* __restrict__ dcount, int* __restrict__ count_next, int* __restrict__ err) {
+  using L = Layout<D>;
+  constexpr int NPB = Geo<D>::kNodesPerBlock, TN = Geo<D>::kTileNodes;
+  constexpr int E8 = G2P2GGeo<D>::kEdge, TN8 = G2P2GGeo<D>::kNodes;
+  constexpr int CELLS = NPB;
+  constexpr int GROUPS = 64 / CELLS;
+  constexpr int WAVES = kThreads / 64;
+  constexpr int SUBS = WAVES * GROUPS;
+  constexpr int kBatch = 2;
+  __shared__ float4 tin[TN];
+  __shared__ float4 stage[kChunkMax][4];
+  __shared__ float4 ptile[SUBS][TN8];
+  __shared__ int wcnt[WAVES][CELLS];
+  __shared__ int cstart[CELLS], ccount[CELLS];
+  __shared__ unsigned char order[kChunkMax];
+  __shared__ int dwcnt[WAVES][32];
+  __shared__ float red[WAVES];
+
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  float vmax = 0.0f;
+  const int nchunks = totals[kTotChunks];
+  for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    const int4 ck = chunks[c];
+    const int nc = ck.z - ck.y;
+    int lb[3], lbl[3], b[3] = {0, 0, 0};
+    decode_local<D>(ck.x, scenes[0], lb);
+    const Params& P = scenes[scene_of<D>(lb, slab)];
+    to_local<D>(lb, P, lbl);
+#pragma unroll
+    for (int a = 0; a < D; a++) b[a] = (lbl[a] + P.blo[a]) * kBlock;
+    for (int i = threadIdx.x; i < SUBS * TN8; i += kThreads)
+      (&ptile[0][0])[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (int i = threadIdx.x; i < WAVES * CELLS; i += kThreads) (&wcnt[0][0])[i] = 0;
+    for (int i = threadIdx.x; i < WAVES * 32; i += kThreads) (&dwcnt[0][0])[i] = 0;
+    for (int i = threadIdx.x; i < TN; i += kThreads) {
+      const int tl[3] = {i % kTile, (i / kTile) % kTile, i / (kTile * kTile)};
+      int q[3], ln[3];
+#pragma unroll
+      for (int a = 0; a < D; a++) {
+        const int hi = tl[a] >= kBlock;
+        q[a] = lb[a] + hi;
+        ln[a] = tl[a] - kBlock * hi;
+      }
+      const int gb = encode_local<D>(q, P);
+      const int node = D == 3 ? ln[0] + 4 * (ln[1] + 4 * ln[2]) : ln[0] + 4 * ln[1];
+      tin[i] = grid_in[(int64_t)gb * NPB + node];
+    }
+    __syncthreads();
+
+    // ---- phase 1: G2P, advection, constitutive update, staging ----
+    const int t = threadIdx.x;
+    const bool live = t < nc;
+    bool inner = false;
+    int cell = 0, shell = -1;  // shell: output-tile node of the new base (crossed particles)
+    int dir = 13, key_new = ck.x;
+    if (live) {
+      const int p = perm[ck.y + t];
+      const int mat = pi[(int64_t)L::MAT * cap + p];
+      float xp[3] = {0, 0, 0}, fx[3] = {0, 0, 0}, w[3][3], vp[3] = {0, 0, 0};
+      int l[3] = {0, 0, 0};
+#pragma unroll
+      for (int a = 0; a < 3; a++) w[a][0] = w[a][1] = w[a][2] = 0.0f;
+#pragma unroll
+      for (int a = 0; a < D; a++) {
+        xp[a] = pf[(int64_t)(L::X + a) * cap + p];
+        vp[a] = pf[(int64_t)(L::V + a) * cap + p];
+        int base;
+        base_fx(xp[a], P.inv_dx, base, fx[a]);
+        bspline(fx[a], w[a]);
+        l[a] = min(max(base - b[a], 0), kBlock - 1);
+      }
+      float nv[3] = {0.0f, 0.0f, 0.0f}, nC[3][3] = {{0.0f}};
+      const bool fresh = pi[(int64_t)L::ORIG * cap + p] >= n_final;  // mpm:396-399
+      if (!fresh) {  // G2P, mpm:386-394 (tensor-product form of k_g2p)
+        float pe[3][3] = {{0.0f}};
+#pragma unroll
+        for (int oz = 0; oz < (D == 3 ? 3 : 1); oz++) {
+          float b0[3] = {0, 0, 0}, bx[3] = {0, 0, 0}, by[3] = {0, 0, 0};
+#pragma unroll
+          for (int oy = 0; oy < 3; oy++) {
+            float a0[3] = {0, 0, 0}, ax[3] = {0, 0, 0};
+#pragma unroll
+            for (int ox = 0; ox < 3; ox++) {
+              const int node = D == 3 ? (l[0] + ox) + kTile * ((l[1] + oy) + kTile * (l[2] + oz))
+                                      : (l[0] + ox) + kTile * (l[1] + oy);
+              const float4 g4 = tin[node];
+              const float gg[3] = {g4.x, g4.y, g4.z};
+#pragma unroll
+              for (int cc = 0; cc < D; cc++) {
+                a0[cc] += w[0][ox] * gg[cc];
+                if (ox) ax[cc] += ((float)ox * w[0][ox]) * gg[cc];
+              }
+            }
+#pragma unroll
+            for (int cc = 0; cc < D; cc++) {
+              b0[cc] += w[1][oy] * a0[cc];
+              bx[cc] += w[1][oy] * ax[cc];
+              if (oy) by[cc] += ((float)oy * w[1][oy]) * a0[cc];
+            }
+          }
+          const float wz = D == 3 ? w[2][oz] : 1.0f;
+#pragma unroll
+          for (int cc = 0; cc < D; cc++) {
+            nv[cc] += wz * b0[cc];
+            pe[cc][0] += wz * bx[cc];
+            pe[cc][1] += wz * by[cc];
+            if (D == 3 && oz) pe[cc][2] += ((float)oz * wz) * b0[cc];
+          }
+        }
+#pragma unroll
+        for (int cc = 0; cc < D; cc++)
+#pragma unroll
+          for (int e = 0; e < D; e++) nC[cc][e] = P.c4 * (pe[cc][e] - nv[cc] * fx[e]);
+      } else {
+#pragma unroll
+        for (int a = 0; a < D; a++) nv[a] = vp[a];
+      }
+      float xn[3] = {xp[0], xp[1], xp[2]};
+      if (mat != kStationary) {  // mpm:401-403
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+          vp[a] = nv[a];
+          xn[a] = xp[a] + P.dt * nv[a];
+          pf[(int64_t)(L::V + a) * cap + p] = nv[a];
+          pf[(int64_t)(L::X + a) * cap + p] = xn[a];
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < D; a++) vmax = fmaxf(vmax, fabsf(vp[a]));
+      // new base relative to the output tile origin (one node below the block)
+      float fn[3] = {0, 0, 0};
+      int rel[3] = {1, 1, 1};
+      bool fits = true;
+#pragma unroll
+      for (int a = 0; a < D; a++) {
+        int base;
+        base_fx(xn[a], P.inv_dx, base, fn[a]);
+        rel[a] = base - (b[a] - 1);
+        fits &= (rel[a] >= 0) & (rel[a] <= E8 - 3);
+      }
+      if (!fits) {
+        atomicOr(err, 8);  // moved more than one cell in a substep
+      } else {
+        inner = true;
+#pragma unroll
+        for (int a = 0; a < D; a++) inner &= (rel[a] >= 1) & (rel[a] <= kBlock);
+        if (inner)
+          cell = D == 3 ? (rel[0] - 1) + 4 * ((rel[1] - 1) + 4 * (rel[2] - 1))
+                        : (rel[0] - 1) + 4 * (rel[1] - 1);
+        else
+          shell = D == 3 ? rel[0] + E8 * (rel[1] + E8 * rel[2]) : rel[0] + E8 * rel[1];
+      }
+      // constitutive update with the G2P's C (mpm:409-473)
+      float F[3][3] = {{0}}, stress[3][3] = {{0}};
+#pragma unroll
+      for (int a = 0; a < D; a++)
+#pragma unroll
+        for (int e = 0; e < D; e++) F[a][e] = pf[(int64_t)(L::F + a * D + e) * cap + p];
+      float Jp = pf[(int64_t)L::JP * cap + p];
+      constitutive<D, true>(mat, P, P.mu_0, P.lambda_0, F, nC, Jp, stress);
+#pragma unroll
+      for (int a = 0; a < D; a++)
+#pragma unroll
+        for (int e = 0; e < D; e++) pf[(int64_t)(L::F + a * D + e) * cap + p] = F[a][e];
+      pf[(int64_t)L::JP * cap + p] = Jp;
+      const float mass = P.p_mass;  // mpm:474-483 (no water density here)
+      float Ad[3][3] = {{0.0f}}, q[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int a = 0; a < D; a++) {
+#pragma unroll
+        for (int e = 0; e < D; e++) Ad[a][e] = (P.stress_scale * stress[a][e] + mass * nC[a][e]) * P.dx;
+        float s = mass * vp[a];
+#pragma unroll
+        for (int e = 0; e < D; e++) s -= Ad[a][e] * fn[e];
+        q[a] = s;
+      }
+      if (!fits) {  // contributes nothing; the caller is told
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+          q[a] = 0.0f;
+#pragma unroll
+          for (int e = 0; e < 3; e++) Ad[a][e] = 0.0f;
+        }
+      }
+      if (D == 3) {
+        stage[t][0] = make_float4(Ad[0][0], Ad[0][1], Ad[0][2], Ad[1][0]);
+        stage[t][1] = make_float4(Ad[1][1], Ad[1][2], Ad[2][0], Ad[2][1]);
+        stage[t][2] = make_float4(Ad[2][2], q[0], q[1], q[2]);
+        stage[t][3] = make_float4(fn[0], fn[1], fn[2], fits ? mass : 0.0f);
+      } else {
+        stage[t][0] = make_float4(Ad[0][0], Ad[0][1], Ad[1][0], Ad[1][1]);
+        stage[t][1] = make_float4(q[0], q[1], fn[0], fn[1]);
+        stage[t][2] = make_float4(fits ? mass : 0.0f, 0.0f, 0.0f, 0.0f);
+      }
+      // block of the new position for the next substep's binning
+      int dd[3] = {0, 0, 0}, qb[3] = {0, 0, 0};
+      bool near = true;
+#pragma unroll
+      for (int a = 0; a < D; a++) {
+        qb[a] = ((b[a] - 1 + rel[a]) >> 2) - P.blo[a];
+        dd[a] = qb[a] - lbl[a];
+        near &= (dd[a] >= -1) & (dd[a] <= 1) & (qb[a] >= 0) & (qb[a] < P.bnl[a] - 1);
+      }
+      if (fits && near) {
+        dir = (dd[2] + 1) * 9 + (dd[1] + 1) * 3 + (dd[0] + 1);
+        qb[D - 1] += P.zoff;
+        key_new = encode_local<D>(qb, P);
+      } else if (fits) {
+        atomicOr(err, 1);  // left the window
+      }
+    }
+    __syncthreads();
+
+    // ---- phase 2: stable counting sort of the inner particles by cell ----
+    const bool in_cell = live && inner;
+    unsigned long long peers = __ballot(in_cell);
+#pragma unroll
+    for (int bit = 0; bit < (D == 3 ? 6 : 4); bit++) {
+      const bool on = (cell >> bit) & 1;
+      const unsigned long long bal = __ballot(on);
+      peers &= on ? bal : ~bal;
+    }
+    const int rank = (int)__popcll(peers & lt);
+    if (in_cell && rank == 0) wcnt[wave][cell] = (int)__popcll(peers);
+    __syncthreads();
+    if (threadIdx.x < 64) {
+      int run = 0;
+      if (lane < CELLS) {
+#pragma unroll
+        for (int w = 0; w < WAVES; w++) {
+          const int cnt = wcnt[w][lane];
+          wcnt[w][lane] = run;
+          run += cnt;
+        }
+      }
+      int inc = run;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(inc, off);
+        if (lane >= off) inc += y;
+      }
+      if (lane < CELLS) {
+        ccount[lane] = run;
+        cstart[lane] = inc - run;
+      }
+    }
+    __syncthreads();
+    if (in_cell) order[cstart[cell] + wcnt[wave][cell] + rank] = (unsigned char)t;
+    __syncthreads();
+
+    // ---- phase 3: cell-major scatter of the inner particles ----
+    {
+      const int my_cell = lane % CELLS;
+      const int sub = wave * GROUPS + lane / CELLS;
+      const int lc[3] = {(my_cell & 3) + 1, ((my_cell >> 2) & 3) + 1, (my_cell >> 4) + 1};
+      const int cbeg = cstart[my_cell], ccnt = ccount[my_cell];
+      float4* tp = ptile[sub];
+      for (int k0 = sub; k0 < ccnt; k0 += SUBS * kBatch) {
+        float Ad[kBatch][3][3], q[kBatch][3], mass[kBatch], w[kBatch][3][3];
+#pragma unroll
+        for (int bi = 0; bi < kBatch; bi++) {
+          const int k = k0 + bi * SUBS;
+          float4 s0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), s1 = s0, s2 = s0, s3 = s0;
+          if (k < ccnt) {
+            const int pt = order[cbeg + k];
+            s0 = stage[pt][0];
+            s1 = stage[pt][1];
+            s2 = stage[pt][2];
+            s3 = stage[pt][3];
+          }
+          float f3[3];
+          if (D == 3) {
+            Ad[bi][0][0] = s0.x; Ad[bi][0][1] = s0.y; Ad[bi][0][2] = s0.z;
+            Ad[bi][1][0] = s0.w; Ad[bi][1][1] = s1.x; Ad[bi][1][2] = s1.y;
+            Ad[bi][2][0] = s1.z; Ad[bi][2][1] = s1.w; Ad[bi][2][2] = s2.x;
+            q[bi][0] = s2.y; q[bi][1] = s2.z; q[bi][2] = s2.w;
+            f3[0] = s3.x; f3[1] = s3.y; f3[2] = s3.z;
+            mass[bi] = s3.w;
+          } else {
+            Ad[bi][0][0] = s0.x; Ad[bi][0][1] = s0.y; Ad[bi][1][0] = s0.z; Ad[bi][1][1] = s0.w;
+            Ad[bi][0][2] = Ad[bi][1][2] = Ad[bi][2][0] = Ad[bi][2][1] = Ad[bi][2][2] = 0.0f;
+            q[bi][0] = s1.x; q[bi][1] = s1.y; q[bi][2] = 0.0f;
+            f3[0] = s1.z; f3[1] = s1.w; f3[2] = 0.0f;
+            mass[bi] = s2.x;
+          }
+#pragma unroll
+          for (int a = 0; a < 3; a++) bspline(f3[a], w[bi][a]);
+          if (k >= ccnt) {
+#pragma unroll
+            for (int a = 0; a < 3; a++) w[bi][a][0] = w[bi][a][1] = w[bi][a][2] = 0.0f;
+          }
+        }
+#pragma unroll
+        for (int oz = 0; oz < (D == 3 ? 3 : 1); oz++)
+#pragma unroll
+          for (int oy = 0; oy < 3; oy++)
+#pragma unroll
+            for (int ox = 0; ox < 3; ox++) {
+              float4 add = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+              for (int bi = 0; bi < kBatch; bi++) {
+                float weight = w[bi][0][ox] * w[bi][1][oy];
+                if (D == 3) weight *= w[bi][2][oz];
+                const float fo[3] = {(float)ox, (float)oy, (float)oz};
+                float m3[3];
+#pragma unroll
+                for (int a = 0; a < 3; a++)
+                  m3[a] = q[bi][a] + fo[0] * Ad[bi][a][0] + fo[1] * Ad[bi][a][1] +
+                          fo[2] * Ad[bi][a][2];
+                add.x += weight * m3[0];
+                add.y += weight * m3[1];
+                if (D == 3) add.z += weight * m3[2];
+                add.w += weight * mass[bi];
+              }
+              const int node = D == 3 ? (lc[0] + ox) + E8 * ((lc[1] + oy) + E8 * (lc[2] + oz))
+                                      : (lc[0] + ox) + E8 * (lc[1] + oy);
+              float4 acc = tp[node];
+              acc.x += add.x;
+              acc.y += add.y;
+              acc.z += add.z;
+              acc.w += add.w;
+              tp[node] = acc;
+              __asm__ volatile("" ::: "memory");  // keep the LDS RMWs in program order
+            }
+      }
+    }
+    __syncthreads();
+
+    // ---- phase 3b: particles that crossed into a neighbour's cells ----
+    if (live && shell >= 0) {
+      float Ad[3][3], q[3], mass, w[3][3], f3[3];
+      const float4 s0 = stage[t][0], s1 = stage[t][1], s2 = stage[t][2], s3 = stage[t][3];
+      if (D == 3) {
+        Ad[0][0] = s0.x; Ad[0][1] = s0.y; Ad[0][2] = s0.z;
+        Ad[1][0] = s0.w; Ad[1][1] = s1.x; Ad[1][2] = s1.y;
+        Ad[2][0] = s1.z; Ad[2][1] = s1.w; Ad[2][2] = s2.x;
+        q[0] = s2.y; q[1] = s2.z; q[2] = s2.w;
+        f3[0] = s3.x; f3[1] = s3.y; f3[2] = s3.z;
+        mass = s3.w;
+      } else {
+        Ad[0][0] = s0.x; Ad[0][1] = s0.y; Ad[1][0] = s0.z; Ad[1][1] = s0.w;
+        Ad[0][2] = Ad[1][2] = Ad[2][0] = Ad[2][1] = Ad[2][2] = 0.0f;
+        q[0] = s1.x; q[1] = s1.y; q[2] = 0.0f;
+        f3[0] = s1.z; f3[1] = s1.w; f3[2] = 0.0f;
+        mass = s2.x;
+      }
+#pragma unroll
+      for (int a = 0; a < 3; a++) bspline(f3[a], w[a]);
+      float4* tp = ptile[0];
+#pragma unroll
+      for (int oz = 0; oz < (D == 3 ? 3 : 1); oz++)
+#pragma unroll
+        for (int oy = 0; oy < 3; oy++)
+#pragma unroll
+          for (int ox = 0; ox < 3; ox++) {
+            float weight = w[0][ox] * w[1][oy];
+            if (D == 3) weight *= w[2][oz];
+            const float fo[3] = {(float)ox, (float)oy, (float)oz};
+            float m3[3];
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+              m3[a] = q[a] + fo[0] * Ad[a][0] + fo[1] * Ad[a][1] + fo[2] * Ad[a][2];
+            const int node = shell + (D == 3 ? ox + E8 * (oy + E8 * oz) : ox + E8 * oy);
+            atomicAdd(&tp[node].x, weight * m3[0]);
+            atomicAdd(&tp[node].y, weight * m3[1]);
+            if (D == 3) atomicAdd(&tp[node].z, weight * m3[2]);
+            atomicAdd(&tp[node].w, weight * mass);
+          }
+    }
+
+    // ---- next substep's binning hints (as k_g2p) ----
+    unsigned long long dpeers = __ballot(live);
+#pragma unroll
+    for (int bit = 0; bit < 5; bit++) {
+      const bool on = (dir >> bit) & 1;
+      const unsigned long long bal = __ballot(on);
+      dpeers &= on ? bal : ~bal;
+    }
+    const int drank = (int)__popcll(dpeers & lt);
+    if (live && drank == 0) dwcnt[wave][dir] = (int)__popcll(dpeers);
+    __syncthreads();
+    if (threadIdx.x < 27) {
+      const int d = threadIdx.x;
+      int run = 0;
+#pragma unroll
+      for (int w = 0; w < WAVES; w++) {
+        const int cnt = dwcnt[w][d];
+        dwcnt[w][d] = run;
+        run += cnt;
+      }
+      dcount[(int64_t)c * 27 + d] = run;
+      if (run > 0) {
+        int q[3] = {lb[0] + d % 3 - 1, lb[1] + (d / 3) % 3 - 1, lb[2] + d / 9 - 1};
+        atomicAdd(&count_next[encode_local<D>(q, P)], run);
+      }
+    }
+
+    // ---- phase 4: sum the private tiles, write the node tile ----
+    float4* out = tiles + (int64_t)c * TN8;
+    for (int i = threadIdx.x; i < TN8; i += kThreads) {
+      float4 acc = ptile[0][i];
+#pragma unroll
+      for (int sdx = 1; sdx < SUBS; sdx++) {
+        const float4 v = ptile[sdx][i];
+        acc.x += v.x;
+        acc.y += v.y;
+        acc.z += v.z;
+        acc.w += v.w;
+      }
+      out[i] = acc;
+    }
+    __syncthreads();
+    if (live) {
+      code[ck.y + t] = dir | ((dwcnt[wave][dir] + drank) << 8);
+      nkey[ck.y + t] = key_new;
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = vmax;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float m = red[0];
+#pragma unroll
+    for (int w = 1; w < WAVES; w++) m = fmaxf(m, red[w]);
+    maxv[blockIdx.x] = __float_as_uint(m);
+  }
+}
+
+// Grid operator of the use_g2p2g mode: the node sums of the (4+4)^d tiles of
+// the chunks of blocks b + {-1, 0, 1}^d, then normalisation and gravity
+// (mpm:591-593), the velocity clamp (mpm:596-598) and the post-processes.
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_grid_op_g2p2g(
+    const Params* __restrict__ scenes, const int* __restrict__ slab,
+    const int* __restrict__ active, const int* __restrict__ totals,
+    const int* __restrict__ pchunk_begin, const int* __restrict__ pchunk_cnt,
+    const float4* __restrict__ tiles, float4* __restrict__ grid) {
+  constexpr int NPB = Geo<D>::kNodesPerBlock;
+  constexpr int E8 = G2P2GGeo<D>::kEdge, TN8 = G2P2GGeo<D>::kNodes;
+  const int64_t total = (int64_t)totals[kTotActive] * NPB;
+  for (int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * kThreads) {
+    const int blk = (int)(t / NPB), node = (int)(t % NPB);
+    const int bid = active[blk];
+    int lb[3];
+    decode_local<D>(bid, scenes[0], lb);
+    const Params& P = scenes[scene_of<D>(lb, slab)];
+    if (P.skip) continue;
+    const int l[3] = {node & 3, (node >> 2) & 3, node >> 4};
+    float m = 0.0f, p[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int dz = (D == 3 ? -1 : 0); dz <= (D == 3 ? 1 : 0); dz++)
+#pragma unroll
+      for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+        for (int dx = -1; dx <= 1; dx++) {
+          // tile of block q = b - d starts one node below q: this node sits
+          // at tile coordinate l + 1 + 4 d
+          const int d[3] = {dx, dy, dz};
+          int q[3] = {lb[0], lb[1], lb[2]}, tl[3] = {0, 0, 0};
+          bool ok = true;
+#pragma unroll
+          for (int a = 0; a < D; a++) {
+            tl[a] = l[a] + 1 + kBlock * d[a];
+            q[a] = lb[a] - d[a];
+            ok &= (tl[a] >= 0) & (tl[a] < E8) & (q[a] >= 0) & (q[a] < scenes[0].bn[a]);
+          }
+          if (!ok) continue;
+          const int nb = encode_local<D>(q, P);
+          const int c0 = pchunk_begin[nb], cn = pchunk_cnt[nb];
+          const int tn = D == 3 ? tl[0] + E8 * (tl[1] + E8 * tl[2]) : tl[0] + E8 * tl[1];
+          for (int k = c0; k < c0 + cn; k++) {
+            const float4 tv = tiles[(int64_t)k * TN8 + tn];
+            p[0] += tv.x;
+            p[1] += tv.y;
+            if (D == 3) p[2] += tv.z;
+            m += tv.w;
+          }
+        }
+    if (m > 0.0f) {  // mpm_solver.py:591-593
+      const float inv = 1.0f / m;
+#pragma unroll
+      for (int a = 0; a < D; a++) p[a] = inv * p[a];
+#pragma unroll
+      for (int a = 0; a < D; a++) p[a] += P.dt * P.gravity[a];
+    }
+    if (P.v_allowed > 0.0f) {  // mpm_solver.py:596-598
+#pragma unroll
+      for (int a = 0; a < D; a++) p[a] = fminf(fmaxf(p[a], -P.v_allowed), P.v_allowed);
+    }
+    int I[3], lbl[3];
+    to_local<D>(lb, P, lbl);
+#pragma unroll
+    for (int a = 0; a < D; a++) I[a] = (lbl[a] + P.blo[a]) * kBlock + l[a];
+    apply_post<D>(P, I, p);
+    grid[(int64_t)bid * NPB + node] = make_float4(p[0], p[1], D == 3 ? p[2] : 0.0f, m);
+  }
+}
+
+}  // namespace mpm

@@ -82,6 +82,11 @@ struct SubstepBuffers {
   int* dcount;      // [chunk_cap][27]
   int* doff;        // [chunk_cap][27]
   int4* tile_prefix;  // two-level block scan
+  // use_g2p2g mode: the fused transfer reads grid_in and writes grid;
+  // particles with an insertion index >= n_final skip the G2P
+  bool g2p2g;
+  float4* grid_in;
+  int n_final;
   bool full_sort;     // radix-sort the slots (first substep / positions set by the host)
   bool perm_identity; // the slots were just reordered into the previous order
   int* keys_sorted;   // output: block keys in sorted order

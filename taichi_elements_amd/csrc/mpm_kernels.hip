@@ -14,6 +14,7 @@
 //                     advection, max |v|                                (g2p, mpm:694-735)
 #include "mpm_kernels.h"
 #include "mpm_adjoint.h"
+#include "mpm_g2p2g.h"
 
 namespace mpm {
 
@@ -113,14 +114,20 @@ __device__ __forceinline__ int4 block_values(int bid, const int* __restrict__ co
   int lb[3];
   decode_local<D>(bid, P, lb);
   bool act = false;
+  // p2g reaches blocks b + {0, 1}^d of a particle block b; the fused g2p2g
+  // transfer (new positions up to one cell away) b + {-1, 0, 1}^d
+  const int lo = P.g2p2g ? -1 : 0;
 #pragma unroll
-  for (int dz = 0; dz < (D == 3 ? 2 : 1); dz++)
+  for (int dz = (D == 3 ? -1 : 0); dz < (D == 3 ? 2 : 1); dz++)
 #pragma unroll
-    for (int dy = 0; dy < 2; dy++)
+    for (int dy = -1; dy < 2; dy++)
 #pragma unroll
-      for (int dx = 0; dx < 2; dx++) {
+      for (int dx = -1; dx < 2; dx++) {
+        if (dx < lo || dy < lo || (D == 3 && dz < lo)) continue;
         int q[3] = {lb[0] - dx, lb[1] - dy, lb[2] - dz};
-        if (q[0] < 0 || q[1] < 0 || q[2] < 0) continue;
+        if (q[0] < 0 || q[1] < 0 || q[2] < 0 || q[0] >= P.bn[0] || q[1] >= P.bn[1] ||
+            (D == 3 && q[2] >= P.bn[2]))
+          continue;
         act |= count[encode_local<D>(q, P)] > 0;
       }
   return make_int4(c, (c + chunk - 1) / chunk, act ? 1 : 0, c > 0 ? 1 : 0);
@@ -1285,10 +1292,10 @@ struct TimerScope {
   }
 };
 
-// binning, p2g and the grid operator of one substep
+// binning of one substep: block counts, the dense-window scan (chunks,
+// active blocks) and the sorted order
 template <int D>
-void launch_front(const Params* P, const int* slab, SubstepBuffers& B, int raw, hipStream_t st,
-                  Timer* tm) {
+static void launch_binning(const Params* P, SubstepBuffers& B, hipStream_t st, Timer* tm) {
   const int n = B.n;
   const int G = B.persistent_groups;
   if (n == 0) return;
@@ -1348,7 +1355,40 @@ void launch_front(const Params* P, const int* slab, SubstepBuffers& B, int raw, 
     std::swap(B.perm, B.perm_alt);
     B.keys_sorted = B.keys_alt;
   }
+}
+
+// binning, p2g and the grid operator of one substep
+template <int D>
+void launch_front(const Params* P, const int* slab, SubstepBuffers& B, int raw, hipStream_t st,
+                  Timer* tm) {
+  launch_binning<D>(P, B, st, tm);
   launch_transfer<D>(P, slab, B, raw, st, tm);
+}
+
+// one substep of the use_g2p2g mode: binning, the fused transfer from
+// B.grid_in, and the grid operator into B.grid
+template <int D>
+static void launch_g2p2g(const Params* P, const int* slab, SubstepBuffers& B, int s_index,
+                         hipStream_t st, Timer* tm) {
+  const int n = B.n;
+  const int G = B.persistent_groups;
+  if (n == 0) return;
+  launch_binning<D>(P, B, st, tm);
+  const int chunk_bound = std::min(B.total_blocks, n) + div_up(n, B.chunk);
+  {
+    TimerScope ts(tm, "k_g2p2g", st);
+    k_g2p2g<D><<<clamp_groups(chunk_bound, G), kThreads, 0, st>>>(
+        P, slab, B.pf, B.pi, B.cap, B.perm, B.chunks, B.totals, B.grid_in, B.tiles, B.n_final,
+        B.maxv + (int64_t)s_index * B.persistent_groups, B.code, B.nkey, B.dcount, B.count_next,
+        B.err);
+  }
+  {
+    TimerScope ts(tm, "k_grid_op", st);
+    const int64_t node_bound = (int64_t)B.total_blocks * Geo<D>::kNodesPerBlock;
+    k_grid_op_g2p2g<D><<<clamp_groups(div_up(node_bound, kThreads), G / 2), kThreads, 0, st>>>(
+        P, slab, B.active, B.totals, B.pchunk_begin, B.pchunk_cnt, B.tiles, B.grid);
+  }
+  std::swap(B.count, B.count_next);
 }
 
 // p2g and the grid operator on an existing binning
@@ -1393,6 +1433,10 @@ void launch_g2p(const Params* P, const int* slab, SubstepBuffers& B, int s_index
 template <int D>
 void launch_substep(const Params* P, const int* slab, SubstepBuffers& B, int s_index, int raw,
                     hipStream_t st, Timer* tm) {
+  if (B.g2p2g) {
+    launch_g2p2g<D>(P, slab, B, s_index, st, tm);
+    return;
+  }
   launch_front<D>(P, slab, B, raw, st, tm);
   launch_g2p<D>(P, slab, B, s_index, st, tm);
 }

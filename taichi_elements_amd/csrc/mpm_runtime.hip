@@ -166,6 +166,15 @@ struct mpm_solver {
   bool timing = false;
   EventTimer timer;
 
+  // use_g2p2g mode: the fused transfer writes the next grid while reading
+  // this one (grid_alt), particles with an insertion index >= n_final are
+  // new (no G2P on their first substep)
+  int g2p2g = 0;
+  int g2p2g_clamp = 1;
+  double g2p2g_cfl = 0.9;
+  float4* grid_alt = nullptr;
+  int64_t n_final = 0;
+
   // differentiable mode (mpm_diff_*): the particle state lives in caller
   // arrays; the handle supplies the block window, workspace and grid
   int64_t diff_n = 0;
@@ -256,6 +265,10 @@ void fill_params(Params& P, const mpm_solver* s, const SceneHost& sc, float dt, 
   P.skip = skip;
   P.youngs = nullptr;
   P.kmu = P.kla = P.dkmu = P.dkla = 0.0f;
+  P.g2p2g = s->g2p2g;
+  P.v_allowed = s->g2p2g && s->g2p2g_clamp && s->g2p2g_cfl > 0
+                    ? (float)(c.dx * s->g2p2g_cfl) / dt
+                    : 0.0f;
   P.n_post = (int)sc.post.size();
   for (size_t k = 0; k < sc.post.size(); k++) P.post[k] = sc.post[k];
 }
@@ -278,6 +291,8 @@ int free_particle_buffers(mpm_solver* s) {
   dfree(s->doff);
   return MPM_OK;
 }
+
+int alloc_chunk_buffers(mpm_solver* s);
 
 int grow(mpm_solver* s, int64_t need) {
   if (need <= s->cap) return MPM_OK;
@@ -312,16 +327,28 @@ int grow(mpm_solver* s, int64_t need) {
   TRY(dalloc(&s->lb_status_radix, kRadixBins * ntiles));
   HIP_TRY(hipMemsetAsync(s->lb_status_radix, 0, sizeof(unsigned long long) * kRadixBins * ntiles,
                          s->stream));
-  s->chunk_cap = std::min<int64_t>(s->total_blocks, ncap) + (ncap + s->chunk - 1) / s->chunk;
-  const int TN = s->D == 3 ? Geo<3>::kTileNodes : Geo<2>::kTileNodes;
-  TRY(dalloc(&s->tiles, s->chunk_cap * TN));
-  TRY(dalloc(&s->chunks, 2 * s->chunk_cap));
   TRY(dalloc(&s->code, ncap));
   TRY(dalloc(&s->nkey, ncap));
-  TRY(dalloc(&s->dcount, 27 * s->chunk_cap));
-  TRY(dalloc(&s->doff, 27 * s->chunk_cap));
+  TRY(alloc_chunk_buffers(s));
   HIP_TRY(hipStreamSynchronize(s->stream));
   s->have_perm = false;
+  return MPM_OK;
+}
+
+// Buffers sized by the number of transfer work items (chunks), which
+// depends on the particle capacity and the chunk size.
+int alloc_chunk_buffers(mpm_solver* s) {
+  dfree(s->tiles);
+  dfree(s->chunks);
+  dfree(s->dcount);
+  dfree(s->doff);
+  s->chunk_cap = std::min<int64_t>(s->total_blocks, s->cap) + (s->cap + s->chunk - 1) / s->chunk;
+  int TN = s->D == 3 ? Geo<3>::kTileNodes : Geo<2>::kTileNodes;
+  if (s->g2p2g) TN = s->D == 3 ? (kBlock + 4) * (kBlock + 4) * (kBlock + 4) : (kBlock + 4) * (kBlock + 4);
+  TRY(dalloc(&s->tiles, s->chunk_cap * TN));
+  TRY(dalloc(&s->chunks, 2 * s->chunk_cap));
+  TRY(dalloc(&s->dcount, 27 * s->chunk_cap));
+  TRY(dalloc(&s->doff, 27 * s->chunk_cap));
   return MPM_OK;
 }
 
@@ -354,6 +381,9 @@ void fill_buffers(mpm_solver* s, SubstepBuffers& B) {
   B.tile_prefix = s->tile_prefix;
   B.full_sort = !s->have_perm;
   B.perm_identity = s->reordered;
+  B.g2p2g = s->g2p2g != 0;
+  B.grid_in = s->g2p2g ? s->grid : nullptr;
+  B.n_final = (int)s->n_final;
   B.keys_sorted = nullptr;
   B.active = s->active;
   B.lb_status_scan = s->lb_status_scan;
@@ -362,7 +392,7 @@ void fill_buffers(mpm_solver* s, SubstepBuffers& B) {
   B.ghist = s->ghist;
   B.totals = s->totals + cur * 16;
   B.tiles = s->tiles;
-  B.grid = s->grid;
+  B.grid = s->g2p2g ? s->grid_alt : s->grid;
   B.err = s->err;
   B.maxv = s->maxv;
   B.total_blocks = s->total_blocks;
@@ -373,6 +403,10 @@ void fill_buffers(mpm_solver* s, SubstepBuffers& B) {
 }
 
 void take_buffers(mpm_solver* s, const SubstepBuffers& B) {
+  if (B.g2p2g && B.n > 0) {  // the output grid becomes the next input
+    std::swap(s->grid, s->grid_alt);
+    s->n_final = s->n;
+  }
   s->perm = B.perm;
   s->perm_alt = B.perm_alt;
   s->keys_sorted = B.keys_sorted;
@@ -518,6 +552,7 @@ int mpm_destroy(mpm_solver* s) {
   dfree(s->lbs);
   dfree(s->ghist);
   dfree(s->grid);
+  dfree(s->grid_alt);
   dfree(s->totals);
   dfree(s->err);
   dfree(s->maxv);
@@ -959,6 +994,7 @@ int mpm_diff_setup(mpm_solver* s, int64_t n, const int32_t* material) {
   for (const PostOp& op : s->scenes[0].post)
     if (op.kind != kPostBBox)
       return fail(MPM_EINVAL, "differentiable mode supports the bounding box only");
+  if (s->g2p2g) return fail(MPM_EINVAL, "differentiable mode uses the p2g / g2p transfers");
   HIP_TRY(hipSetDevice(s->device));
   HIP_TRY(hipStreamSynchronize(s->stream));
   TRY(grow(s, n));
@@ -1044,6 +1080,66 @@ void diff_buffers(mpm_solver* s, SubstepBuffers& B, float* pf) {
 }
 
 }  // namespace
+
+int mpm_set_g2p2g(mpm_solver* s, int32_t enabled, double allowed_cfl, int32_t clamp) {
+  if (!s) return fail(MPM_EINVAL, "null argument");
+  if (s->substep_counter != 0 || s->diff_n != 0)
+    return fail(MPM_EINVAL, "choose the transfer mode before the first substep");
+  if (!(allowed_cfl >= 0.0)) return fail(MPM_EINVAL, "g2p2g_allowed_cfl must be >= 0");
+  HIP_TRY(hipSetDevice(s->device));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  const int want = enabled ? 1 : 0;
+  s->g2p2g_cfl = allowed_cfl;
+  s->g2p2g_clamp = clamp ? 1 : 0;
+  if (want != s->g2p2g) {
+    s->g2p2g = want;
+    dfree(s->grid_alt);
+    if (want) {
+      const int NPB = s->D == 3 ? Geo<3>::kNodesPerBlock : Geo<2>::kNodesPerBlock;
+      const int64_t nodes = (int64_t)s->total_blocks * NPB;
+      TRY(dalloc(&s->grid_alt, nodes));
+      HIP_TRY(hipMemset(s->grid_alt, 0, sizeof(float4) * nodes));
+      HIP_TRY(hipMemset(s->grid, 0, sizeof(float4) * nodes));
+    }
+    if (s->cap > 0) TRY(alloc_chunk_buffers(s));  // tile size depends on the mode
+    s->n_final = 0;
+  }
+  return MPM_OK;
+}
+
+int mpm_set_tuning(mpm_solver* s, int32_t chunk, int32_t groups) {
+  if (!s) return fail(MPM_EINVAL, "null argument");
+  if (chunk != 0 && (chunk < 16 || chunk > kChunkMax))
+    return fail(MPM_EINVAL, "chunk must lie in [16, " + std::to_string(kChunkMax) + "]");
+  if (groups != 0 && (groups < 64 || groups > 16384))
+    return fail(MPM_EINVAL, "groups must lie in [64, 16384]");
+  if (s->diff_n != 0) return fail(MPM_EINVAL, "tune a differentiable handle before mpm_diff_setup");
+  HIP_TRY(hipSetDevice(s->device));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (groups != 0 && groups != s->persistent_groups) {
+    dfree(s->maxv);
+    s->persistent_groups = groups;
+    TRY(dalloc(&s->maxv, 16 * (int64_t)groups));
+    s->maxv_cap = 16 * groups;
+    launch_fill_u32(s->maxv, 0u, s->maxv_cap, s->stream);
+    s->last_batch = 0;
+  }
+  if (chunk != 0 && chunk != s->chunk) {
+    s->chunk = chunk;
+    if (s->cap > 0) TRY(alloc_chunk_buffers(s));
+  }
+  // the next substep bins from scratch (chunk boundaries changed)
+  s->have_perm = false;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return MPM_OK;
+}
+
+int mpm_get_tuning(mpm_solver* s, int32_t* chunk, int32_t* groups) {
+  if (!s || !chunk || !groups) return fail(MPM_EINVAL, "null argument");
+  *chunk = s->chunk;
+  *groups = s->persistent_groups;
+  return MPM_OK;
+}
 
 int mpm_diff_binning_bytes(mpm_solver* s, int64_t* bytes) {
   if (!s || !bytes) return fail(MPM_EINVAL, "null argument");

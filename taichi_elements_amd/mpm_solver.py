@@ -14,7 +14,9 @@ Differences from the reference, all explicit:
   * the grid is a dense window of 4^d blocks around the ``res`` domain (two
     blocks of margin below, >= 4 cells above the walls) instead of Taichi's
     4096^d sparse grid; particles that leave it raise ``DomainError``;
-  * ``unbounded``, ``quant``, ``use_g2p2g``, ``use_adaptive_dt`` and
+  * ``use_g2p2g=True`` runs the fused g2p2g transfer of the reference
+    (mpm_solver.py:363-485) with its grid velocity clamp;
+  * ``unbounded``, ``quant``, ``use_adaptive_dt`` and
     ``support_plasticity=False`` raise ``NotImplementedError``, as do the
     mesh/texture/polygon seeders;
   * ``grid_postprocess`` lists descriptions of the built-in post-processes
@@ -51,6 +53,40 @@ class _Field0D:
         if key is not None:
             raise IndexError("0-D field: index with None")
         self._set(value)
+
+
+def _autotune(obj, handle, dt, get_state, set_state, substeps, chunks, groups, verbose, dts):
+    """Shared by MPMSolver and MPMSolverBatch: time each launch shape on
+    the live particles (two untimed substeps re-bin after each change)."""
+    L = _lib.lib()
+    if obj.n_particles_total() == 0:
+        return {}
+    snap = get_state()
+    steps = np.ascontiguousarray(np.asarray(dts(), np.float32))
+    batched = len(steps) > 1 or hasattr(obj, "scenes")
+
+    def run(k):
+        if batched:
+            check(L.mpm_substeps_batch(handle, k, _lib.fptr(steps)))
+        else:
+            check(L.mpm_substeps(handle, k, float(dt), 0.0))
+        check(L.mpm_sync(handle))
+
+    results = {}
+    for ch in chunks:
+        for gr in groups:
+            check(L.mpm_set_tuning(handle, int(ch), int(gr)))
+            set_state(snap)
+            run(2)
+            t0 = time.perf_counter()
+            run(substeps)
+            results[(ch, gr)] = 1000 * (time.perf_counter() - t0) / substeps
+            if verbose:
+                print(f"chunk {ch:4d} groups {gr:5d}: {results[(ch, gr)]:.4f} ms/substep")
+    best = min(results, key=results.get)
+    check(L.mpm_set_tuning(handle, int(best[0]), int(best[1])))
+    set_state(snap)
+    return results
 
 
 class MPMSolver:
@@ -104,7 +140,7 @@ class MPMSolver:
                  kernel_profiler=False):
         self.dim = len(res)
         assert self.dim in (2, 3), "MPM solver supports only 2D and 3D simulations."
-        for flag, name in ((quant, "quant"), (use_g2p2g, "use_g2p2g"), (unbounded, "unbounded"),
+        for flag, name in ((quant, "quant"), (unbounded, "unbounded"),
                            (use_adaptive_dt, "use_adaptive_dt")):
             if flag:
                 raise NotImplementedError(f"{name}=True is not supported by this engine")
@@ -204,6 +240,8 @@ class MPMSolver:
         h = _lib._P()
         check(L.mpm_create(cfg, h))
         self._handle = h
+        if self.use_g2p2g:
+            check(L.mpm_set_g2p2g(h, 1, float(self.g2p2g_allowed_cfl), int(bool(self.v_clamp_g2p2g))))
         if self._kernel_profiler:
             check(L.mpm_set_timing(self._h(), 1))
 
@@ -212,6 +250,9 @@ class MPMSolver:
         if h is not None and self._owns_handle and _lib._lib is not None:
             _lib._lib.mpm_destroy(h)
             self._handle = None
+
+    def n_particles_total(self):
+        return self._num_particles()
 
     def _num_particles(self):
         n = _lib.ctypes.c_int64()
@@ -548,6 +589,29 @@ class MPMSolver:
 
     def set_debug(self, flags):
         check(_lib.lib().mpm_set_debug(self._h(), int(flags)))
+
+    # ------------------------------------------------------------------ #
+    # launch tuning
+    # ------------------------------------------------------------------ #
+    def tuning(self):
+        """(chunk, groups): particles per transfer work item and workgroups
+        of the persistent transfer kernels."""
+        ch, gr = _lib.ctypes.c_int32(), _lib.ctypes.c_int32()
+        check(_lib.lib().mpm_get_tuning(self._h(), ch, gr))
+        return int(ch.value), int(gr.value)
+
+    def set_tuning(self, chunk=0, groups=0):
+        check(_lib.lib().mpm_set_tuning(self._h(), int(chunk), int(groups)))
+
+    def autotune(self, dt=None, substeps=4, chunks=(64, 128, 256), groups=(1024, 2048, 4096),
+                 verbose=False):
+        """Times every (chunk, groups) candidate on the current particles
+        and keeps the fastest.  The particle state, t and the substep count
+        are restored afterwards; only float32 summation order depends on the
+        choice.  Returns {(chunk, groups): ms per substep}."""
+        return _autotune(self, self._h(), dt if dt is not None else self.default_dt,
+                         self.get_state, lambda st: self.set_state(**st), substeps, chunks,
+                         groups, verbose, lambda: [dt if dt is not None else self.default_dt])
 
     def last_counts(self):
         out = np.zeros(4, np.int64)

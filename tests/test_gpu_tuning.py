@@ -1,0 +1,77 @@
+"""Launch tuning and the autotuner on the GPU: autotune() leaves the scene
+exactly as it found it, and a tuned solver matches an untuned one up to
+float32 summation order: norm-wise relative 1e-5 after one substep, and the
+parity tests' drift bound 1e-4 after eleven (the scene has random
+per-particle velocities, which amplify rounding differences)."""
+import numpy as np
+import pytest
+
+from tests.helpers import cube_particles, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def _quiet():
+    from taichi_elements_amd import mpm_solver
+    mpm_solver.PRINT_PROGRESS = False
+
+
+def _scene(seed=11):
+    from taichi_elements_amd import MPMSolver
+    rng = np.random.default_rng(seed)
+    s = MPMSolver(res=(48, 48, 48))
+    n = 40000
+    x = cube_particles(rng, n, [0.2] * 3, [0.5] * 3, 3)
+    v = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
+    s.read_restart(n, x, v, rng.integers(0, 4, n).astype(np.int32), np.zeros(n, np.int32))
+    return s
+
+
+def test_autotune_restores_the_scene_and_keeps_results():
+    s = _scene()
+    for _ in range(3):
+        s.substep(s.default_dt)
+    before = s.get_state()
+    t0, k0 = s.t, s.total_substeps
+    res = s.autotune(substeps=2, chunks=(64, 256), groups=(512, 2048))
+    assert len(res) == 4 and all(v > 0 for v in res.values())
+    assert s.tuning() == min(res, key=res.get)
+    after = s.get_state()
+    for k in before:
+        assert np.array_equal(before[k], after[k]), k
+    assert s.t == t0 and s.total_substeps == k0
+    ref = _scene()
+    for _ in range(3):
+        ref.substep(ref.default_dt)
+    # one substep: only the float32 summation order differs
+    s.substep(s.default_dt)
+    ref.substep(ref.default_dt)
+    a, b = s.get_state(), ref.get_state()
+    for k in ("x", "v", "F", "Jp"):
+        e = rel_err(a[k], b[k])
+        print("1 substep", k, f"{e:.2e}", flush=True)
+        assert e <= 1e-5, k
+    # ten more: the drift bound of the parity tests
+    for _ in range(10):
+        s.substep(s.default_dt)
+        ref.substep(ref.default_dt)
+    a, b = s.get_state(), ref.get_state()
+    for k in ("x", "v", "F", "Jp"):
+        e = rel_err(a[k], b[k])
+        print("11 substeps", k, f"{e:.2e}", flush=True)
+        assert e <= 1e-4, k
+
+
+def test_set_tuning_checks_its_arguments():
+    s = _scene()
+    with pytest.raises(ValueError):
+        s.set_tuning(chunk=8)
+    with pytest.raises(ValueError):
+        s.set_tuning(chunk=1024)
+    with pytest.raises(ValueError):
+        s.set_tuning(groups=10)
+    s.set_tuning(chunk=128, groups=1024)
+    assert s.tuning() == (128, 1024)
+    s.substep(s.default_dt)
+    assert s.last_counts()["valid"] == 40000
