@@ -116,12 +116,24 @@ def batched_bench(steps, warmup, autotune=True):
     dts = np.ascontiguousarray(np.asarray(b.default_dts(), np.float32))
     tuning = None
     if autotune:
-        b.autotune(substeps=3)
-        tuning = dict(zip(("chunk", "groups"), b.tuning()))
+        # tune a copy, so that the measured batch and the separate solvers
+        # start from the same particle order and stay comparable bit for bit
+        probe = MPMSolverBatch([dict(res=(sp["res"], ) * 3) for sp in specs])
+        for sc, sp in zip(probe, specs):
+            sc.read_restart(len(sp["x"]), sp["x"], sp["v"], sp["m"],
+                            np.zeros(len(sp["x"]), np.int32))
+        probe.autotune(substeps=3)
+        chunk, groups = probe.tuning()
+        probe.close()
+        del probe
+        b.set_tuning(chunk, groups)
+        tuning = dict(chunk=chunk, groups=groups)
     solos = []
     for sp in specs:
         s = MPMSolver(res=(sp["res"], ) * 3)
         s.read_restart(len(sp["x"]), sp["x"], sp["v"], sp["m"], np.zeros(len(sp["x"]), np.int32))
+        if tuning:
+            s.set_tuning(tuning["chunk"], tuning["groups"])
         solos.append(s)
     print(f"[bench] batched: {len(specs)} scenes, {total} particles", file=sys.stderr, flush=True)
 
@@ -343,11 +355,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--reorder-interval", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
-    ap.add_argument("--config", choices=["dense1m", "sparse16m", "fwdbwd200k"], default="dense1m",
+    ap.add_argument("--config", choices=["dense1m", "sparse16m", "fwdbwd200k", "g2p2g1m",
+                                         "batched64"], default="dense1m",
                     help="dense1m: the 3D four-material 128^3 workload (default, the "
                          "headline line); sparse16m: 512^3 window, 16M particles in 64 "
-                         "ellipsoids with a plane collider; fwdbwd200k: only the "
-                         "differentiable 200k-particle forward + backward measurement")
+                         "ellipsoids with a plane collider; fwdbwd200k / g2p2g1m / "
+                         "batched64: only the differentiable, use_g2p2g or 64-scene batched "
+                         "measurement")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shape instead of autotuning it on the scene")
     ap.add_argument("--no-batched", action="store_true",
@@ -367,6 +381,23 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.config in ("g2p2g1m", "batched64"):
+        from taichi_elements_amd import mpm_solver
+        mpm_solver.PRINT_PROGRESS = False
+        if args.config == "g2p2g1m":
+            x, v, m = make_scene()
+            leg = g2p2g_bench(x, v, m, args.steps, args.warmup, not args.no_autotune)
+            data = "synthetic (seeded numpy: 4 cubes x 250k particles, v ~ U[-1,1]^3)"
+        else:
+            leg = batched_bench(args.steps, args.warmup, not args.no_autotune)
+            data = "synthetic (seeded numpy: 64 scenes of 1-4 random cubes)"
+        print(json.dumps({
+            "metric": "particle-substeps/s (3D 1M, 4-material): plain/batched/fwd+bwd; % HBM peak",
+            "value": leg["value"], "unit": "particle-substeps/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": leg["ms_per_substep"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": data, "config": {"workload": leg["workload"]}, args.config: leg}), flush=True)
+        return
     if args.config == "fwdbwd200k":
         import torch  # noqa: F401  (before the engine library loads)
         fb = fwd_bwd_bench()

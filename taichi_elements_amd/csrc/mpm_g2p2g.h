@@ -45,7 +45,8 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(
   constexpr int kBatch = 2;
   __shared__ float4 tin[TN];
   __shared__ float4 stage[kChunkMax][4];
-  __shared__ float4 ptile[SUBS][TN8];
+  __shared__ float4 ptile[SUBS][TN];  // inner particles: nodes 1..6 of the output tile
+  __shared__ float4 stile[TN8];       // particles that crossed a cell face (LDS atomics)
   __shared__ int wcnt[WAVES][CELLS];
   __shared__ int cstart[CELLS], ccount[CELLS];
   __shared__ unsigned char order[kChunkMax];
@@ -65,8 +66,9 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(
     to_local<D>(lb, P, lbl);
 #pragma unroll
     for (int a = 0; a < D; a++) b[a] = (lbl[a] + P.blo[a]) * kBlock;
-    for (int i = threadIdx.x; i < SUBS * TN8; i += kThreads)
+    for (int i = threadIdx.x; i < SUBS * TN; i += kThreads)
       (&ptile[0][0])[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (int i = threadIdx.x; i < TN8; i += kThreads) stile[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     for (int i = threadIdx.x; i < WAVES * CELLS; i += kThreads) (&wcnt[0][0])[i] = 0;
     for (int i = threadIdx.x; i < WAVES * 32; i += kThreads) (&dwcnt[0][0])[i] = 0;
     for (int i = threadIdx.x; i < TN; i += kThreads) {
@@ -289,7 +291,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(
     {
       const int my_cell = lane % CELLS;
       const int sub = wave * GROUPS + lane / CELLS;
-      const int lc[3] = {(my_cell & 3) + 1, ((my_cell >> 2) & 3) + 1, (my_cell >> 4) + 1};
+      const int lc[3] = {my_cell & 3, (my_cell >> 2) & 3, my_cell >> 4};
       const int cbeg = cstart[my_cell], ccnt = ccount[my_cell];
       float4* tp = ptile[sub];
       for (int k0 = sub; k0 < ccnt; k0 += SUBS * kBatch) {
@@ -349,8 +351,8 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(
                 if (D == 3) add.z += weight * m3[2];
                 add.w += weight * mass[bi];
               }
-              const int node = D == 3 ? (lc[0] + ox) + E8 * ((lc[1] + oy) + E8 * (lc[2] + oz))
-                                      : (lc[0] + ox) + E8 * (lc[1] + oy);
+              const int node = D == 3 ? (lc[0] + ox) + kTile * ((lc[1] + oy) + kTile * (lc[2] + oz))
+                                      : (lc[0] + ox) + kTile * (lc[1] + oy);
               float4 acc = tp[node];
               acc.x += add.x;
               acc.y += add.y;
@@ -383,7 +385,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(
       }
 #pragma unroll
       for (int a = 0; a < 3; a++) bspline(f3[a], w[a]);
-      float4* tp = ptile[0];
+      float4* tp = stile;
 #pragma unroll
       for (int oz = 0; oz < (D == 3 ? 3 : 1); oz++)
 #pragma unroll
@@ -435,15 +437,28 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(
     // ---- phase 4: sum the private tiles, write the node tile ----
     float4* out = tiles + (int64_t)c * TN8;
     for (int i = threadIdx.x; i < TN8; i += kThreads) {
-      float4 acc = ptile[0][i];
+      const int t8[3] = {i % E8, (i / E8) % E8, i / (E8 * E8)};
+      bool in6 = true;
 #pragma unroll
-      for (int sdx = 1; sdx < SUBS; sdx++) {
-        const float4 v = ptile[sdx][i];
-        acc.x += v.x;
-        acc.y += v.y;
-        acc.z += v.z;
-        acc.w += v.w;
+      for (int a = 0; a < D; a++) in6 &= (t8[a] >= 1) & (t8[a] <= kTile);
+      float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (in6) {
+        const int j6 = D == 3 ? (t8[0] - 1) + kTile * ((t8[1] - 1) + kTile * (t8[2] - 1))
+                              : (t8[0] - 1) + kTile * (t8[1] - 1);
+#pragma unroll
+        for (int sdx = 0; sdx < SUBS; sdx++) {
+          const float4 v = ptile[sdx][j6];
+          acc.x += v.x;
+          acc.y += v.y;
+          acc.z += v.z;
+          acc.w += v.w;
+        }
       }
+      const float4 sv = stile[i];
+      acc.x += sv.x;
+      acc.y += sv.y;
+      acc.z += sv.z;
+      acc.w += sv.w;
       out[i] = acc;
     }
     __syncthreads();
