@@ -294,6 +294,38 @@ def g2p2g_bench(x, v, m, steps, warmup, autotune=True):
                 workload="3D four-material 1M particles, 128^3, use_g2p2g=True")
 
 
+def small2d_bench(steps=100, repeats=3):
+    """The reference's own small case: 2D, 128^2, 20,000 water particles in
+    [0.2, 0.5] x [0.4, 0.8] (seed 0), 100 substeps at default_dt -- timed
+    with HIP graph replay of substep pairs and kernel by kernel."""
+    from taichi_elements_amd import MPMSolver
+    from taichi_elements_amd._lib import check
+    from taichi_elements_amd._lib import lib as _l
+    L = _l()
+    rng = np.random.default_rng(0)
+    x = (np.array([0.2, 0.4], np.float32)
+         + rng.random((20000, 2), dtype=np.float32) * np.array([0.3, 0.4], np.float32))
+    out = {}
+    for graphs in (True, False):
+        s = MPMSolver(res=(128, 128))
+        s.set_graphs(graphs)
+        s.read_restart(len(x), x, np.zeros_like(x), np.zeros(len(x), np.int32),
+                       np.zeros(len(x), np.int32))
+        dt = s.default_dt
+        best = None
+        for _ in range(repeats + 1):
+            t0 = time.perf_counter()
+            check(L.mpm_substeps(s._handle, steps, dt, 0.0))
+            check(L.mpm_sync(s._handle))
+            el = time.perf_counter() - t0
+            best = el if best is None else min(best, el)
+        out["graphs" if graphs else "kernel_by_kernel"] = 1e6 * best / steps
+        del s
+    return dict(us_per_substep=out, speedup=out["kernel_by_kernel"] / out["graphs"],
+                value=20000 / (out["graphs"] * 1e-6), unit="particle-substeps/s",
+                workload="2D water, 128^2, 20k particles, default_dt, 100 substeps per call")
+
+
 def algorithmic_bytes(n_particles, n_nodes):
     """Bytes a substep must move in the reference's field layout (float32):
     per particle 16 (binning) + 148 (p2g) + 80 (g2p), per active grid node
@@ -500,11 +532,14 @@ def main():
         "timed_with_events_ms_per_step": 1000 * (t3 - t2) / args.steps,
         "tuning": tuning,
         "g2p2g": None,
+        "small2d": None,
         "batched": None,
         "fwd_bwd": None,
     }
     if args.copy_calibration:
         out["copy_calibration_bytes"] = copy_calibration()
+    if args.config == "dense1m":
+        out["small2d"] = small2d_bench()
     if not args.no_g2p2g and args.config == "dense1m":
         del s
         out["g2p2g"] = g2p2g_bench(x, v, m, args.steps, args.warmup, not args.no_autotune)

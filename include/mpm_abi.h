@@ -194,6 +194,13 @@ int mpm_host_svd(int32_t dim, const float* A, float* U, float* sig, float* V);
  * without the clamp) is reported as MPM_EDOMAIN. */
 int mpm_set_g2p2g(mpm_solver* s, int32_t enabled, double allowed_cfl, int32_t clamp);
 
+/* HIP graphs (on by default): inside mpm_substeps / mpm_substeps_batch,
+ * every pair of steady-state substeps (neighbour-sort binning, no physical
+ * reorder due) is captured once as a graph and replayed, so a substep costs
+ * one graph launch instead of six kernel launches.  Results are bitwise the
+ * same as with enabled = 0 (kernel by kernel). */
+int mpm_set_graphs(mpm_solver* s, int32_t enabled);
+
 /* Launch tuning (no reference counterpart; the reference leaves launch
  * shapes to Taichi).  chunk: particles per transfer work item (16-256);
  * groups: workgroups of the persistent transfer kernels (64-16384); 0 keeps
@@ -246,6 +253,32 @@ int mpm_diff_sync(mpm_solver* s, void* stream);
  * colour.  Insertion order, every particle of the handle (all scenes of a
  * batch).  No particles is MPM_EINVAL (the reference's np.min raises). */
 int mpm_quantize_particles(mpm_solver* s, float* ranges, uint32_t* x_and_v, uint8_t* color);
+
+/* Voxelizer(res, dx, super_sample, padding=padding).voxelize(triangles)
+ * (engine/voxelizer.py:18-38, 111-126), run on the GPU of `device`.
+ * voxel_res (3) receives the voxel grid: every axis of res rounded up to a
+ * power of two, times super_sample (voxel size dx / super_sample).  With
+ * voxels non-NULL the (ntri, 9) float64 triangles (a, b, c) are voxelized
+ * and voxels receives the (voxel_res[0], voxel_res[1], voxel_res[2]) int32
+ * counts, C order: +1 per triangle facing up (normal z > 0) and -1 per
+ * triangle facing down above the voxel; positive inside a closed,
+ * outward-oriented mesh.  Heights are capped at voxel_res[2]. */
+int mpm_voxelize(int32_t device, const int32_t* res, double dx, int32_t super_sample,
+                 int32_t padding, int64_t ntri, const double* triangles, int32_t* voxel_res,
+                 int32_t* voxels);
+/* MPMSolver.add_mesh (mpm_solver.py:1049-1079) with seed_from_voxels
+ * (mpm_solver.py:1017-1047): voxelizes the triangles on the active scene's
+ * grid (res, dx, padding of its configuration) and seeds, in every voxel
+ * with a positive count, particle l = 0..sample_density when
+ * r + l < sample_density / super_sample^3, at ((r0, r1, r2) + voxel index) *
+ * (dx / super_sample) + translation, with velocity (3, NULL: zero), the
+ * material and the color.  r are uniform float32 in [0, 1) hashed from
+ * (seed, voxel, l, component) in place of ti.random; particles are appended
+ * in voxel order (C order of the voxel grid, then l).  *added (may be NULL)
+ * receives the number seeded.  3D only; sample_density 0..4096. */
+int mpm_add_mesh(mpm_solver* s, int64_t ntri, const double* triangles, int32_t material,
+                 int32_t color, int32_t sample_density, const float* velocity,
+                 const float* translation, int32_t super_sample, uint32_t seed, int64_t* added);
 
 #ifdef __cplusplus
 }

@@ -61,6 +61,7 @@ SIGNATURES = [
     ("mpm_last_counts", ctypes.c_int, [_P, _I64P]),
     ("mpm_host_svd", ctypes.c_int, [ctypes.c_int32, _FP, _FP, _FP, _FP]),
     ("mpm_set_g2p2g", ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_double, ctypes.c_int32]),
+    ("mpm_set_graphs", ctypes.c_int, [_P, ctypes.c_int32]),
     ("mpm_set_tuning", ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),
     ("mpm_get_tuning", ctypes.c_int, [_P, _I32P, _I32P]),
     ("mpm_diff_setup", ctypes.c_int, [_P, ctypes.c_int64, _I32P]),
@@ -72,6 +73,11 @@ SIGNATURES = [
     ("mpm_diff_sync", ctypes.c_int, [_P, _P]),
     ("mpm_quantize_particles", ctypes.c_int, [_P, _FP, ctypes.POINTER(ctypes.c_uint32),
                                               ctypes.POINTER(ctypes.c_uint8)]),
+    ("mpm_voxelize", ctypes.c_int, [ctypes.c_int32, _I32P, ctypes.c_double, ctypes.c_int32,
+                                    ctypes.c_int32, ctypes.c_int64, _D3, _I32P, _I32P]),
+    ("mpm_add_mesh", ctypes.c_int, [_P, ctypes.c_int64, _D3, ctypes.c_int32, ctypes.c_int32,
+                                    ctypes.c_int32, _FP, _FP, ctypes.c_int32, ctypes.c_uint32,
+                                    _I64P]),
 ]
 
 _lib = None

@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmpm_hip.so")
-SOURCES = ["mpm_kernels.hip", "mpm_io.hip", "mpm_runtime.hip"]
+SOURCES = ["mpm_kernels.hip", "mpm_io.hip", "mpm_mesh.hip", "mpm_runtime.hip"]
 HEADERS = ["mpm_device.h", "mpm_kernels.h"]
 
 HIPCC_FLAGS = [

@@ -33,7 +33,8 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(
     const int* __restrict__ pi, int64_t cap, const int* __restrict__ perm,
     const int4* __restrict__ chunks, const int* __restrict__ totals,
     const float4* __restrict__ grid_in, float4* __restrict__ tiles, int n_final,
-    unsigned* __restrict__ maxv, int* __restrict__ code, int* __restrict__ nkey,
+    unsigned* __restrict__ maxv, const int* __restrict__ step, int stride,
+    int* __restrict__ code, int* __restrict__ nkey,
     int* __restrict__ dcount, int* __restrict__ count_next, int* __restrict__ err) {
   using L = Layout<D>;
   constexpr int NPB = Geo<D>::kNodesPerBlock, TN = Geo<D>::kTileNodes;
@@ -476,7 +477,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(
     float m = red[0];
 #pragma unroll
     for (int w = 1; w < WAVES; w++) m = fmaxf(m, red[w]);
-    maxv[blockIdx.x] = __float_as_uint(m);
+    maxv[(int64_t)(step ? *step : 0) * stride + blockIdx.x] = __float_as_uint(m);
   }
 }
 

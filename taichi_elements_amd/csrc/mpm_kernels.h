@@ -65,6 +65,8 @@ struct SubstepBuffers {
   float4* grid;     // dense block-major grid [total_blocks][nodes] of (v, mass)
   int* err;         // error bits
   unsigned* maxv;   // per-substep, per-g2p-workgroup max |v| (float bits)
+  int* step;        // device substep counter selecting the maxv slot (null: slot 0);
+                    // the block scan increments it, so it starts at -1
   int total_blocks;
   int key_bits;
   int chunk;
@@ -156,6 +158,32 @@ void launch_minmax(int D, const float* pf, int64_t cap, int n, float* partial, i
                    hipStream_t st);
 void launch_quantize(int D, const float* pf, const int* orig, int64_t cap, int n,
                      const QuantRanges& q, unsigned* out, hipStream_t st);
+
+// Mesh voxelization and seeding (mpm_mesh.hip).
+struct VoxGrid {
+  int res[3];  // voxels per axis
+  int padding;
+  double dx, inv_dx;  // voxel size
+};
+struct SeedParams {
+  int sample_density;  // < 0: voxel counts only
+  int ss3;             // super_sample^3
+  int p_limit;         // first-axis voxel index bound (grid_size / 2 - padding)
+  unsigned seed;
+  float pdx;           // voxel size, float32
+  float shift[3];      // translation
+};
+// events: res0 * res1 * (res2 + 1) ints, zeroed by the caller
+void launch_vox_events(const double* tri, int64_t ntri, const VoxGrid& g, int* events,
+                       hipStream_t st);
+// Per column: voxel counts into voxels (may be null), seeded particles per
+// column into counts (may be null); with out_x set, the positions at the
+// column offsets (at most out_cap particles).
+void launch_vox_columns(const VoxGrid& g, const int* events, int* voxels, const SeedParams& sp,
+                        int* counts, const int64_t* offsets, float* out_x, int64_t out_cap,
+                        hipStream_t st);
+void launch_seed_attrs(int64_t n, const float v[3], int mat, float* vout, int* mout,
+                       hipStream_t st);
 
 // Event-bracketed kernel timing (enabled on demand).
 class Timer {

@@ -220,8 +220,10 @@ __global__ __launch_bounds__(kThreads) void k_bscan(
     int* __restrict__ pchunk_cnt, int4* __restrict__ chunks, int* __restrict__ active,
     int* __restrict__ totals, int* __restrict__ ghist, int* __restrict__ ghist_next,
     unsigned long long* __restrict__ status, LBState* __restrict__ lbs,
-    const int4* __restrict__ tile_prefix, int* __restrict__ err) {
+    const int4* __restrict__ tile_prefix, int* __restrict__ err, int* __restrict__ step) {
   __shared__ LBTicket tk_sh;
+  // one launch per substep: the substep counter of the max-velocity slots
+  if (step && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(step, 1);
   __shared__ int4 wsum[kThreads / 64];
   __shared__ unsigned excl_sh[4];
   __shared__ int lh[kMaxRadixPasses][kRadixBins];
@@ -880,6 +882,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p(const Params* __restrict__ sce
                                                   const int* __restrict__ totals,
                                                   const float4* __restrict__ grid,
                                                   unsigned* __restrict__ maxv,
+                                                  const int* __restrict__ step, int stride,
                                                   int* __restrict__ code, int* __restrict__ nkey,
                                                   int* __restrict__ dcount,
                                                   int* __restrict__ count_next,
@@ -1096,7 +1099,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p(const Params* __restrict__ sce
     for (int w = 1; w < kThreads / 64; w++) m = fmaxf(m, red[w]);
     // one partial maximum per workgroup, reduced by the reader: a single
     // device-wide atomic serialised thousands of workgroups
-    maxv[blockIdx.x] = __float_as_uint(m);
+    maxv[(int64_t)(step ? *step : 0) * stride + blockIdx.x] = __float_as_uint(m);
   }
 }
 
@@ -1323,7 +1326,7 @@ static void launch_binning(const Params* P, SubstepBuffers& B, hipStream_t st, T
     k_bscan<D><<<nparts, kThreads, 0, st>>>(
         P, B.count, B.count_next, B.total_blocks, B.chunk, nparts, B.full_sort ? passes : 0,
         B.pstart, B.pchunk_begin, B.pchunk_cnt, B.chunks, B.active, B.totals, ghist, ghist_next,
-        B.lb_status_scan, B.lbs, two_level ? B.tile_prefix : nullptr, B.err);
+        B.lb_status_scan, B.lbs, two_level ? B.tile_prefix : nullptr, B.err, B.step);
   }
   if (B.full_sort) {
     TimerScope ts(tm, "k_radix", st);
@@ -1379,8 +1382,7 @@ static void launch_g2p2g(const Params* P, const int* slab, SubstepBuffers& B, in
     TimerScope ts(tm, "k_g2p2g", st);
     k_g2p2g<D><<<clamp_groups(chunk_bound, G), kThreads, 0, st>>>(
         P, slab, B.pf, B.pi, B.cap, B.perm, B.chunks, B.totals, B.grid_in, B.tiles, B.n_final,
-        B.maxv + (int64_t)s_index * B.persistent_groups, B.code, B.nkey, B.dcount, B.count_next,
-        B.err);
+        B.maxv, B.step, B.persistent_groups, B.code, B.nkey, B.dcount, B.count_next, B.err);
   }
   {
     TimerScope ts(tm, "k_grid_op", st);
@@ -1412,7 +1414,8 @@ void launch_transfer(const Params* P, const int* slab, SubstepBuffers& B, int ra
   }
 }
 
-// g2p of one substep (after launch_front); s_index selects the max-velocity slot
+// g2p of one substep (after launch_front); the max-velocity slot is the
+// device substep counter B.step (s_index is kept for the interface)
 template <int D>
 void launch_g2p(const Params* P, const int* slab, SubstepBuffers& B, int s_index, hipStream_t st,
                 Timer* tm) {
@@ -1424,8 +1427,8 @@ void launch_g2p(const Params* P, const int* slab, SubstepBuffers& B, int s_index
     TimerScope ts(tm, "k_g2p", st);
     k_g2p<D><<<clamp_groups(chunk_bound, G), kThreads, 0, st>>>(
         P, slab, B.pf, B.pi, B.cap, B.perm, B.chunks, B.totals, B.grid,
-        B.maxv + (int64_t)s_index * B.persistent_groups, B.code, B.nkey, B.dcount, B.count_next,
-        B.err, B.pf_out, n);
+        B.maxv, B.step, B.persistent_groups, B.code, B.nkey, B.dcount, B.count_next, B.err,
+        B.pf_out, n);
   }
   std::swap(B.count, B.count_next);
 }

@@ -159,6 +159,15 @@ struct mpm_solver {
   bool have_perm = false;  // perm of the previous substep matches the slots
   int since_reorder = 1 << 30;  // reorder at the first opportunity
   int64_t substep_counter = 0;
+  int* d_step = nullptr;  // device substep counter (max-velocity slot)
+  // HIP graphs of two steady-state substeps, keyed by the buffer pointers
+  // and sizes their kernels were captured with
+  bool use_graphs = true;
+  struct GraphEntry {
+    std::vector<intptr_t> key;
+    hipGraphExec_t exec;
+  };
+  std::vector<GraphEntry> graphs;
   int reorder_interval = 16;
   int chunk = kChunkMax;
   int persistent_groups = 2048;
@@ -294,8 +303,16 @@ int free_particle_buffers(mpm_solver* s) {
 
 int alloc_chunk_buffers(mpm_solver* s);
 
+// Captured graphs hold buffer addresses: drop them when buffers move.
+void drop_graphs(mpm_solver* s) {
+  if (!s->graphs.empty()) hipStreamSynchronize(s->stream);
+  for (auto& g : s->graphs) hipGraphExecDestroy(g.exec);
+  s->graphs.clear();
+}
+
 int grow(mpm_solver* s, int64_t need) {
   if (need <= s->cap) return MPM_OK;
+  drop_graphs(s);
   if (need > (int64_t)INT32_MAX / 2) return fail(MPM_EINVAL, "too many particles (max 2^30)");
   int64_t ncap = std::max<int64_t>(need, std::max<int64_t>(1024, s->cap * 3 / 2));
   const int nf = s->nf();
@@ -338,6 +355,7 @@ int grow(mpm_solver* s, int64_t need) {
 // Buffers sized by the number of transfer work items (chunks), which
 // depends on the particle capacity and the chunk size.
 int alloc_chunk_buffers(mpm_solver* s) {
+  drop_graphs(s);
   dfree(s->tiles);
   dfree(s->chunks);
   dfree(s->dcount);
@@ -395,6 +413,7 @@ void fill_buffers(mpm_solver* s, SubstepBuffers& B) {
   B.grid = s->g2p2g ? s->grid_alt : s->grid;
   B.err = s->err;
   B.maxv = s->maxv;
+  B.step = s->d_step;
   B.total_blocks = s->total_blocks;
   B.key_bits = s->key_bits;
   B.chunk = s->chunk;
@@ -413,6 +432,154 @@ void take_buffers(mpm_solver* s, const SubstepBuffers& B) {
   s->last_parity = B.parity;
   s->count = B.count;
   s->count_next = B.count_next;
+}
+
+// One substep launched kernel by kernel (also what a graph captures).
+int eager_substep(mpm_solver* s, int k, Timer* tm) {
+  {
+    SubstepBuffers B;
+    fill_buffers(s, B);
+    if (s->have_perm && s->reorder_interval > 0 && s->since_reorder >= s->reorder_interval) {
+      if (s->D == 3)
+        launch_reorder<3>(B, s->pf_alt, s->pi_alt, s->stream, tm);
+      else
+        launch_reorder<2>(B, s->pf_alt, s->pi_alt, s->stream, tm);
+      std::swap(s->pf, s->pf_alt);
+      std::swap(s->pi, s->pi_alt);
+      s->since_reorder = 0;
+      s->reordered = true;
+      fill_buffers(s, B);
+    }
+    if (s->D == 3)
+      launch_substep<3>(s->d_params, s->d_slab, B, k, s->debug & 1, s->stream, tm);
+    else
+      launch_substep<2>(s->d_params, s->d_slab, B, k, s->debug & 1, s->stream, tm);
+    take_buffers(s, B);
+    s->substep_counter++;
+    s->have_perm = true;
+    s->reordered = false;
+    s->since_reorder++;
+  }
+  return MPM_OK;
+}
+
+// The next two substeps can run as one HIP graph: binning by the
+// neighbour sort (not the first substep, not right after positions were
+// set), even parity (the pair returns every double-buffered pointer to its
+// place), no physical reorder due, g2p2g past its first substep, no
+// per-kernel timing.
+bool steady_pair(const mpm_solver* s, int left, const Timer* tm) {
+  if (!s->use_graphs || tm || left < 2 || s->n == 0) return false;
+  if (!s->have_perm || s->reordered || (s->substep_counter & 1)) return false;
+  if (s->reorder_interval > 0 && s->since_reorder + 1 >= s->reorder_interval) return false;
+  if (s->g2p2g && s->n_final != s->n) return false;
+  return true;
+}
+
+std::vector<intptr_t> graph_key(const mpm_solver* s) {
+  return {(intptr_t)s->pf, (intptr_t)s->pi, (intptr_t)s->perm, (intptr_t)s->perm_alt,
+          (intptr_t)s->count, (intptr_t)s->count_next, (intptr_t)s->grid,
+          (intptr_t)s->grid_alt, (intptr_t)s->maxv, (intptr_t)s->tiles, (intptr_t)s->chunks,
+          (intptr_t)s->keys_alt, (intptr_t)s->dcount, (intptr_t)s->doff, (intptr_t)s->n,
+          (intptr_t)s->cap, (intptr_t)s->debug, (intptr_t)s->chunk,
+          (intptr_t)s->persistent_groups, (intptr_t)s->g2p2g};
+}
+
+// Two substeps as one graph launch: captured on first use (the capture
+// itself advances the host-side bookkeeping, as the eager path does), then
+// replayed with the same bookkeeping applied by hand.
+int run_graph_pair(mpm_solver* s) {
+  const std::vector<intptr_t> key = graph_key(s);
+  for (auto& g : s->graphs) {
+    if (g.key != key) continue;
+    HIP_TRY(hipGraphLaunch(g.exec, s->stream));
+    // bookkeeping of two substeps: every swapped pointer is back in place
+    s->last_parity = 1;
+    s->keys_sorted = s->keys_alt;
+    s->substep_counter += 2;
+    s->since_reorder += 2;
+    if (s->g2p2g) s->n_final = s->n;
+    return MPM_OK;
+  }
+  if (s->graphs.size() >= 8) {
+    for (auto& g : s->graphs) hipGraphExecDestroy(g.exec);
+    s->graphs.clear();
+  }
+  hipGraph_t graph = nullptr;
+  HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  int r = eager_substep(s, 0, nullptr);
+  if (r == MPM_OK) r = eager_substep(s, 1, nullptr);
+  hipError_t e = hipStreamEndCapture(s->stream, &graph);
+  if (r != MPM_OK) {
+    if (graph) hipGraphDestroy(graph);
+    return r;
+  }
+  if (e != hipSuccess) return fail(MPM_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (e != hipSuccess) return fail(MPM_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+  s->graphs.push_back({key, exec});
+  HIP_TRY(hipGraphLaunch(exec, s->stream));
+  return MPM_OK;
+}
+
+}  // namespace
+
+namespace {
+
+// Device array freed on scope exit (after the stream work that uses it has
+// been waited for by the caller).
+template <typename T>
+struct DeviceArray {
+  T* p = nullptr;
+  DeviceArray() = default;
+  DeviceArray(const DeviceArray&) = delete;
+  DeviceArray& operator=(const DeviceArray&) = delete;
+  ~DeviceArray() { dfree(p); }
+  int alloc(int64_t count) { return dalloc(&p, count); }
+};
+
+// Voxelizer.__init__ (engine/voxelizer.py:18-38): every axis of the solver
+// resolution rounded up to a power of two, times super_sample; voxel size
+// dx / super_sample.
+int voxel_grid(const int32_t* res, double dx, int32_t super_sample, int32_t padding,
+               VoxGrid* g) {
+  if (super_sample < 1 || super_sample > 16) return fail(MPM_EINVAL, "super_sample must be 1..16");
+  if (padding < 0) return fail(MPM_EINVAL, "negative padding");
+  if (!(dx > 0.0) || !std::isfinite(dx)) return fail(MPM_EINVAL, "dx must be positive");
+  for (int k = 0; k < 3; k++) {
+    if (res[k] < 1 || res[k] > (1 << 16)) return fail(MPM_EINVAL, "resolution out of range");
+    int r = 1;
+    while (r < res[k]) r *= 2;
+    g->res[k] = r * super_sample;
+  }
+  g->padding = padding;
+  g->dx = dx / super_sample;
+  g->inv_dx = 1.0 / g->dx;
+  return MPM_OK;
+}
+
+// Column-fill events of the triangles on stream st (events: res0 * res1 *
+// (res2 + 1) ints).  Waits for the stream before returning.
+int voxel_events(const VoxGrid& g, int64_t ntri, const double* triangles, hipStream_t st,
+                 DeviceArray<int>& events) {
+  if (ntri < 0) return fail(MPM_EINVAL, "negative triangle count");
+  if (ntri > 0 && !triangles) return fail(MPM_EINVAL, "null triangles");
+  for (int64_t i = 0; i < ntri * 9; i++)
+    if (!std::isfinite(triangles[i])) return fail(MPM_EINVAL, "triangle coordinates must be finite");
+  const int64_t nev = (int64_t)g.res[0] * g.res[1] * (g.res[2] + 1);
+  TRY(events.alloc(nev));
+  HIP_TRY(hipMemsetAsync(events.p, 0, sizeof(int) * nev, st));
+  DeviceArray<double> tri;
+  if (ntri > 0) {
+    TRY(tri.alloc(ntri * 9));
+    HIP_TRY(hipMemcpyAsync(tri.p, triangles, sizeof(double) * ntri * 9, hipMemcpyHostToDevice, st));
+    launch_vox_events(tri.p, ntri, g, events.p, st);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipStreamSynchronize(st));
+  return MPM_OK;
 }
 
 }  // namespace
@@ -489,7 +656,8 @@ static int create_impl(const mpm_config* cfgs, int nscenes, mpm_solver** out) {
       (r = dalloc(&s->tile_prefix, nparts)) ||
       (r = dalloc(&s->grid, tb * NPB)) || (r = dalloc(&s->totals, 32)) ||
       (r = dalloc(&s->err, 1)) || (r = dalloc(&s->maxv, 16 * s->persistent_groups)) ||
-      (r = dalloc(&s->d_params, nscenes)) || (r = dalloc(&s->d_slab, s->bn[D - 1]))) {
+      (r = dalloc(&s->d_params, nscenes)) || (r = dalloc(&s->d_slab, s->bn[D - 1])) ||
+      (r = dalloc(&s->d_step, 1))) {
     mpm_destroy(s);
     return r;
   }
@@ -551,6 +719,9 @@ int mpm_destroy(mpm_solver* s) {
   dfree(s->lb_status_scan);
   dfree(s->lbs);
   dfree(s->ghist);
+  for (auto& g : s->graphs) hipGraphExecDestroy(g.exec);
+  s->graphs.clear();
+  dfree(s->d_step);
   dfree(s->grid);
   dfree(s->grid_alt);
   dfree(s->totals);
@@ -684,6 +855,7 @@ static int substeps_impl(mpm_solver* s, int32_t nsub, const float* dts, bool bat
       return fail(MPM_EINVAL, "time steps of a batch must be finite and >= 0");
   HIP_TRY(hipSetDevice(s->device));
   if ((int64_t)nsub * s->persistent_groups > s->maxv_cap) {
+    drop_graphs(s);
     HIP_TRY(hipStreamSynchronize(s->stream));
     dfree(s->maxv);
     TRY(dalloc(&s->maxv, (int64_t)nsub * s->persistent_groups));
@@ -699,33 +871,20 @@ static int substeps_impl(mpm_solver* s, int32_t nsub, const float* dts, bool bat
   HIP_TRY(hipMemcpyAsync(s->d_params, s->h_params, sizeof(Params) * s->scenes.size(),
                          hipMemcpyHostToDevice, s->stream));
   Timer* tm = s->timing ? &s->timer : nullptr;
-  for (int k = 0; k < nsub; k++) {
-    SubstepBuffers B;
-    fill_buffers(s, B);
-    if (s->have_perm && s->reorder_interval > 0 && s->since_reorder >= s->reorder_interval) {
-      if (s->D == 3)
-        launch_reorder<3>(B, s->pf_alt, s->pi_alt, s->stream, tm);
-      else
-        launch_reorder<2>(B, s->pf_alt, s->pi_alt, s->stream, tm);
-      std::swap(s->pf, s->pf_alt);
-      std::swap(s->pi, s->pi_alt);
-      s->since_reorder = 0;
-      s->reordered = true;
-      fill_buffers(s, B);
+  launch_fill_u32((unsigned*)s->d_step, 0xFFFFFFFFu, 1, s->stream);  // slot k for substep k
+  for (int k = 0; k < nsub;) {
+    if (steady_pair(s, nsub - k, tm)) {
+      TRY(run_graph_pair(s));
+      k += 2;
+      continue;
     }
-    if (s->D == 3)
-      launch_substep<3>(s->d_params, s->d_slab, B, k, s->debug & 1, s->stream, tm);
-    else
-      launch_substep<2>(s->d_params, s->d_slab, B, k, s->debug & 1, s->stream, tm);
-    take_buffers(s, B);
-    s->substep_counter++;
-    s->have_perm = true;
-    s->reordered = false;
-    s->since_reorder++;
+    TRY(eager_substep(s, k, tm));
+    k++;
   }
   HIP_TRY(hipGetLastError());
   return MPM_OK;
 }
+
 
 int mpm_substeps(mpm_solver* s, int32_t nsub, float dt, double t0) {
   (void)t0;  // the built-in post-processes do not depend on t
@@ -1070,6 +1229,7 @@ BinLayout bin_layout(const mpm_solver* s) {
 
 void diff_buffers(mpm_solver* s, SubstepBuffers& B, float* pf) {
   fill_buffers(s, B);
+  B.step = nullptr;
   B.pf = pf;
   B.pf_out = pf;
   B.pi = s->diff_pi;
@@ -1107,6 +1267,13 @@ int mpm_set_g2p2g(mpm_solver* s, int32_t enabled, double allowed_cfl, int32_t cl
   return MPM_OK;
 }
 
+int mpm_set_graphs(mpm_solver* s, int32_t enabled) {
+  if (!s) return fail(MPM_EINVAL, "null argument");
+  s->use_graphs = enabled != 0;
+  if (!s->use_graphs) drop_graphs(s);
+  return MPM_OK;
+}
+
 int mpm_set_tuning(mpm_solver* s, int32_t chunk, int32_t groups) {
   if (!s) return fail(MPM_EINVAL, "null argument");
   if (chunk != 0 && (chunk < 16 || chunk > kChunkMax))
@@ -1117,6 +1284,7 @@ int mpm_set_tuning(mpm_solver* s, int32_t chunk, int32_t groups) {
   HIP_TRY(hipSetDevice(s->device));
   HIP_TRY(hipStreamSynchronize(s->stream));
   if (groups != 0 && groups != s->persistent_groups) {
+    drop_graphs(s);
     dfree(s->maxv);
     s->persistent_groups = groups;
     TRY(dalloc(&s->maxv, 16 * (int64_t)groups));
@@ -1294,6 +1462,108 @@ int mpm_quantize_particles(mpm_solver* s, float* ranges, uint32_t* x_and_v, uint
     const uint32_t u = (uint32_t)(int64_t)f;
     for (int c = 0; c < 3; c++) color[i * 3 + c] = (uint8_t)((u >> (8 * (2 - c))) & 255u);
   }
+  return check_device_errors(s);
+}
+
+
+int mpm_voxelize(int32_t device, const int32_t* res, double dx, int32_t super_sample,
+                 int32_t padding, int64_t ntri, const double* triangles, int32_t* voxel_res,
+                 int32_t* voxels) {
+  if (!res || !voxel_res) return fail(MPM_EINVAL, "null argument");
+  VoxGrid g{};
+  TRY(voxel_grid(res, dx, super_sample, padding, &g));
+  for (int k = 0; k < 3; k++) voxel_res[k] = g.res[k];
+  if (!voxels) return MPM_OK;
+  HIP_TRY(hipSetDevice(device));
+  hipStream_t st = nullptr;
+  HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  const int64_t nvox = (int64_t)g.res[0] * g.res[1] * g.res[2];
+  int rc = MPM_OK;
+  {
+    DeviceArray<int> events, vox;
+    rc = voxel_events(g, ntri, triangles, st, events);
+    if (rc == MPM_OK) rc = vox.alloc(nvox);
+    if (rc == MPM_OK) {
+      SeedParams sp{};
+      sp.sample_density = -1;
+      launch_vox_columns(g, events.p, vox.p, sp, nullptr, nullptr, nullptr, 0, st);
+      hipError_t e = hipGetLastError();
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(voxels, vox.p, sizeof(int) * nvox, hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess) e = hipStreamSynchronize(st);
+      if (e != hipSuccess) rc = fail(MPM_EHIP, std::string("voxelize: ") + hipGetErrorString(e));
+    }
+    hipStreamSynchronize(st);
+  }
+  hipStreamDestroy(st);
+  return rc;
+}
+
+int mpm_add_mesh(mpm_solver* s, int64_t ntri, const double* triangles, int32_t material,
+                 int32_t color, int32_t sample_density, const float* velocity,
+                 const float* translation, int32_t super_sample, uint32_t seed, int64_t* added) {
+  if (!s) return fail(MPM_EINVAL, "null argument");
+  if (added) *added = 0;
+  if (s->D != 3) return fail(MPM_EINVAL, "add_mesh needs a 3D solver");
+  if (material < 0 || material > MPM_STATIONARY)
+    return fail(MPM_EINVAL, "unknown material id " + std::to_string(material));
+  if (sample_density < 0 || sample_density > 4096)
+    return fail(MPM_EINVAL, "sample_density must be 0..4096");
+  TRY(check_device_errors(s));
+  HIP_TRY(hipSetDevice(s->device));
+  const mpm_config& cfg = s->scenes[s->active_scene].cfg;
+  VoxGrid g{};
+  TRY(voxel_grid(cfg.res, cfg.dx, super_sample, cfg.padding, &g));
+  SeedParams sp{};
+  sp.sample_density = sample_density;
+  sp.ss3 = super_sample * super_sample * super_sample;
+  sp.p_limit = 4096 / 2 - cfg.padding;  // bounded grid_size (mpm_solver.py:74, 1026-1029)
+  sp.seed = seed;
+  sp.pdx = (float)(cfg.dx / super_sample);
+  for (int k = 0; k < 3; k++) sp.shift[k] = translation ? translation[k] : 0.0f;
+  const int64_t ncol = (int64_t)g.res[0] * g.res[1];
+
+  DeviceArray<int> events, counts;
+  TRY(voxel_events(g, ntri, triangles, s->stream, events));
+  TRY(counts.alloc(ncol));
+  launch_vox_columns(g, events.p, nullptr, sp, counts.p, nullptr, nullptr, 0, s->stream);
+  HIP_TRY(hipGetLastError());
+  std::vector<int> hcount(ncol);
+  HIP_TRY(hipMemcpyAsync(hcount.data(), counts.p, sizeof(int) * ncol, hipMemcpyDeviceToHost,
+                         s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  std::vector<int64_t> offsets(ncol);
+  int64_t total = 0;
+  for (int64_t c = 0; c < ncol; c++) {
+    offsets[c] = total;
+    total += hcount[c];
+  }
+  if (total == 0) return MPM_OK;
+  if (s->n + total > (int64_t)INT32_MAX)
+    return fail(MPM_EINVAL, "mesh seeds " + std::to_string(total) + " particles: too many");
+  TRY(grow(s, s->n + total));
+  DeviceArray<int64_t> doff;
+  DeviceArray<float> dx, dv;
+  DeviceArray<int> dm;
+  TRY(doff.alloc(ncol));
+  TRY(dx.alloc(total * 3));
+  TRY(dv.alloc(total * 3));
+  TRY(dm.alloc(total));
+  HIP_TRY(hipMemcpyAsync(doff.p, offsets.data(), sizeof(int64_t) * ncol, hipMemcpyHostToDevice,
+                         s->stream));
+  launch_vox_columns(g, events.p, nullptr, sp, nullptr, doff.p, dx.p, total, s->stream);
+  const float zero[3] = {0.0f, 0.0f, 0.0f};
+  launch_seed_attrs(total, velocity ? velocity : zero, material, dv.p, dm.p, s->stream);
+  SubstepBuffers B;
+  fill_buffers(s, B);
+  launch_append<3>(B, dx.p, dv.p, dm.p, (int)total, (int)s->n, s->active_scene, s->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  s->color.insert(s->color.end(), (size_t)total, color);
+  s->n += total;
+  s->have_perm = false;
+  s->since_reorder = 1 << 30;  // the new particles sit at the end: reorder soon
+  if (added) *added = total;
   return check_device_errors(s);
 }
 

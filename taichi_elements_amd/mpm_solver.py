@@ -414,8 +414,41 @@ class MPMSolver:
                 self.add_surface_collider(**kw)
         self._emitter_ids = np.zeros(0, np.int32)
 
-    def add_mesh(self, *args, **kwargs):
-        raise NotImplementedError("add_mesh (voxelizer seeding) is not part of this engine")
+    def add_mesh(self,
+                 triangles,
+                 material,
+                 color=0xFFFFFF,
+                 sample_density=None,
+                 velocity=None,
+                 translation=None,
+                 emmiter_id=0):
+        """mpm_solver.py:1049-1079: voxelize the closed triangle mesh
+        ((n, 9): a, b, c per row; engine/voxelizer.py) and seed particles in
+        the voxels inside it (seed_from_voxels, mpm_solver.py:1017-1047).
+        Both run on the GPU; the random numbers come from a counter hash
+        seeded by this solver's generator, in place of ti.random."""
+        assert self.dim == 3
+        if not self.use_voxelizer:
+            raise ValueError("add_mesh needs use_voxelizer=True")
+        if sample_density is None:
+            sample_density = 2**self.dim
+        self.set_source_velocity(velocity=velocity)
+        assert isinstance(triangles, np.ndarray)
+        tri = np.ascontiguousarray(triangles.astype(np.float64))  # engine/voxelizer.py:113
+        assert tri.ndim == 2 and tri.shape[1] == 9
+        shift = np.zeros(3, np.float32)
+        if translation:
+            shift[:] = [float(translation[i]) for i in range(3)]
+        vel = np.asarray(self._source_velocity, np.float32)
+        seed = int(self._rng.integers(0, 2**32))
+        added = _lib.ctypes.c_int64()
+        check(_lib.lib().mpm_add_mesh(
+            self._h(), len(tri), tri.ctypes.data_as(_lib._D3), int(material),
+            int(np.int64(color).astype(np.int32)), int(sample_density), _lib.fptr(vel),
+            _lib.fptr(shift), int(self.voxelizer_super_sample), seed, added))
+        self._emitter_ids = np.concatenate(
+            [self._emitter_ids, np.full(int(added.value), emmiter_id, np.int32)])
+        return int(added.value)
 
     def add_ngon(self, *args, **kwargs):
         raise NotImplementedError("add_ngon is not part of this engine")
@@ -602,6 +635,10 @@ class MPMSolver:
 
     def set_tuning(self, chunk=0, groups=0):
         check(_lib.lib().mpm_set_tuning(self._h(), int(chunk), int(groups)))
+
+    def set_graphs(self, enabled):
+        """Replay steady-state substep pairs as HIP graphs (default on)."""
+        check(_lib.lib().mpm_set_graphs(self._h(), int(bool(enabled))))
 
     def autotune(self, dt=None, substeps=4, chunks=(64, 128, 256), groups=(1024, 2048, 4096),
                  verbose=False):
