@@ -263,14 +263,15 @@ def fwd_bwd_bench(nsub=64, checkpoint_every=8, repeats=3, seed=4):
                          "gradients w.r.t. v0, E, nu")
 
 
-def g2p2g_bench(x, v, m, steps, warmup, autotune=True):
-    """The same 1M-particle workload with use_g2p2g=True (one fused transfer
-    per substep, grid velocity clamp at 0.9 cells)."""
+def variant_bench(x, v, m, steps, warmup, autotune, workload, **solver_kw):
+    """The same 1M-particle workload with other solver options (use_g2p2g:
+    one fused transfer per substep, grid velocity clamp at 0.9 cells; quant:
+    packed particle storage; support_plasticity)."""
     from taichi_elements_amd import MPMSolver
     from taichi_elements_amd._lib import check
     from taichi_elements_amd._lib import lib as _l
     L = _l()
-    s = MPMSolver(res=(RES, RES, RES), use_g2p2g=True)
+    s = MPMSolver(res=(RES, RES, RES), **solver_kw)
     s.read_restart(len(x), x, v, m, np.zeros(len(x), np.int32))
     if autotune:
         s.autotune(substeps=3)
@@ -290,8 +291,26 @@ def g2p2g_bench(x, v, m, steps, warmup, autotune=True):
     check(L.mpm_set_timing(s._handle, 0))
     return dict(value=len(x) * steps / el, ms_per_substep=1000 * el / steps,
                 unit="particle-substeps/s", tuning=dict(zip(("chunk", "groups"), s.tuning())),
+                bytes_per_particle=s.particle_bytes(),
                 kernels_us={nm: round(1000 * t / max(c, 1), 2) for nm, c, t in zip(names, calls, ms)},
-                workload="3D four-material 1M particles, 128^3, use_g2p2g=True")
+                workload=workload)
+
+
+def g2p2g_bench(x, v, m, steps, warmup, autotune=True):
+    return variant_bench(x, v, m, steps, warmup, autotune,
+                         "3D four-material 1M particles, 128^3, use_g2p2g=True", use_g2p2g=True)
+
+
+def quant_bench(x, v, m, steps, warmup, autotune=True):
+    """Packed particle storage on the p2g / g2p path, and the reference's
+    large-scale combination (quant, use_g2p2g, support_plasticity=False)."""
+    out = variant_bench(x, v, m, steps, warmup, autotune,
+                        "3D four-material 1M particles, 128^3, quant=True", quant=True)
+    out["g2p2g_no_plasticity"] = variant_bench(
+        x, v, m, steps, warmup, autotune,
+        "3D four-material 1M particles, 128^3, quant=True, use_g2p2g=True, "
+        "support_plasticity=False", quant=True, use_g2p2g=True, support_plasticity=False)
+    return out
 
 
 def small2d_bench(steps=100, repeats=3):
@@ -388,16 +407,18 @@ def main():
     ap.add_argument("--reorder-interval", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--config", choices=["dense1m", "sparse16m", "fwdbwd200k", "g2p2g1m",
-                                         "batched64"], default="dense1m",
+                                         "batched64", "quant1m"], default="dense1m",
                     help="dense1m: the 3D four-material 128^3 workload (default, the "
                          "headline line); sparse16m: 512^3 window, 16M particles in 64 "
                          "ellipsoids with a plane collider; fwdbwd200k / g2p2g1m / "
-                         "batched64: only the differentiable, use_g2p2g or 64-scene batched "
-                         "measurement")
+                         "batched64 / quant1m: only the differentiable, use_g2p2g, 64-scene "
+                         "batched or packed-storage measurement")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shape instead of autotuning it on the scene")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the 64-scene batched measurement")
+    ap.add_argument("--no-quant", action="store_true",
+                    help="skip the packed-storage (quant=True) legs")
     ap.add_argument("--no-g2p2g", action="store_true",
                     help="skip the use_g2p2g measurement")
     ap.add_argument("--no-fwd-bwd", action="store_true",
@@ -413,12 +434,13 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    if args.config in ("g2p2g1m", "batched64"):
+    if args.config in ("g2p2g1m", "batched64", "quant1m"):
         from taichi_elements_amd import mpm_solver
         mpm_solver.PRINT_PROGRESS = False
-        if args.config == "g2p2g1m":
+        if args.config in ("g2p2g1m", "quant1m"):
             x, v, m = make_scene()
-            leg = g2p2g_bench(x, v, m, args.steps, args.warmup, not args.no_autotune)
+            fn = g2p2g_bench if args.config == "g2p2g1m" else quant_bench
+            leg = fn(x, v, m, args.steps, args.warmup, not args.no_autotune)
             data = "synthetic (seeded numpy: 4 cubes x 250k particles, v ~ U[-1,1]^3)"
         else:
             leg = batched_bench(args.steps, args.warmup, not args.no_autotune)
@@ -532,6 +554,7 @@ def main():
         "timed_with_events_ms_per_step": 1000 * (t3 - t2) / args.steps,
         "tuning": tuning,
         "g2p2g": None,
+        "quant": None,
         "small2d": None,
         "batched": None,
         "fwd_bwd": None,
@@ -543,6 +566,8 @@ def main():
     if not args.no_g2p2g and args.config == "dense1m":
         del s
         out["g2p2g"] = g2p2g_bench(x, v, m, args.steps, args.warmup, not args.no_autotune)
+    if not args.no_quant and args.config == "dense1m":
+        out["quant"] = quant_bench(x, v, m, args.steps, args.warmup, not args.no_autotune)
     if not args.no_batched and args.config == "dense1m":
         out["batched"] = batched_bench(args.steps, args.warmup, not args.no_autotune)
     if not args.no_fwd_bwd and args.config == "dense1m":

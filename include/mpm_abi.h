@@ -254,6 +254,25 @@ int mpm_diff_sync(mpm_solver* s, void* stream);
  * batch).  No particles is MPM_EINVAL (the reference's np.min raises). */
 int mpm_quantize_particles(mpm_solver* s, float* ranges, uint32_t* x_and_v, uint8_t* color);
 
+/* MPMSolver(quant=..., support_plasticity=...) (mpm_solver.py:47, 62,
+ * 106-136, 216-262).  quant = 1 stores each particle packed: x as three
+ * signed 21-bit fixed-point values (step 2^-19, range [-2, 2)), v as a
+ * 7-bit shared exponent and three 19-bit sign-magnitude mantissas, F as
+ * signed 16-bit fixed-point values (step 4.1 / 32768); every store rounds to
+ * nearest and saturates, and the kernels compute with what a load returns
+ * (x advances by the stored v; in use_g2p2g mode the new F is clamped to
+ * +-4 and rounded before the SVD, mpm_solver.py:413-416).  C (absent in
+ * use_g2p2g mode) and Jp stay float32.  support_plasticity = 0 drops Jp: no
+ * hardening, no Jp updates, water restarts each substep from F = I, sand
+ * has no stress.  Only before particles are added; not for the
+ * differentiable mode.  mpm_get_state / mpm_particle_info return the stored
+ * (rounded) values. */
+int mpm_set_storage(mpm_solver* s, int32_t quant, int32_t support_plasticity);
+/* Bytes one particle occupies (device state plus its host-side colour) in
+ * the current storage mode (the reference's figure is 40 B for quant with
+ * use_g2p2g and no plasticity, README.md:60). */
+int mpm_particle_bytes(mpm_solver* s, int32_t* bytes);
+
 /* Voxelizer(res, dx, super_sample, padding=padding).voxelize(triangles)
  * (engine/voxelizer.py:18-38, 111-126), run on the GPU of `device`.
  * voxel_res (3) receives the voxel grid: every axis of res rounded up to a

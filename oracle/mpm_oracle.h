@@ -67,7 +67,24 @@ typedef struct {
   /* g2p2g mode's grid velocity clamp (mpm:589, 596-598): each component
    * limited to dx * v_clamp_cfl / dt; 0 = no clamp (the p2g/g2p path). */
   double v_clamp_cfl;
+  /* MPMSolver(quant=True) (mpm_solver.py:106-114, 216-262): x, v and F are
+   * stored quantised -- every store is replaced by the value a load of the
+   * packed field returns (oracle_quant_*); in g2p2g the new F is clamped to
+   * +-F_bound first (mpm:413-416). */
+  int32_t quant;
+  /* MPMSolver(support_plasticity=False) (mpm_solver.py:62, 135-136, 419-421,
+   * 436-437, 456, 510-518, 533-534, 541-542, 553). */
+  int32_t no_plasticity;
 } oracle_cfg;
+
+/* The packed particle formats, in place: the value each element reads back
+ * after a store (round to nearest even, saturating).  x: signed 21-bit
+ * fixed point, step 2^-19.  F: signed 16-bit fixed point, step 4.1/32768.
+ * v (n vectors of dim): one 7-bit exponent shared by the vector's
+ * components and 18-bit magnitudes (+ sign). */
+void oracle_quant_x(int64_t n, float* x);
+void oracle_quant_f(int64_t n, float* f);
+void oracle_quant_v(int dim, int64_t n, float* v);
 
 /* ---- float instantiation ---- */
 void oracle_svd_f32(int dim, const float* A, float* U, float* sig, float* V);

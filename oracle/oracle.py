@@ -41,7 +41,8 @@ class Cfg(ctypes.Structure):
                 ("grid_lo", ctypes.c_int32 * 3), ("grid_n", ctypes.c_int32 * 3),
                 ("post", PostOp * MAX_POST),
                 ("youngs", ctypes.POINTER(ctypes.c_double)), ("nu", ctypes.c_double),
-                ("v_clamp_cfl", ctypes.c_double)]
+                ("v_clamp_cfl", ctypes.c_double),
+                ("quant", ctypes.c_int32), ("no_plasticity", ctypes.c_int32)]
 
 
 def build():
@@ -290,6 +291,40 @@ def with_clamp(cfg, cfl):
     if hasattr(cfg, "_youngs_ref"):
         c._youngs_ref = cfg._youngs_ref
     return c
+
+
+def with_storage(cfg, quant=False, plasticity=True):
+    """A copy of cfg with MPMSolver(quant=..., support_plasticity=...)."""
+    c = Cfg()
+    ctypes.pointer(c)[0] = cfg
+    c.quant = 1 if quant else 0
+    c.no_plasticity = 0 if plasticity else 1
+    if hasattr(cfg, "_youngs_ref"):
+        c._youngs_ref = cfg._youngs_ref
+    return c
+
+
+def quant_x(x):
+    """Positions as the quantised x field stores them (float32 copy)."""
+    a = np.array(x, np.float32, copy=True, order="C")
+    lib().oracle_quant_x(ctypes.c_int64(a.size), _ptr(a, ctypes.c_float))
+    return a
+
+
+def quant_f(f):
+    """F entries as the quantised F field stores them (float32 copy)."""
+    a = np.array(f, np.float32, copy=True, order="C")
+    lib().oracle_quant_f(ctypes.c_int64(a.size), _ptr(a, ctypes.c_float))
+    return a
+
+
+def quant_v(v):
+    """Velocities (..., dim) as the shared-exponent v field stores them."""
+    a = np.array(v, np.float32, copy=True, order="C")
+    dim = a.shape[-1]
+    lib().oracle_quant_v(ctypes.c_int(dim), ctypes.c_int64(a.size // dim),
+                         _ptr(a, ctypes.c_float))
+    return a
 
 
 class G2P2GStepper:

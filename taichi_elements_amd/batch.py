@@ -123,6 +123,13 @@ class MPMSolverBatch:
         h = _lib._P()
         check(L.mpm_create_batch(cfgs, len(self.scenes), h))
         self._handle = h
+        storage = {(bool(s.quant), bool(s.support_plasticity)) for s in self.scenes}
+        if len(storage) > 1:
+            raise ValueError("quant and support_plasticity must be the same for every scene "
+                             "of a batch")
+        quant, plastic = storage.pop()
+        if quant or not plastic:
+            check(L.mpm_set_storage(h, int(quant), int(plastic)))
         if kernel_profiler:
             check(L.mpm_set_timing(h, 1))
         for s in self.scenes:

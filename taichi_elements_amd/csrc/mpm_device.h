@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "mpm_particles.h"
+
 namespace mpm {
 
 enum Material : int { kWater = 0, kElastic = 1, kSnow = 2, kSand = 3, kStationary = 4 };
@@ -66,6 +68,9 @@ struct Params {
   // velocity component (mpm:589, 596-598; <= 0: no clamp).
   int g2p2g;
   float v_allowed;
+  // support_plasticity (mpm_solver.py:62, 135-136): 0 drops Jp -- no
+  // hardening, no Jp updates, sand without its plastic branch
+  int plastic;
   int n_post;
   PostOp post[kMaxPost];
 };
@@ -306,23 +311,39 @@ __device__ __forceinline__ void sand_projection(float sig[3], float& Jp, const P
   }
 }
 
+// F rounded to the packed layout's 16-bit fixed point (what a store then a
+// load of the quantised F field returns).
+template <int D>
+__device__ __forceinline__ void quant_f_round(float F[3][3]) {
+#pragma unroll
+  for (int i = 0; i < D; i++)
+#pragma unroll
+    for (int j = 0; j < D; j++) F[i][j] = quant_f_value(quant_f_code(F[i][j]));
+}
+
 // Per-particle F / Jp update and Kirchhoff stress, mpm_solver.py:507-568.
 // F is updated in place; stress receives tau (before the p2g scale).
 // G2P2G = true: the update inside g2p2g (mpm_solver.py:416-473), which
 // differs for water: F is not rebuilt from Jp first, the hardening
 // exp(10 (1 - Jp)) applies, and Jp is left unchanged (the volume is kept
-// in F[0][0] only).
-template <int D, bool G2P2G = false>
+// in F[0][0] only).  QF (quantised storage, g2p2g only): the new F is
+// clamped to +-F_bound and, like every F the reference writes to its
+// quantised field and reads back before the stress, rounded to the packed
+// format (mpm_solver.py:413-416, 428, 444-456).  P.plastic = 0
+// (support_plasticity=False): h = 1, Jp is neither read nor updated, water
+// restarts from F = I, sand keeps F and has no stress.
+template <int D, bool G2P2G = false, bool QF = false>
 __device__ __forceinline__ void constitutive(int mat, const Params& P, float mu_0, float lambda_0,
                                              float F[3][3], const float C[3][3], float& Jp,
                                              float stress[3][3]) {
+  const bool plastic = P.plastic != 0;
   float G[3][3];
   if (mat == kWater && !G2P2G) {  // mpm_solver.py:508-511
 #pragma unroll
     for (int i = 0; i < D; i++)
 #pragma unroll
       for (int j = 0; j < D; j++) F[i][j] = i == j ? 1.0f : 0.0f;
-    F[0][0] = Jp;
+    if (plastic) F[0][0] = Jp;
   }
 #pragma unroll
   for (int i = 0; i < D; i++)  // (I + dt C) F, mpm_solver.py:513
@@ -336,7 +357,8 @@ __device__ __forceinline__ void constitutive(int mat, const Params& P, float mu_
 #pragma unroll
   for (int i = 0; i < D; i++)
 #pragma unroll
-    for (int j = 0; j < D; j++) F[i][j] = G[i][j];
+    for (int j = 0; j < D; j++) F[i][j] = QF ? fmaxf(-kFBound, fminf(kFBound, G[i][j])) : G[i][j];
+  if (QF) quant_f_round<D>(F);
 
   if (mat == kWater) {
     // Water (mpm_solver.py:515-551 with mu = 0, h = 1): the singular values
@@ -355,10 +377,11 @@ __device__ __forceinline__ void constitutive(int mat, const Params& P, float mu_
       }
     F[0][0] = J;
     float la = lambda_0;
-    if (G2P2G)
-      la = lambda_0 * __expf(10.0f * (1.0f - Jp));  // mpm:422-424 (Jp stays 1 unless set)
-    else
+    if (G2P2G) {
+      if (plastic) la = lambda_0 * __expf(10.0f * (1.0f - Jp));  // mpm:419-424 (Jp stays 1 unless set)
+    } else if (plastic) {
       Jp = J;
+    }
     const float iso = la * J * (J - 1.0f);
 #pragma unroll
     for (int i = 0; i < D; i++) stress[i][i] = iso;
@@ -366,9 +389,17 @@ __device__ __forceinline__ void constitutive(int mat, const Params& P, float mu_
   }
 
   float h = 1.0f;  // mpm_solver.py:515-524 (sand ignores h: its stress uses mu_0, lambda_0)
-  if (mat == kSnow) h = __expf(10.0f * (1.0f - Jp));
+  if (mat == kSnow && plastic) h = __expf(10.0f * (1.0f - Jp));
   if (mat == kElastic) h = 0.3f;
   float mu = mu_0 * h, la = lambda_0 * h;
+
+  if (mat == kSand && !plastic) {  // mpm_solver.py:552-553: no stress, F kept
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+      for (int j = 0; j < D; j++) stress[i][j] = 0.0f;
+    return;
+  }
 
   float U[3][3], V[3][3], sig[3];
   if (D == 3)
@@ -382,18 +413,10 @@ __device__ __forceinline__ void constitutive(int mat, const Params& P, float mu_
     for (int i = 0; i < D; i++) {
       float ns = sig[i];
       if (mat == kSnow) ns = fminf(fmaxf(sig[i], 1.0f - 2.5e-2f), 1.0f + 4.5e-3f);
-      Jp *= sig[i] / ns;
+      if (plastic) Jp *= sig[i] / ns;
       sig[i] = ns;
       J *= ns;
     }
-  }
-  if (mat == kWater) {  // mpm_solver.py:537-542
-#pragma unroll
-    for (int i = 0; i < D; i++)
-#pragma unroll
-      for (int j = 0; j < D; j++) F[i][j] = i == j ? 1.0f : 0.0f;
-    F[0][0] = J;
-    Jp = J;
   }
   if (mat == kSand) sand_projection<D>(sig, Jp, P);  // mpm_solver.py:554
   if (mat == kSnow || mat == kSand) {                // F = U sig V^T, mpm_solver.py:545, 555
@@ -406,6 +429,7 @@ __device__ __forceinline__ void constitutive(int mat, const Params& P, float mu_
         for (int k = 0; k < D; k++) s += U[i][k] * sig[k] * V[j][k];
         F[i][j] = s;
       }
+    if (QF) quant_f_round<D>(F);
   }
   if (mat != kSand) {  // tau = 2 mu (F - U V^T) F^T + la J (J - 1) I, mpm_solver.py:549-551
     float Dm[3][3];

@@ -27,7 +27,7 @@ struct G2P2GGeo {
   static constexpr int kNodes = D == 3 ? kEdge * kEdge * kEdge : kEdge * kEdge;
 };
 
-template <int D>
+template <int D, bool Q>
 __global__ __launch_bounds__(kThreads) void k_g2p2g(
     const Params* __restrict__ scenes, const int* __restrict__ slab, float* __restrict__ pf,
     const int* __restrict__ pi, int64_t cap, const int* __restrict__ perm,
@@ -94,16 +94,17 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(
     int cell = 0, shell = -1;  // shell: output-tile node of the new base (crossed particles)
     int dir = 13, key_new = ck.x;
     if (live) {
+      using PS = Particles<D, Q, true>;
       const int p = perm[ck.y + t];
       const int mat = pi[(int64_t)L::MAT * cap + p];
       float xp[3] = {0, 0, 0}, fx[3] = {0, 0, 0}, w[3][3], vp[3] = {0, 0, 0};
       int l[3] = {0, 0, 0};
 #pragma unroll
       for (int a = 0; a < 3; a++) w[a][0] = w[a][1] = w[a][2] = 0.0f;
+      PS::load_x(pf, cap, p, xp);
+      PS::load_v(pf, cap, p, vp);
 #pragma unroll
       for (int a = 0; a < D; a++) {
-        xp[a] = pf[(int64_t)(L::X + a) * cap + p];
-        vp[a] = pf[(int64_t)(L::V + a) * cap + p];
         int base;
         base_fx(xp[a], P.inv_dx, base, fx[a]);
         bspline(fx[a], w[a]);
@@ -156,14 +157,14 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(
         for (int a = 0; a < D; a++) nv[a] = vp[a];
       }
       float xn[3] = {xp[0], xp[1], xp[2]};
-      if (mat != kStationary) {  // mpm:401-403
+      if (mat != kStationary) {  // mpm:401-403: x advances by the stored v
+        PS::store_v(pf, cap, p, nv);
 #pragma unroll
         for (int a = 0; a < D; a++) {
           vp[a] = nv[a];
           xn[a] = xp[a] + P.dt * nv[a];
-          pf[(int64_t)(L::V + a) * cap + p] = nv[a];
-          pf[(int64_t)(L::X + a) * cap + p] = xn[a];
         }
+        PS::store_x(pf, cap, p, xn);  // xn: the stored x, scattered around below
       }
 #pragma unroll
       for (int a = 0; a < D; a++) vmax = fmaxf(vmax, fabsf(vp[a]));
@@ -192,17 +193,11 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(
       }
       // constitutive update with the G2P's C (mpm:409-473)
       float F[3][3] = {{0}}, stress[3][3] = {{0}};
-#pragma unroll
-      for (int a = 0; a < D; a++)
-#pragma unroll
-        for (int e = 0; e < D; e++) F[a][e] = pf[(int64_t)(L::F + a * D + e) * cap + p];
-      float Jp = pf[(int64_t)L::JP * cap + p];
-      constitutive<D, true>(mat, P, P.mu_0, P.lambda_0, F, nC, Jp, stress);
-#pragma unroll
-      for (int a = 0; a < D; a++)
-#pragma unroll
-        for (int e = 0; e < D; e++) pf[(int64_t)(L::F + a * D + e) * cap + p] = F[a][e];
-      pf[(int64_t)L::JP * cap + p] = Jp;
+      PS::load_F(pf, cap, p, F);
+      float Jp = P.plastic ? PS::load_Jp(pf, cap, p) : 1.0f;
+      constitutive<D, true, Q>(mat, P, P.mu_0, P.lambda_0, F, nC, Jp, stress);
+      PS::store_F(pf, cap, p, F);
+      if (P.plastic) PS::store_Jp(pf, cap, p, Jp);
       const float mass = P.p_mass;  // mpm:474-483 (no water density here)
       float Ad[3][3] = {{0.0f}}, q[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll

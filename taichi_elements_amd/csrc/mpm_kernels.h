@@ -5,18 +5,9 @@
 #include <stdint.h>
 
 #include "mpm_device.h"
+#include "mpm_particles.h"
 
 namespace mpm {
-
-// Float particle components, each a [cap] array inside one allocation:
-// x (D), v (D), C (D*D), F (D*D), Jp (1).  Int components: material, slot
-// insertion index.
-template <int D>
-struct Layout {
-  static constexpr int X = 0, V = D, C = 2 * D, F = 2 * D + D * D, JP = 2 * D + 2 * D * D;
-  static constexpr int NF = 2 * D + 2 * D * D + 1;
-  static constexpr int MAT = 0, ORIG = 1, SCENE = 2, NI = 3;
-};
 
 constexpr int kThreads = 256;
 constexpr int kChunkMax = 256;                      // particles per transfer work item (max)
@@ -89,6 +80,8 @@ struct SubstepBuffers {
   bool g2p2g;
   float4* grid_in;
   int n_final;
+  bool quant;         // packed particle layout (PackedLayout<D>)
+  int nw;             // particle float-buffer words per slot
   bool full_sort;     // radix-sort the slots (first substep / positions set by the host)
   bool perm_identity; // the slots were just reordered into the previous order
   int* keys_sorted;   // output: block keys in sorted order
@@ -128,7 +121,16 @@ void launch_reorder(SubstepBuffers& B, float* pf_alt, int* pi_alt, hipStream_t s
 
 template <int D>
 void launch_append(SubstepBuffers& B, const float* x, const float* v, const int* mat, int k,
-                   int n_old, int scene, hipStream_t st);
+                   int n_old, int scene, int plastic, hipStream_t st);
+// Packed layout <-> float layout (Layout<D>, same cap) for the first n slots;
+// unpacking fills absent components (C in use_g2p2g mode: 0, Jp without
+// plasticity: 1).
+template <int D>
+void launch_unpack(const float* packed, float* flt, int n, int64_t cap, bool g2p2g, bool plastic,
+                   hipStream_t st);
+template <int D>
+void launch_pack(const float* flt, float* packed, int n, int64_t cap, bool g2p2g, bool plastic,
+                 hipStream_t st);
 
 void launch_gather_f(const float* src, int64_t cap, int ncomp, const int* orig, int n,
                      float* dst, hipStream_t st);

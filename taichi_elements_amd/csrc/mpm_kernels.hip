@@ -60,7 +60,7 @@ __device__ __forceinline__ int4 block_exscan4(int4 v, int4* total, int4* wsum) {
 // ---------------------------------------------------------------------------
 // binning: block keys and per-block counts (build_pid, mpm_solver.py:357-361)
 // ---------------------------------------------------------------------------
-template <int D>
+template <int D, bool Q>
 __global__ __launch_bounds__(kThreads) void k_keys(const Params* __restrict__ scenes,
                                                    const float* __restrict__ pf,
                                                    const int* __restrict__ pi, int64_t cap, int n,
@@ -74,13 +74,15 @@ __global__ __launch_bounds__(kThreads) void k_keys(const Params* __restrict__ sc
     int key = total_blocks;
     if (i < n) {
       const Params& P = scenes[pi[(int64_t)L::SCENE * cap + i]];
+      float xi[3];
+      Particles<D, Q>::load_x(pf, cap, i, xi);
       int lin = 0;
       valid = true;
 #pragma unroll
       for (int a = D - 1; a >= 0; a--) {
         int base;
         float fx;
-        base_fx(pf[(int64_t)(L::X + a) * cap + i], P.inv_dx, base, fx);
+        base_fx(xi[a], P.inv_dx, base, fx);
         const int lb = (base >> 2) - P.blo[a];
         // the block and its +1 neighbour must lie inside the scene's window
         if (lb < 0 || lb >= P.bnl[a] - 1) valid = false;
@@ -470,7 +472,7 @@ __global__ __launch_bounds__(kThreads) void k_onesweep(
 // a = vbar' + dt xbar' and Cbar the adjoint of the new C (gf holds the
 // adjoints of the new particle state, same layout as pf); the mass channel
 // stays zero.  Both scatter "q + Ad o" per node through the same LDS path.
-template <int D, int MODE>
+template <int D, int MODE, bool Q = false>
 __global__ __launch_bounds__(kThreads) void k_p2g(const Params* __restrict__ scenes,
                                                   const int* __restrict__ slab,
                                                   float* __restrict__ pf,
@@ -551,22 +553,15 @@ __global__ __launch_bounds__(kThreads) void k_p2g(const Params* __restrict__ sce
         stage[t][2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       }
     } else if (MODE == 0 && valid) {
+      using PS = Particles<D, Q>;
       const int p = perm[ck.y + t];
       const int mat = pi[(int64_t)L::MAT * cap + p];
       float xp[3], vp[3], C[3][3], F[3][3], stress[3][3];
-#pragma unroll
-      for (int a = 0; a < D; a++) {
-        xp[a] = pf[(int64_t)(L::X + a) * cap + p];
-        vp[a] = pf[(int64_t)(L::V + a) * cap + p];
-      }
-#pragma unroll
-      for (int a = 0; a < D; a++)
-#pragma unroll
-        for (int e = 0; e < D; e++) {
-          C[a][e] = pf[(int64_t)(L::C + a * D + e) * cap + p];
-          F[a][e] = pf[(int64_t)(L::F + a * D + e) * cap + p];
-        }
-      float Jp = pf[(int64_t)L::JP * cap + p];
+      PS::load_x(pf, cap, p, xp);
+      PS::load_v(pf, cap, p, vp);
+      PS::load_C(pf, cap, p, C);
+      PS::load_F(pf, cap, p, F);
+      float Jp = P.plastic ? PS::load_Jp(pf, cap, p) : 1.0f;
       float fx[3] = {0.0f, 0.0f, 0.0f};
       int l[3] = {0, 0, 0};
 #pragma unroll
@@ -584,11 +579,8 @@ __global__ __launch_bounds__(kThreads) void k_p2g(const Params* __restrict__ sce
         lambda_0 = E * P.kla;
       }
       constitutive<D>(mat, P, mu_0, lambda_0, F, C, Jp, stress);
-#pragma unroll
-      for (int a = 0; a < D; a++)
-#pragma unroll
-        for (int e = 0; e < D; e++) pf_out[(int64_t)(L::F + a * D + e) * cap + p] = F[a][e];
-      pf_out[(int64_t)L::JP * cap + p] = Jp;
+      PS::store_F(pf_out, cap, p, F);  // the stress already used F before rounding (mpm:567)
+      if (P.plastic) PS::store_Jp(pf_out, cap, p, Jp);
       float mass = P.p_mass;  // mpm_solver.py:571-573
       if (mat == kWater) mass *= P.water_density;
       // contribution to node o: w_o * (mass v + A (o - fx) dx) = w_o * (q + Adx o)
@@ -872,7 +864,7 @@ __global__ __launch_bounds__(kThreads) void k_grid_op(const Params* __restrict__
 // ---------------------------------------------------------------------------
 // G2P (mpm_solver.py:694-735)
 // ---------------------------------------------------------------------------
-template <int D>
+template <int D, bool Q = false>
 __global__ __launch_bounds__(kThreads) void k_g2p(const Params* __restrict__ scenes,
                                                   const int* __restrict__ slab,
                                                   float* __restrict__ pf,
@@ -930,14 +922,15 @@ __global__ __launch_bounds__(kThreads) void k_g2p(const Params* __restrict__ sce
     int dir = 13;  // block offset of the particle after advection, (dz+1)*9 + (dy+1)*3 + (dx+1)
     int key_new = ck.x;
     if (live) {
+      using PS = Particles<D, Q>;
       const int p = perm[j];
       const int mat = pi[(int64_t)L::MAT * cap + p];
-      float xp[3], fx[3], w[3][3];
+      float xp[3] = {0.0f, 0.0f, 0.0f}, fx[3], w[3][3];
       int l[3];
       bool inside = true;
+      PS::load_x(pf, cap, p, xp);
 #pragma unroll
       for (int a = 0; a < D; a++) {
-        xp[a] = pf[(int64_t)(L::X + a) * cap + p];
         int base;
         base_fx(xp[a], P.inv_dx, base, fx[a]);
         bspline(fx[a], w[a]);
@@ -994,21 +987,22 @@ __global__ __launch_bounds__(kThreads) void k_g2p(const Params* __restrict__ sce
       float xn[3] = {xp[0], xp[1], xp[2]};
       if (skip) {
       } else if (mat != kStationary) {  // mpm_solver.py:722-724
+        PS::store_v(pf_out, cap, p, nv);  // x advances by the stored v
 #pragma unroll
         for (int a = 0; a < D; a++) {
           xn[a] = xp[a] + P.dt * nv[a];
-          pf_out[(int64_t)(L::V + a) * cap + p] = nv[a];
-          pf_out[(int64_t)(L::X + a) * cap + p] = xn[a];
           vmax = fmaxf(vmax, fabsf(nv[a]));
-#pragma unroll
-          for (int e = 0; e < D; e++) pf_out[(int64_t)(L::C + a * D + e) * cap + p] = nC[a][e];
         }
+        PS::store_x(pf_out, cap, p, xn);  // xn: the stored x (binned next)
+        PS::store_C(pf_out, cap, p, nC);
       } else {
+        float vs[3];
+        PS::load_v(pf, cap, p, vs);
 #pragma unroll
         for (int a = 0; a < D; a++) {
-          const float va = pf[(int64_t)(L::V + a) * cap + p];
+          const float va = vs[a];
           vmax = fmaxf(vmax, fabsf(va));
-          if (pf_out != pf) {  // out of place: a stationary particle keeps x, v, C
+          if (!Q && pf_out != pf) {  // out of place: a stationary particle keeps x, v, C
             pf_out[(int64_t)(L::V + a) * cap + p] = va;
             pf_out[(int64_t)(L::X + a) * cap + p] = xp[a];
 #pragma unroll
@@ -1080,7 +1074,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p(const Params* __restrict__ sce
   }
   // out of place: particles outside every chunk (escaped the window; sorted
   // to the end) keep their whole state
-  if (pf_out != pf) {
+  if (!Q && pf_out != pf) {
     for (int j = totals[kTotValid] + blockIdx.x * kThreads + threadIdx.x; j < n;
          j += gridDim.x * kThreads) {
       const int p = perm[j];
@@ -1215,31 +1209,94 @@ __global__ __launch_bounds__(kThreads) void k_reorder(const float* __restrict__ 
   }
 }
 
-template <int D>
+// Packed layout: the word count depends on the mode, so it is a parameter.
+__global__ __launch_bounds__(kThreads) void k_reorder_words(const float* __restrict__ pf,
+                                                            float* __restrict__ pf2,
+                                                            const int* __restrict__ pi,
+                                                            int* __restrict__ pi2,
+                                                            const int* __restrict__ perm, int n,
+                                                            int64_t cap, int nw, int ni) {
+  for (int j = blockIdx.x * kThreads + threadIdx.x; j < n; j += gridDim.x * kThreads) {
+    const int p = perm[j];
+#pragma unroll 4
+    for (int c = 0; c < nw; c++) pf2[(int64_t)c * cap + j] = pf[(int64_t)c * cap + p];
+    for (int c = 0; c < ni; c++) pi2[(int64_t)c * cap + j] = pi[(int64_t)c * cap + p];
+  }
+}
+
+// New particles: seed_particle (mpm_solver.py:823-838).
+template <int D, bool Q, bool G>
 __global__ __launch_bounds__(kThreads) void k_append(const float* __restrict__ x,
                                                      const float* __restrict__ v,
                                                      const int* __restrict__ mat, int k,
-                                                     int n_old, int scene, float* __restrict__ pf,
+                                                     int n_old, int scene, int plastic,
+                                                     float* __restrict__ pf,
                                                      int* __restrict__ pi, int64_t cap) {
   using L = Layout<D>;
+  using PS = Particles<D, Q, G>;
   const int t = blockIdx.x * kThreads + threadIdx.x;
   if (t >= k) return;
   const int s = n_old + t;
   const int m = mat[t];
+  float xs[3] = {0.0f, 0.0f, 0.0f}, vs[3] = {0.0f, 0.0f, 0.0f}, C[3][3], F[3][3];
 #pragma unroll
   for (int a = 0; a < D; a++) {
-    pf[(int64_t)(L::X + a) * cap + s] = x[(int64_t)t * D + a];
-    pf[(int64_t)(L::V + a) * cap + s] = v ? v[(int64_t)t * D + a] : 0.0f;
+    xs[a] = x[(int64_t)t * D + a];
+    vs[a] = v ? v[(int64_t)t * D + a] : 0.0f;
 #pragma unroll
     for (int e = 0; e < D; e++) {
-      pf[(int64_t)(L::C + a * D + e) * cap + s] = 0.0f;
-      pf[(int64_t)(L::F + a * D + e) * cap + s] = a == e ? 1.0f : 0.0f;
+      C[a][e] = 0.0f;
+      F[a][e] = a == e ? 1.0f : 0.0f;
     }
   }
-  pf[(int64_t)L::JP * cap + s] = m == kSand ? 0.0f : 1.0f;  // mpm_solver.py:831-835
+  PS::store_x(pf, cap, s, xs);
+  PS::store_v(pf, cap, s, vs);
+  PS::store_C(pf, cap, s, C);
+  PS::store_F(pf, cap, s, F);
+  if (!Q || plastic) PS::store_Jp(pf, cap, s, m == kSand ? 0.0f : 1.0f);  // mpm_solver.py:831-835
   pi[(int64_t)L::MAT * cap + s] = m;
   pi[(int64_t)L::ORIG * cap + s] = s;
   pi[(int64_t)L::SCENE * cap + s] = scene;
+}
+
+template <int D, bool G>
+__global__ __launch_bounds__(kThreads) void k_unpack(const float* __restrict__ packed,
+                                                     float* __restrict__ flt, int n, int64_t cap,
+                                                     int plastic) {
+  using PQ = Particles<D, true, G>;
+  using PF = Particles<D, false>;
+  for (int p = blockIdx.x * kThreads + threadIdx.x; p < n; p += gridDim.x * kThreads) {
+    float x[3], v[3], C[3][3], F[3][3];
+    PQ::load_x(packed, cap, p, x);
+    PQ::load_v(packed, cap, p, v);
+    PQ::load_C(packed, cap, p, C);
+    PQ::load_F(packed, cap, p, F);
+    PF::store_x(flt, cap, p, x);
+    PF::store_v(flt, cap, p, v);
+    PF::store_C(flt, cap, p, C);
+    PF::store_F(flt, cap, p, F);
+    PF::store_Jp(flt, cap, p, plastic ? PQ::load_Jp(packed, cap, p) : 1.0f);
+  }
+}
+
+template <int D, bool G>
+__global__ __launch_bounds__(kThreads) void k_pack(const float* __restrict__ flt,
+                                                   float* __restrict__ packed, int n, int64_t cap,
+                                                   int plastic) {
+  using PQ = Particles<D, true, G>;
+  using PF = Particles<D, false>;
+  for (int p = blockIdx.x * kThreads + threadIdx.x; p < n; p += gridDim.x * kThreads) {
+    float x[3], v[3], C[3][3], F[3][3];
+    PF::load_x(flt, cap, p, x);
+    PF::load_v(flt, cap, p, v);
+    PF::load_C(flt, cap, p, C);
+    PF::load_F(flt, cap, p, F);
+    PQ::store_x(packed, cap, p, x);
+    PQ::store_v(packed, cap, p, v);
+    PQ::store_C(packed, cap, p, C);
+    PQ::store_F(packed, cap, p, F);
+    if (plastic) PQ::store_Jp(packed, cap, p, PF::load_Jp(flt, cap, p));
+  }
 }
 
 template <typename T>
@@ -1312,8 +1369,12 @@ static void launch_binning(const Params* P, SubstepBuffers& B, hipStream_t st, T
     // scratch, then a stable radix sort of the slots
     TimerScope ts(tm, "k_keys", st);
     launch_fill_u32((unsigned*)B.count, 0u, B.total_blocks + 1, st);
-    k_keys<D><<<clamp_groups(div_up(n, kThreads), 8192), kThreads, 0, st>>>(
-        P, B.pf, B.pi, B.cap, n, B.total_blocks, B.keys, B.count, B.err);
+    if (B.quant)
+      k_keys<D, true><<<clamp_groups(div_up(n, kThreads), 8192), kThreads, 0, st>>>(
+          P, B.pf, B.pi, B.cap, n, B.total_blocks, B.keys, B.count, B.err);
+    else
+      k_keys<D, false><<<clamp_groups(div_up(n, kThreads), 8192), kThreads, 0, st>>>(
+          P, B.pf, B.pi, B.cap, n, B.total_blocks, B.keys, B.count, B.err);
   }
   {
     TimerScope ts(tm, "k_bscan", st);
@@ -1380,7 +1441,8 @@ static void launch_g2p2g(const Params* P, const int* slab, SubstepBuffers& B, in
   const int chunk_bound = std::min(B.total_blocks, n) + div_up(n, B.chunk);
   {
     TimerScope ts(tm, "k_g2p2g", st);
-    k_g2p2g<D><<<clamp_groups(chunk_bound, G), kThreads, 0, st>>>(
+    auto kern = B.quant ? k_g2p2g<D, true> : k_g2p2g<D, false>;
+    kern<<<clamp_groups(chunk_bound, G), kThreads, 0, st>>>(
         P, slab, B.pf, B.pi, B.cap, B.perm, B.chunks, B.totals, B.grid_in, B.tiles, B.n_final,
         B.maxv, B.step, B.persistent_groups, B.code, B.nkey, B.dcount, B.count_next, B.err);
   }
@@ -1403,7 +1465,8 @@ void launch_transfer(const Params* P, const int* slab, SubstepBuffers& B, int ra
   const int chunk_bound = std::min(B.total_blocks, n) + div_up(n, B.chunk);
   {
     TimerScope ts(tm, "k_p2g", st);
-    k_p2g<D, 0><<<clamp_groups(chunk_bound, G), kThreads, 0, st>>>(
+    auto kern = B.quant ? k_p2g<D, 0, true> : k_p2g<D, 0, false>;
+    kern<<<clamp_groups(chunk_bound, G), kThreads, 0, st>>>(
         P, slab, B.pf, B.pi, B.cap, B.perm, B.chunks, B.totals, B.tiles, nullptr, B.pf_out);
   }
   {
@@ -1425,7 +1488,8 @@ void launch_g2p(const Params* P, const int* slab, SubstepBuffers& B, int s_index
   const int chunk_bound = std::min(B.total_blocks, n) + div_up(n, B.chunk);
   {
     TimerScope ts(tm, "k_g2p", st);
-    k_g2p<D><<<clamp_groups(chunk_bound, G), kThreads, 0, st>>>(
+    auto kern = B.quant ? k_g2p<D, true> : k_g2p<D, false>;
+    kern<<<clamp_groups(chunk_bound, G), kThreads, 0, st>>>(
         P, slab, B.pf, B.pi, B.cap, B.perm, B.chunks, B.totals, B.grid,
         B.maxv, B.step, B.persistent_groups, B.code, B.nkey, B.dcount, B.count_next, B.err,
         B.pf_out, n);
@@ -1490,16 +1554,50 @@ void launch_adjoint(const Params* P, const int* slab, SubstepBuffers& B, const f
 template <int D>
 void launch_reorder(SubstepBuffers& B, float* pf_alt, int* pi_alt, hipStream_t st, Timer* tm) {
   TimerScope ts(tm, "k_reorder", st);
-  k_reorder<D><<<clamp_groups(div_up(B.n, kThreads), 8192), kThreads, 0, st>>>(
-      B.pf, pf_alt, B.pi, pi_alt, B.perm, B.n, B.cap);
+  if (B.quant)
+    k_reorder_words<<<clamp_groups(div_up(B.n, kThreads), 8192), kThreads, 0, st>>>(
+        B.pf, pf_alt, B.pi, pi_alt, B.perm, B.n, B.cap, B.nw, Layout<D>::NI);
+  else
+    k_reorder<D><<<clamp_groups(div_up(B.n, kThreads), 8192), kThreads, 0, st>>>(
+        B.pf, pf_alt, B.pi, pi_alt, B.perm, B.n, B.cap);
 }
 
 template <int D>
 void launch_append(SubstepBuffers& B, const float* x, const float* v, const int* mat, int k,
-                   int n_old, int scene, hipStream_t st) {
+                   int n_old, int scene, int plastic, hipStream_t st) {
   if (k <= 0) return;
-  k_append<D><<<div_up(k, kThreads), kThreads, 0, st>>>(x, v, mat, k, n_old, scene, B.pf, B.pi,
-                                                        B.cap);
+  const int g = div_up(k, kThreads);
+  if (!B.quant)
+    k_append<D, false, false><<<g, kThreads, 0, st>>>(x, v, mat, k, n_old, scene, plastic, B.pf,
+                                                       B.pi, B.cap);
+  else if (B.g2p2g)
+    k_append<D, true, true><<<g, kThreads, 0, st>>>(x, v, mat, k, n_old, scene, plastic, B.pf,
+                                                     B.pi, B.cap);
+  else
+    k_append<D, true, false><<<g, kThreads, 0, st>>>(x, v, mat, k, n_old, scene, plastic, B.pf,
+                                                      B.pi, B.cap);
+}
+
+template <int D>
+void launch_unpack(const float* packed, float* flt, int n, int64_t cap, bool g2p2g, bool plastic,
+                   hipStream_t st) {
+  if (n <= 0) return;
+  const int g = clamp_groups(div_up(n, kThreads), 8192);
+  if (g2p2g)
+    k_unpack<D, true><<<g, kThreads, 0, st>>>(packed, flt, n, cap, plastic);
+  else
+    k_unpack<D, false><<<g, kThreads, 0, st>>>(packed, flt, n, cap, plastic);
+}
+
+template <int D>
+void launch_pack(const float* flt, float* packed, int n, int64_t cap, bool g2p2g, bool plastic,
+                 hipStream_t st) {
+  if (n <= 0) return;
+  const int g = clamp_groups(div_up(n, kThreads), 8192);
+  if (g2p2g)
+    k_pack<D, true><<<g, kThreads, 0, st>>>(flt, packed, n, cap, plastic);
+  else
+    k_pack<D, false><<<g, kThreads, 0, st>>>(flt, packed, n, cap, plastic);
 }
 
 void launch_gather_f(const float* src, int64_t cap, int ncomp, const int* orig, int n,
@@ -1559,8 +1657,12 @@ template void launch_substep<3>(const Params*, const int*, SubstepBuffers&, int,
 template void launch_reorder<2>(SubstepBuffers&, float*, int*, hipStream_t, Timer*);
 template void launch_reorder<3>(SubstepBuffers&, float*, int*, hipStream_t, Timer*);
 template void launch_append<2>(SubstepBuffers&, const float*, const float*, const int*, int, int,
-                               int, hipStream_t);
+                               int, int, hipStream_t);
 template void launch_append<3>(SubstepBuffers&, const float*, const float*, const int*, int, int,
-                               int, hipStream_t);
+                               int, int, hipStream_t);
+template void launch_unpack<2>(const float*, float*, int, int64_t, bool, bool, hipStream_t);
+template void launch_unpack<3>(const float*, float*, int, int64_t, bool, bool, hipStream_t);
+template void launch_pack<2>(const float*, float*, int, int64_t, bool, bool, hipStream_t);
+template void launch_pack<3>(const float*, float*, int, int64_t, bool, bool, hipStream_t);
 
 }  // namespace mpm

@@ -197,7 +197,17 @@ struct mpm_solver {
   const float* diff_youngs = nullptr;
   double diff_nu = 0.0;
 
-  int nf() const { return D == 3 ? Layout<3>::NF : Layout<2>::NF; }
+  // particle storage: quant = 1 packs x, v and F (PackedLayout); plastic = 0
+  // is support_plasticity=False (no Jp in the packed layout)
+  int quant = 0;
+  int plastic = 1;
+
+  // 32-bit words per particle slot in the float buffers
+  int nf() const {
+    if (!quant) return D == 3 ? Layout<3>::NF : Layout<2>::NF;
+    return D == 3 ? PackedLayout<3>::words(g2p2g != 0, plastic != 0)
+                  : PackedLayout<2>::words(g2p2g != 0, plastic != 0);
+  }
 };
 
 namespace {
@@ -275,6 +285,7 @@ void fill_params(Params& P, const mpm_solver* s, const SceneHost& sc, float dt, 
   P.youngs = nullptr;
   P.kmu = P.kla = P.dkmu = P.dkla = 0.0f;
   P.g2p2g = s->g2p2g;
+  P.plastic = s->plastic;
   P.v_allowed = s->g2p2g && s->g2p2g_clamp && s->g2p2g_cfl > 0
                     ? (float)(c.dx * s->g2p2g_cfl) / dt
                     : 0.0f;
@@ -303,11 +314,26 @@ int free_particle_buffers(mpm_solver* s) {
 
 int alloc_chunk_buffers(mpm_solver* s);
 
+
 // Captured graphs hold buffer addresses: drop them when buffers move.
 void drop_graphs(mpm_solver* s) {
   if (!s->graphs.empty()) hipStreamSynchronize(s->stream);
   for (auto& g : s->graphs) hipGraphExecDestroy(g.exec);
   s->graphs.clear();
+}
+
+// The particle layout is about to change: only without particles; the
+// buffers are reallocated by the next grow().
+int drop_particle_layout(mpm_solver* s) {
+  if (s->n > 0) return fail(MPM_EINVAL, "the particle storage can only change before particles are added");
+  if (s->cap > 0) {
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    drop_graphs(s);
+    free_particle_buffers(s);
+    s->cap = 0;
+    s->have_perm = false;
+  }
+  return MPM_OK;
 }
 
 int grow(mpm_solver* s, int64_t need) {
@@ -400,6 +426,8 @@ void fill_buffers(mpm_solver* s, SubstepBuffers& B) {
   B.full_sort = !s->have_perm;
   B.perm_identity = s->reordered;
   B.g2p2g = s->g2p2g != 0;
+  B.quant = s->quant != 0;
+  B.nw = s->nf();
   B.grid_in = s->g2p2g ? s->grid : nullptr;
   B.n_final = (int)s->n_final;
   B.keys_sorted = nullptr;
@@ -539,6 +567,19 @@ struct DeviceArray {
   ~DeviceArray() { dfree(p); }
   int alloc(int64_t count) { return dalloc(&p, count); }
 };
+
+// Float-layout copy (Layout<D>, same cap) of every slot of a solver with
+// packed particle storage, on the solver's stream.
+int unpacked(mpm_solver* s, DeviceArray<float>& flt) {
+  const int nf = s->D == 3 ? Layout<3>::NF : Layout<2>::NF;
+  TRY(flt.alloc((int64_t)nf * s->cap));
+  if (s->D == 3)
+    launch_unpack<3>(s->pf, flt.p, (int)s->n, s->cap, s->g2p2g != 0, s->plastic != 0, s->stream);
+  else
+    launch_unpack<2>(s->pf, flt.p, (int)s->n, s->cap, s->g2p2g != 0, s->plastic != 0, s->stream);
+  HIP_TRY(hipGetLastError());
+  return MPM_OK;
+}
 
 // Voxelizer.__init__ (engine/voxelizer.py:18-38): every axis of the solver
 // resolution rounded up to a power of two, times super_sample; voxel size
@@ -825,9 +866,9 @@ int mpm_add_particles(mpm_solver* s, int64_t k, const float* x, const float* v,
   SubstepBuffers B;
   fill_buffers(s, B);
   if (D == 3)
-    launch_append<3>(B, dx, dv, dm, (int)k, (int)s->n, s->active_scene, s->stream);
+    launch_append<3>(B, dx, dv, dm, (int)k, (int)s->n, s->active_scene, s->plastic, s->stream);
   else
-    launch_append<2>(B, dx, dv, dm, (int)k, (int)s->n, s->active_scene, s->stream);
+    launch_append<2>(B, dx, dv, dm, (int)k, (int)s->n, s->active_scene, s->plastic, s->stream);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(s->stream));
   dfree(dx);
@@ -946,6 +987,12 @@ int mpm_get_state(mpm_solver* s, int32_t field, void* host) {
   TRY(field_info(s, field, &c0, &nc, &is_int));
   HIP_TRY(hipSetDevice(s->device));
   if (s->n == 0) return MPM_OK;
+  DeviceArray<float> flt;  // packed storage: the float layout of every slot
+  const float* src = s->pf;
+  if (s->quant && !is_int) {
+    TRY(unpacked(s, flt));
+    src = flt.p;
+  }
   void* tmp = nullptr;
   HIP_TRY(hipMalloc(&tmp, sizeof(float) * s->n * nc));
   const int* orig = s->pi + s->cap;  // Layout ORIG == 1
@@ -953,7 +1000,7 @@ int mpm_get_state(mpm_solver* s, int32_t field, void* host) {
     launch_gather_i(s->pi + (int64_t)c0 * s->cap, s->cap, nc, orig, (int)s->n, (int*)tmp,
                     s->stream);
   else
-    launch_gather_f(s->pf + (int64_t)c0 * s->cap, s->cap, nc, orig, (int)s->n, (float*)tmp,
+    launch_gather_f(src + (int64_t)c0 * s->cap, s->cap, nc, orig, (int)s->n, (float*)tmp,
                     s->stream);
   int r = check_device_errors(s);
   hipError_t e = hipMemcpy(host, tmp, sizeof(float) * s->n * nc, hipMemcpyDeviceToHost);
@@ -983,13 +1030,28 @@ int mpm_set_state(mpm_solver* s, int32_t field, const void* host) {
   HIP_TRY(hipMemcpyAsync(tmp, host, sizeof(float) * s->n * nc, hipMemcpyHostToDevice, s->stream));
   if (field == MPM_FIELD_X) s->have_perm = false;  // block keys must be recomputed
   const int* orig = s->pi + s->cap;
-  if (is_int)
+  int r = MPM_OK;
+  if (is_int) {
     launch_scatter_i((const int*)tmp, s->cap, nc, orig, (int)s->n,
                      s->pi + (int64_t)c0 * s->cap, s->stream);
-  else
+  } else if (s->quant) {  // unpack, replace the field, pack (the others round-trip exactly)
+    DeviceArray<float> flt;
+    r = unpacked(s, flt);
+    if (r == MPM_OK) {
+      launch_scatter_f((const float*)tmp, s->cap, nc, orig, (int)s->n,
+                       flt.p + (int64_t)c0 * s->cap, s->stream);
+      if (s->D == 3)
+        launch_pack<3>(flt.p, s->pf, (int)s->n, s->cap, s->g2p2g != 0, s->plastic != 0, s->stream);
+      else
+        launch_pack<2>(flt.p, s->pf, (int)s->n, s->cap, s->g2p2g != 0, s->plastic != 0, s->stream);
+      hipStreamSynchronize(s->stream);
+    }
+  } else {
     launch_scatter_f((const float*)tmp, s->cap, nc, orig, (int)s->n,
                      s->pf + (int64_t)c0 * s->cap, s->stream);
-  int r = check_device_errors(s);
+  }
+  if (r == MPM_OK) r = check_device_errors(s);
+  hipStreamSynchronize(s->stream);
   hipFree(tmp);
   return r;
 }
@@ -1154,6 +1216,8 @@ int mpm_diff_setup(mpm_solver* s, int64_t n, const int32_t* material) {
     if (op.kind != kPostBBox)
       return fail(MPM_EINVAL, "differentiable mode supports the bounding box only");
   if (s->g2p2g) return fail(MPM_EINVAL, "differentiable mode uses the p2g / g2p transfers");
+  if (s->quant || !s->plastic)
+    return fail(MPM_EINVAL, "differentiable mode uses the float particle layout with plasticity");
   HIP_TRY(hipSetDevice(s->device));
   HIP_TRY(hipStreamSynchronize(s->stream));
   TRY(grow(s, n));
@@ -1249,6 +1313,7 @@ int mpm_set_g2p2g(mpm_solver* s, int32_t enabled, double allowed_cfl, int32_t cl
   HIP_TRY(hipSetDevice(s->device));
   HIP_TRY(hipStreamSynchronize(s->stream));
   const int want = enabled ? 1 : 0;
+  if (want != s->g2p2g && s->quant) TRY(drop_particle_layout(s));  // C present or not
   s->g2p2g_cfl = allowed_cfl;
   s->g2p2g_clamp = clamp ? 1 : 0;
   if (want != s->g2p2g) {
@@ -1264,6 +1329,25 @@ int mpm_set_g2p2g(mpm_solver* s, int32_t enabled, double allowed_cfl, int32_t cl
     if (s->cap > 0) TRY(alloc_chunk_buffers(s));  // tile size depends on the mode
     s->n_final = 0;
   }
+  return MPM_OK;
+}
+
+int mpm_set_storage(mpm_solver* s, int32_t quant, int32_t support_plasticity) {
+  if (!s) return fail(MPM_EINVAL, "null argument");
+  if (s->diff_n != 0) return fail(MPM_EINVAL, "the differentiable mode keeps the float layout");
+  const int q = quant ? 1 : 0, pl = support_plasticity ? 1 : 0;
+  if (q == s->quant && pl == s->plastic) return MPM_OK;
+  HIP_TRY(hipSetDevice(s->device));
+  TRY(drop_particle_layout(s));
+  s->quant = q;
+  s->plastic = pl;
+  return MPM_OK;
+}
+
+int mpm_particle_bytes(mpm_solver* s, int32_t* bytes) {
+  if (!s || !bytes) return fail(MPM_EINVAL, "null argument");
+  const int ni = s->D == 3 ? Layout<3>::NI : Layout<2>::NI;
+  *bytes = 4 * (s->nf() + ni) + 4;  // device words + the host-side colour
   return MPM_OK;
 }
 
@@ -1424,7 +1508,18 @@ int mpm_quantize_particles(mpm_solver* s, float* ranges, uint32_t* x_and_v, uint
     return fail(MPM_ENOMEM, "device allocation for the quantised particles failed");
   }
   std::vector<float> part((size_t)groups * NC * 2);
-  launch_minmax(D, s->pf, s->cap, n, partial, groups, s->stream);
+  DeviceArray<float> flt;  // packed storage: read x and v from the float layout
+  const float* src = s->pf;
+  if (s->quant) {
+    const int r = unpacked(s, flt);
+    if (r != MPM_OK) {
+      hipFree(partial);
+      hipFree(out);
+      return r;
+    }
+    src = flt.p;
+  }
+  launch_minmax(D, src, s->cap, n, partial, groups, s->stream);
   hipError_t e = hipMemcpyAsync(part.data(), partial, sizeof(float) * part.size(),
                                 hipMemcpyDeviceToHost, s->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
@@ -1447,7 +1542,7 @@ int mpm_quantize_particles(mpm_solver* s, float* ranges, uint32_t* x_and_v, uint
       const float width = hi - lo;  // float32 subtraction
       q.inv[c] = (float)(1.0 / (double)width);  // correctly rounded 1 / width
     }
-    launch_quantize(D, s->pf, s->pi + s->cap /* Layout ORIG == 1 */, s->cap, n, q, out, s->stream);
+    launch_quantize(D, src, s->pi + s->cap /* Layout ORIG == 1 */, s->cap, n, q, out, s->stream);
     e = hipMemcpyAsync(x_and_v, out, sizeof(unsigned) * s->n * D, hipMemcpyDeviceToHost,
                        s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
@@ -1556,7 +1651,8 @@ int mpm_add_mesh(mpm_solver* s, int64_t ntri, const double* triangles, int32_t m
   launch_seed_attrs(total, velocity ? velocity : zero, material, dv.p, dm.p, s->stream);
   SubstepBuffers B;
   fill_buffers(s, B);
-  launch_append<3>(B, dx.p, dv.p, dm.p, (int)total, (int)s->n, s->active_scene, s->stream);
+  launch_append<3>(B, dx.p, dv.p, dm.p, (int)total, (int)s->n, s->active_scene, s->plastic,
+                   s->stream);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(s->stream));
   s->color.insert(s->color.end(), (size_t)total, color);

@@ -140,12 +140,9 @@ class MPMSolver:
                  kernel_profiler=False):
         self.dim = len(res)
         assert self.dim in (2, 3), "MPM solver supports only 2D and 3D simulations."
-        for flag, name in ((quant, "quant"), (unbounded, "unbounded"),
-                           (use_adaptive_dt, "use_adaptive_dt")):
+        for flag, name in ((unbounded, "unbounded"), (use_adaptive_dt, "use_adaptive_dt")):
             if flag:
                 raise NotImplementedError(f"{name}=True is not supported by this engine")
-        if not support_plasticity:
-            raise NotImplementedError("support_plasticity=False is not supported by this engine")
         self.quant = quant
         self.use_g2p2g = use_g2p2g
         self.v_clamp_g2p2g = v_clamp_g2p2g
@@ -240,6 +237,8 @@ class MPMSolver:
         h = _lib._P()
         check(L.mpm_create(cfg, h))
         self._handle = h
+        if self.quant or not self.support_plasticity:  # mpm_solver.py:106-136, 216-262
+            check(L.mpm_set_storage(h, int(bool(self.quant)), int(bool(self.support_plasticity))))
         if self.use_g2p2g:
             check(L.mpm_set_g2p2g(h, 1, float(self.g2p2g_allowed_cfl), int(bool(self.v_clamp_g2p2g))))
         if self._kernel_profiler:
@@ -635,6 +634,12 @@ class MPMSolver:
 
     def set_tuning(self, chunk=0, groups=0):
         check(_lib.lib().mpm_set_tuning(self._h(), int(chunk), int(groups)))
+
+    def particle_bytes(self):
+        """Bytes per particle in this storage mode (device state + colour)."""
+        b = _lib.ctypes.c_int32()
+        check(_lib.lib().mpm_particle_bytes(self._h(), b))
+        return int(b.value)
 
     def set_graphs(self, enabled):
         """Replay steady-state substep pairs as HIP graphs (default on)."""

@@ -63,6 +63,8 @@ def oracle_cfg_for(O, solver, post=None, gravity=None):
     cfg = O.make_cfg(tuple(solver.res), gravity=gravity, post=post, grid_lo=lo, grid_n=n, **kw)
     for name in ("dx", "inv_dx", "p_vol", "p_mass", "mu_0", "lambda_0", "alpha"):
         setattr(cfg, name, getattr(solver, name))
+    cfg.quant = 1 if getattr(solver, "quant", False) else 0
+    cfg.no_plasticity = 0 if getattr(solver, "support_plasticity", True) else 1
     return cfg
 
 

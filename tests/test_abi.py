@@ -44,8 +44,7 @@ def test_config_struct_layout_matches_header():
 
 def test_unsupported_options_fail_before_touching_the_gpu():
     from taichi_elements_amd import MPMSolver
-    for kw in (dict(quant=True), dict(unbounded=True),
-               dict(use_adaptive_dt=True), dict(support_plasticity=False)):
+    for kw in (dict(unbounded=True), dict(use_adaptive_dt=True)):
         with pytest.raises(NotImplementedError):
             MPMSolver(res=(16, 16, 16), **kw)
     with pytest.raises(AssertionError):
