@@ -44,8 +44,8 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(
   constexpr int WAVES = kThreads / 64;
   constexpr int SUBS = WAVES * GROUPS;
   constexpr int kBatch = 2;
-  __shared__ float4 tin[TN];
-  __shared__ float4 stage[kChunkMax][4];
+  __shared__ float tin_x[TN], tin_y[TN], tin_z[TN];  // input grid velocity (as k_g2p)
+  __shared__ float4 stage[4][kChunkMax];  // [record part][particle]: lane-contiguous
   __shared__ float4 ptile[SUBS][TN];  // inner particles: nodes 1..6 of the output tile
   __shared__ float4 stile[TN8];       // particles that crossed a cell face (LDS atomics)
   __shared__ int wcnt[WAVES][CELLS];
@@ -83,7 +83,10 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(
       }
       const int gb = encode_local<D>(q, P);
       const int node = D == 3 ? ln[0] + 4 * (ln[1] + 4 * ln[2]) : ln[0] + 4 * ln[1];
-      tin[i] = grid_in[(int64_t)gb * NPB + node];
+      const float4 g = grid_in[(int64_t)gb * NPB + node];
+      tin_x[i] = g.x;
+      tin_y[i] = g.y;
+      tin_z[i] = g.z;
     }
     __syncthreads();
 
@@ -124,8 +127,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(
             for (int ox = 0; ox < 3; ox++) {
               const int node = D == 3 ? (l[0] + ox) + kTile * ((l[1] + oy) + kTile * (l[2] + oz))
                                       : (l[0] + ox) + kTile * (l[1] + oy);
-              const float4 g4 = tin[node];
-              const float gg[3] = {g4.x, g4.y, g4.z};
+              const float gg[3] = {tin_x[node], tin_y[node], D == 3 ? tin_z[node] : 0.0f};
 #pragma unroll
               for (int cc = 0; cc < D; cc++) {
                 a0[cc] += w[0][ox] * gg[cc];
@@ -218,14 +220,14 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(
         }
       }
       if (D == 3) {
-        stage[t][0] = make_float4(Ad[0][0], Ad[0][1], Ad[0][2], Ad[1][0]);
-        stage[t][1] = make_float4(Ad[1][1], Ad[1][2], Ad[2][0], Ad[2][1]);
-        stage[t][2] = make_float4(Ad[2][2], q[0], q[1], q[2]);
-        stage[t][3] = make_float4(fn[0], fn[1], fn[2], fits ? mass : 0.0f);
+        stage[0][t] = make_float4(Ad[0][0], Ad[0][1], Ad[0][2], Ad[1][0]);
+        stage[1][t] = make_float4(Ad[1][1], Ad[1][2], Ad[2][0], Ad[2][1]);
+        stage[2][t] = make_float4(Ad[2][2], q[0], q[1], q[2]);
+        stage[3][t] = make_float4(fn[0], fn[1], fn[2], fits ? mass : 0.0f);
       } else {
-        stage[t][0] = make_float4(Ad[0][0], Ad[0][1], Ad[1][0], Ad[1][1]);
-        stage[t][1] = make_float4(q[0], q[1], fn[0], fn[1]);
-        stage[t][2] = make_float4(fits ? mass : 0.0f, 0.0f, 0.0f, 0.0f);
+        stage[0][t] = make_float4(Ad[0][0], Ad[0][1], Ad[1][0], Ad[1][1]);
+        stage[1][t] = make_float4(q[0], q[1], fn[0], fn[1]);
+        stage[2][t] = make_float4(fits ? mass : 0.0f, 0.0f, 0.0f, 0.0f);
       }
       // block of the new position for the next substep's binning
       int dd[3] = {0, 0, 0}, qb[3] = {0, 0, 0};
@@ -298,10 +300,10 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(
           float4 s0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), s1 = s0, s2 = s0, s3 = s0;
           if (k < ccnt) {
             const int pt = order[cbeg + k];
-            s0 = stage[pt][0];
-            s1 = stage[pt][1];
-            s2 = stage[pt][2];
-            s3 = stage[pt][3];
+            s0 = stage[0][pt];
+            s1 = stage[1][pt];
+            s2 = stage[2][pt];
+            s3 = stage[3][pt];
           }
           float f3[3];
           if (D == 3) {
@@ -364,7 +366,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(
     // ---- phase 3b: particles that crossed into a neighbour's cells ----
     if (live && shell >= 0) {
       float Ad[3][3], q[3], mass, w[3][3], f3[3];
-      const float4 s0 = stage[t][0], s1 = stage[t][1], s2 = stage[t][2], s3 = stage[t][3];
+      const float4 s0 = stage[0][t], s1 = stage[1][t], s2 = stage[2][t], s3 = stage[3][t];
       if (D == 3) {
         Ad[0][0] = s0.x; Ad[0][1] = s0.y; Ad[0][2] = s0.z;
         Ad[1][0] = s0.w; Ad[1][1] = s1.x; Ad[1][2] = s1.y;
