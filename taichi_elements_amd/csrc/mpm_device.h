@@ -75,6 +75,19 @@ struct Params {
   PostOp post[kMaxPost];
 };
 
+// Workgroup barrier for hand-offs through LDS only: waits for this wave's
+// LDS operations, not for its global stores.  __syncthreads() also waits for
+// every outstanding global store of the wave (a workgroup-scope release),
+// which puts the HBM write latency of the particle and tile stores on the
+// critical path of kernels whose stores no other wave of the workgroup reads.
+// The fences are workgroup-scope and restricted to LDS ("local"), so the
+// compiler orders LDS accesses across the barrier and waits for LDS only.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // base = floor(x * inv_dx - 0.5), fx = x * inv_dx - base (mpm_solver.py:497,
 // 503).  Contraction is disabled so the binning pass and both transfers see
 // bit-identical bases.
@@ -84,6 +97,43 @@ __device__ __forceinline__ void base_fx(float xa, float inv_dx, int& base, float
   float u = t - 0.5f;
   base = (int)floorf(u);
   fx = t - (float)base;
+}
+
+// Contributions w_o (q + Ad o, mass) of one particle to the three nodes
+// o = (0..2, oy, oz) of one x line of its stencil (p2g, mpm_solver.py:
+// 575-584, with the affine term folded as q = mass v - Ad fx, Ad = A dx),
+// accumulated into c[0..2].  The tensor-product weight is factored: with
+// wyz = wy[oy] wz[oz], P = wyz (q + Ad (0, oy, oz)) and Q = wyz Ad e_x, node
+// ox receives wx[ox] (P + ox Q) -- a third of the arithmetic of evaluating
+// every node on its own.  oy and oz are compile-time after unrolling.
+template <int D>
+__device__ __forceinline__ void line_contrib(int oy, int oz, const float Ad[3][3],
+                                             const float q[3], float mass, const float w[3][3],
+                                             float4 c[3]) {
+  const float wyz = D == 3 ? w[1][oy] * w[2][oz] : w[1][oy];
+  float r1[3], r2[3], r0[3];
+#pragma unroll
+  for (int a = 0; a < D; a++) {
+    float b = q[a];
+    if (oy == 1) b += Ad[a][1];
+    if (oy == 2) b += 2.0f * Ad[a][1];
+    if (D == 3 && oz == 1) b += Ad[a][2];
+    if (D == 3 && oz == 2) b += 2.0f * Ad[a][2];
+    const float P = wyz * b, Q = wyz * Ad[a][0];
+    r0[a] = P;
+    r1[a] = P + Q;
+    r2[a] = r1[a] + Q;
+  }
+  const float M = wyz * mass;
+  const float* r[3] = {r0, r1, r2};
+#pragma unroll
+  for (int ox = 0; ox < 3; ox++) {
+    const float wx = w[0][ox];
+    c[ox].x += wx * r[ox][0];
+    c[ox].y += wx * r[ox][1];
+    if (D == 3) c[ox].z += wx * r[ox][2];
+    c[ox].w += wx * M;
+  }
 }
 
 // Quadratic B-spline weights, mpm_solver.py:505.

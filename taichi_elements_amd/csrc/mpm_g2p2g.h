@@ -330,25 +330,15 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(
 #pragma unroll
         for (int oz = 0; oz < (D == 3 ? 3 : 1); oz++)
 #pragma unroll
-          for (int oy = 0; oy < 3; oy++)
+          for (int oy = 0; oy < 3; oy++) {
+            float4 line[3];
+#pragma unroll
+            for (int ox = 0; ox < 3; ox++) line[ox] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+            for (int bi = 0; bi < kBatch; bi++) line_contrib<D>(oy, oz, Ad[bi], q[bi], mass[bi], w[bi], line);
 #pragma unroll
             for (int ox = 0; ox < 3; ox++) {
-              float4 add = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-#pragma unroll
-              for (int bi = 0; bi < kBatch; bi++) {
-                float weight = w[bi][0][ox] * w[bi][1][oy];
-                if (D == 3) weight *= w[bi][2][oz];
-                const float fo[3] = {(float)ox, (float)oy, (float)oz};
-                float m3[3];
-#pragma unroll
-                for (int a = 0; a < 3; a++)
-                  m3[a] = q[bi][a] + fo[0] * Ad[bi][a][0] + fo[1] * Ad[bi][a][1] +
-                          fo[2] * Ad[bi][a][2];
-                add.x += weight * m3[0];
-                add.y += weight * m3[1];
-                if (D == 3) add.z += weight * m3[2];
-                add.w += weight * mass[bi];
-              }
+              const float4 add = line[ox];
               const int node = D == 3 ? (lc[0] + ox) + kTile * ((lc[1] + oy) + kTile * (lc[2] + oz))
                                       : (lc[0] + ox) + kTile * (lc[1] + oy);
               float4 acc = tp[node];
@@ -359,6 +349,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(
               tp[node] = acc;
               __asm__ volatile("" ::: "memory");  // keep the LDS RMWs in program order
             }
+          }
       }
     }
     __syncthreads();

@@ -604,7 +604,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
         stage[2][t] = make_float4(mass, 0.0f, 0.0f, 0.0f);
       }
     }
-    __syncthreads();
+    lds_barrier();
 
     // ---- phase 2: stable counting sort of the chunk by cell ----
     unsigned long long peers = __ballot(valid);
@@ -616,7 +616,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     }
     const int rank = (int)__popcll(peers & lt);
     if (valid && rank == 0) wcnt[wave][cell] = (int)__popcll(peers);
-    __syncthreads();
+    lds_barrier();
     if (threadIdx.x < 64) {  // one wave: per-cell totals and exclusive scan
       int run = 0;
       if (lane < CELLS) {
@@ -638,9 +638,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
         cstart[lane] = inc - run;
       }
     }
-    __syncthreads();
+    lds_barrier();
     if (valid) order[cstart[cell] + wcnt[wave][cell] + rank] = (unsigned char)t;
-    __syncthreads();
+    lds_barrier();
 
     // ---- phase 3: cell-major scatter into private tiles ----
     {
@@ -689,25 +689,15 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
 #pragma unroll
         for (int oz = 0; oz < (D == 3 ? 3 : 1); oz++)
 #pragma unroll
-          for (int oy = 0; oy < 3; oy++)
+          for (int oy = 0; oy < 3; oy++) {
+            float4 line[3];
+#pragma unroll
+            for (int ox = 0; ox < 3; ox++) line[ox] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+            for (int bi = 0; bi < kBatch; bi++) line_contrib<D>(oy, oz, Ad[bi], q[bi], mass[bi], w[bi], line);
 #pragma unroll
             for (int ox = 0; ox < 3; ox++) {
-              float4 add = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-#pragma unroll
-              for (int bi = 0; bi < kBatch; bi++) {
-                float weight = w[bi][0][ox] * w[bi][1][oy];
-                if (D == 3) weight *= w[bi][2][oz];
-                const float fo[3] = {(float)ox, (float)oy, (float)oz};
-                float m3[3];
-#pragma unroll
-                for (int a = 0; a < 3; a++)
-                  m3[a] = q[bi][a] + fo[0] * Ad[bi][a][0] + fo[1] * Ad[bi][a][1] +
-                          fo[2] * Ad[bi][a][2];
-                add.x += weight * m3[0];
-                add.y += weight * m3[1];
-                if (D == 3) add.z += weight * m3[2];
-                add.w += weight * mass[bi];
-              }
+              const float4 add = line[ox];
               const int node = D == 3 ? (lc[0] + ox) + kTile * ((lc[1] + oy) + kTile * (lc[2] + oz))
                                       : (lc[0] + ox) + kTile * (lc[1] + oy);
               float4 acc = tp[node];
@@ -721,9 +711,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
               // every LDS access in program order (LDS serves a wave in order).
               __asm__ volatile("" ::: "memory");
             }
+          }
       }
     }
-    __syncthreads();
+    lds_barrier();
 
     // ---- phase 4: sum the private tiles, write the node tile ----
     float4* out = tiles + (int64_t)c * TN;
@@ -739,7 +730,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
       }
       out[i] = acc;
     }
-    __syncthreads();
+    lds_barrier();
   }
 }
 
@@ -780,44 +771,57 @@ __global__ __launch_bounds__(kThreads) void k_grid_op(const Params* __restrict__
                                                       float4* __restrict__ grid, int raw,
                                                       float4* __restrict__ gbar) {
   constexpr int NPB = Geo<D>::kNodesPerBlock, TN = Geo<D>::kTileNodes;
-  const int64_t total = (int64_t)totals[kTotActive] * NPB;
+  // SRC lanes per node, one per particle block b - d (d in {0,1}^D) whose
+  // tiles reach the node; the partial sums are combined with a butterfly of
+  // lane exchanges (every lane of the group ends with the same, bitwise
+  // reproducible total).  A lane per source block gives this latency-bound
+  // kernel SRC times the loads in flight of a lane per node.
+  constexpr int SRC = D == 3 ? 8 : 4;
+  const int64_t total = (int64_t)totals[kTotActive] * NPB * SRC;
   for (int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * kThreads) {
-    const int blk = (int)(t / NPB), node = (int)(t % NPB);
+    // groups of SRC lanes are aligned and enter or leave the loop together
+    const int64_t nt = t / SRC;
+    const int src = (int)(t % SRC);
+    const int blk = (int)(nt / NPB), node = (int)(nt % NPB);
     const int bid = active[blk];
     int lb[3];
     decode_local<D>(bid, scenes[0], lb);
     const Params& P = scenes[scene_of<D>(lb, slab)];
-    if (P.skip) continue;
+    if (P.skip) continue;  // uniform over the group: one node, one scene
     const int l[3] = {node & 3, (node >> 2) & 3, node >> 4};
     float m = 0.0f, p[3] = {0.0f, 0.0f, 0.0f};
+    {
+      const int d[3] = {src & 1, (src >> 1) & 1, src >> 2};
+      int q[3], tl[3];
+      bool ok = true;
 #pragma unroll
-    for (int dz = 0; dz < (D == 3 ? 2 : 1); dz++)
-#pragma unroll
-      for (int dy = 0; dy < 2; dy++)
-#pragma unroll
-        for (int dx = 0; dx < 2; dx++) {
-          const int d[3] = {dx, dy, dz};
-          int q[3], tl[3];
-          bool ok = true;
-#pragma unroll
-          for (int a = 0; a < D; a++) {
-            tl[a] = l[a] + kBlock * d[a];
-            q[a] = lb[a] - d[a];
-            ok &= (tl[a] < kTile) & (q[a] >= 0);
-          }
-          if (!ok) continue;
-          const int nb = encode_local<D>(q, P);
-          const int c0 = pchunk_begin[nb], cn = pchunk_cnt[nb];
-          const int tn = D == 3 ? tl[0] + kTile * (tl[1] + kTile * tl[2]) : tl[0] + kTile * tl[1];
-          for (int k = c0; k < c0 + cn; k++) {
-            const float4 tv = tiles[(int64_t)k * TN + tn];
-            p[0] += tv.x;
-            p[1] += tv.y;
-            if (D == 3) p[2] += tv.z;
-            m += tv.w;
-          }
+      for (int a = 0; a < D; a++) {
+        tl[a] = l[a] + kBlock * d[a];
+        q[a] = lb[a] - d[a];
+        ok &= (tl[a] < kTile) & (q[a] >= 0);
+      }
+      if (ok) {
+        const int nb = encode_local<D>(q, P);
+        const int c0 = pchunk_begin[nb], cn = pchunk_cnt[nb];
+        const int tn = D == 3 ? tl[0] + kTile * (tl[1] + kTile * tl[2]) : tl[0] + kTile * tl[1];
+        for (int k = c0; k < c0 + cn; k++) {
+          const float4 tv = tiles[(int64_t)k * TN + tn];
+          p[0] += tv.x;
+          p[1] += tv.y;
+          if (D == 3) p[2] += tv.z;
+          m += tv.w;
         }
+      }
+    }
+#pragma unroll
+    for (int off = 1; off < SRC; off <<= 1) {
+      p[0] += __shfl_xor(p[0], off);
+      p[1] += __shfl_xor(p[1], off);
+      if (D == 3) p[2] += __shfl_xor(p[2], off);
+      m += __shfl_xor(m, off);
+    }
+    if (src != 0) continue;
     if (MODE == 1) {
       const float4 f = grid[(int64_t)bid * NPB + node];
       float pb[3] = {0.0f, 0.0f, 0.0f}, mb = 0.0f;
@@ -916,7 +920,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p(const Params* __restrict__ sce
       tile_y[i] = g.y;
       tile_z[i] = g.z;
     }
-    __syncthreads();
+    lds_barrier();
     int b[3];
 #pragma unroll
     for (int a = 0; a < D; a++) b[a] = (lbl[a] + P.blo[a]) * kBlock;
@@ -1052,7 +1056,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p(const Params* __restrict__ sce
     }
     const int rank = (int)__popcll(peers & lt);
     if (live && rank == 0) dwcnt[wave][dir] = (int)__popcll(peers);
-    __syncthreads();
+    lds_barrier();
     if (threadIdx.x < 27) {
       const int d = threadIdx.x;
       int run = 0;
@@ -1068,12 +1072,12 @@ __global__ __launch_bounds__(kThreads) void k_g2p(const Params* __restrict__ sce
         atomicAdd(&count_next[encode_local<D>(q, P)], run);
       }
     }
-    __syncthreads();
+    lds_barrier();
     if (live) {
       code[j] = dir | ((dwcnt[wave][dir] + rank) << 8);
       nkey[j] = key_new;
     }
-    __syncthreads();
+    lds_barrier();
   }
   // out of place: particles outside every chunk (escaped the window; sorted
   // to the end) keep their whole state
@@ -1089,7 +1093,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p(const Params* __restrict__ sce
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off));
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = vmax;
-  __syncthreads();
+  lds_barrier();
   if (threadIdx.x == 0) {
     float m = red[0];
 #pragma unroll
@@ -1475,7 +1479,7 @@ void launch_transfer(const Params* P, const int* slab, SubstepBuffers& B, int ra
   {
     TimerScope ts(tm, "k_grid_op", st);
     const int64_t node_bound = (int64_t)B.total_blocks * Geo<D>::kNodesPerBlock;
-    k_grid_op<D, 0><<<clamp_groups(div_up(node_bound, kThreads), G / 2), kThreads, 0, st>>>(
+    k_grid_op<D, 0><<<clamp_groups(div_up(node_bound * (D == 3 ? 8 : 4), kThreads), 2 * G), kThreads, 0, st>>>(
         P, slab, B.active, B.totals, B.pchunk_begin, B.pchunk_cnt, B.tiles, B.grid, raw, nullptr);
   }
 }
@@ -1535,7 +1539,7 @@ void launch_adjoint(const Params* P, const int* slab, SubstepBuffers& B, const f
     {
       TimerScope ts(tm, "k_grid_op_adj", st);
       const int64_t node_bound = (int64_t)B.total_blocks * Geo<D>::kNodesPerBlock;
-      k_grid_op<D, 1><<<clamp_groups(div_up(node_bound, kThreads), G / 2), kThreads, 0, st>>>(
+      k_grid_op<D, 1><<<clamp_groups(div_up(node_bound * (D == 3 ? 8 : 4), kThreads), 2 * G), kThreads, 0, st>>>(
           P, slab, B.active, B.totals, B.pchunk_begin, B.pchunk_cnt, B.tiles, B.grid, 0, gbar);
     }
     {
