@@ -480,46 +480,55 @@ __global__ __launch_bounds__(kThreads) void k_grid_op_g2p2g(
     const float4* __restrict__ tiles, float4* __restrict__ grid) {
   constexpr int NPB = Geo<D>::kNodesPerBlock;
   constexpr int E8 = G2P2GGeo<D>::kEdge, TN8 = G2P2GGeo<D>::kNodes;
-  const int64_t total = (int64_t)totals[kTotActive] * NPB;
+  // The tile of block q = b - d starts one node below q, so this node sits at
+  // tile coordinate l + 1 + 4 d, inside the tile for d = 0 and, per axis,
+  // for d = -1 (l = 3) or d = +1 (l < 3): 2^D source blocks, one lane each,
+  // combined with a butterfly of lane exchanges as in k_grid_op.
+  constexpr int SRC = D == 3 ? 8 : 4;
+  const int64_t total = (int64_t)totals[kTotActive] * NPB * SRC;
   for (int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * kThreads) {
-    const int blk = (int)(t / NPB), node = (int)(t % NPB);
+    const int64_t nt = t / SRC;
+    const int src = (int)(t % SRC);
+    const int blk = (int)(nt / NPB), node = (int)(nt % NPB);
     const int bid = active[blk];
     int lb[3];
     decode_local<D>(bid, scenes[0], lb);
     const Params& P = scenes[scene_of<D>(lb, slab)];
-    if (P.skip) continue;
+    if (P.skip) continue;  // uniform over the group
     const int l[3] = {node & 3, (node >> 2) & 3, node >> 4};
     float m = 0.0f, p[3] = {0.0f, 0.0f, 0.0f};
+    {
+      int q[3] = {lb[0], lb[1], lb[2]}, tl[3] = {0, 0, 0};
+      bool ok = true;
 #pragma unroll
-    for (int dz = (D == 3 ? -1 : 0); dz <= (D == 3 ? 1 : 0); dz++)
-#pragma unroll
-      for (int dy = -1; dy <= 1; dy++)
-#pragma unroll
-        for (int dx = -1; dx <= 1; dx++) {
-          // tile of block q = b - d starts one node below q: this node sits
-          // at tile coordinate l + 1 + 4 d
-          const int d[3] = {dx, dy, dz};
-          int q[3] = {lb[0], lb[1], lb[2]}, tl[3] = {0, 0, 0};
-          bool ok = true;
-#pragma unroll
-          for (int a = 0; a < D; a++) {
-            tl[a] = l[a] + 1 + kBlock * d[a];
-            q[a] = lb[a] - d[a];
-            ok &= (tl[a] >= 0) & (tl[a] < E8) & (q[a] >= 0) & (q[a] < scenes[0].bn[a]);
-          }
-          if (!ok) continue;
-          const int nb = encode_local<D>(q, P);
-          const int c0 = pchunk_begin[nb], cn = pchunk_cnt[nb];
-          const int tn = D == 3 ? tl[0] + E8 * (tl[1] + E8 * tl[2]) : tl[0] + E8 * tl[1];
-          for (int k = c0; k < c0 + cn; k++) {
-            const float4 tv = tiles[(int64_t)k * TN8 + tn];
-            p[0] += tv.x;
-            p[1] += tv.y;
-            if (D == 3) p[2] += tv.z;
-            m += tv.w;
-          }
+      for (int a = 0; a < D; a++) {
+        const int d = ((src >> a) & 1) ? (l[a] == kBlock - 1 ? -1 : 1) : 0;
+        tl[a] = l[a] + 1 + kBlock * d;
+        q[a] = lb[a] - d;
+        ok &= (q[a] >= 0) & (q[a] < scenes[0].bn[a]);
+      }
+      if (ok) {
+        const int nb = encode_local<D>(q, P);
+        const int c0 = pchunk_begin[nb], cn = pchunk_cnt[nb];
+        const int tn = D == 3 ? tl[0] + E8 * (tl[1] + E8 * tl[2]) : tl[0] + E8 * tl[1];
+        for (int k = c0; k < c0 + cn; k++) {
+          const float4 tv = tiles[(int64_t)k * TN8 + tn];
+          p[0] += tv.x;
+          p[1] += tv.y;
+          if (D == 3) p[2] += tv.z;
+          m += tv.w;
         }
+      }
+    }
+#pragma unroll
+    for (int off = 1; off < SRC; off <<= 1) {
+      p[0] += __shfl_xor(p[0], off);
+      p[1] += __shfl_xor(p[1], off);
+      if (D == 3) p[2] += __shfl_xor(p[2], off);
+      m += __shfl_xor(m, off);
+    }
+    if (src != 0) continue;
     if (m > 0.0f) {  // mpm_solver.py:591-593
       const float inv = 1.0f / m;
 #pragma unroll

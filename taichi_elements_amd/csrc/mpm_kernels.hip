@@ -1456,7 +1456,8 @@ static void launch_g2p2g(const Params* P, const int* slab, SubstepBuffers& B, in
   {
     TimerScope ts(tm, "k_grid_op", st);
     const int64_t node_bound = (int64_t)B.total_blocks * Geo<D>::kNodesPerBlock;
-    k_grid_op_g2p2g<D><<<clamp_groups(div_up(node_bound, kThreads), G / 2), kThreads, 0, st>>>(
+    k_grid_op_g2p2g<D><<<clamp_groups(div_up(node_bound * (D == 3 ? 8 : 4), kThreads), 2 * G), kThreads,
+                         0, st>>>(
         P, slab, B.active, B.totals, B.pchunk_begin, B.pchunk_cnt, B.tiles, B.grid);
   }
   std::swap(B.count, B.count_next);
