@@ -417,6 +417,8 @@ def main():
                     help="keep the default launch shape instead of autotuning it on the scene")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the 64-scene batched measurement")
+    ap.add_argument("--no-small2d", action="store_true",
+                    help="skip the 2D reference-sized scene (graphs vs kernel by kernel)")
     ap.add_argument("--no-quant", action="store_true",
                     help="skip the packed-storage (quant=True) legs")
     ap.add_argument("--no-g2p2g", action="store_true",
@@ -561,7 +563,7 @@ def main():
     }
     if args.copy_calibration:
         out["copy_calibration_bytes"] = copy_calibration()
-    if args.config == "dense1m":
+    if args.config == "dense1m" and not args.no_small2d:
         out["small2d"] = small2d_bench()
     if not args.no_g2p2g and args.config == "dense1m":
         del s

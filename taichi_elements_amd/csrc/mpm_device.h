@@ -136,6 +136,46 @@ __device__ __forceinline__ void line_contrib(int oy, int oz, const float Ad[3][3
   }
 }
 
+// Lanes per grid node for the grid operators: as many (a power of two up to
+// src) as the launched threads cover in one pass over the active nodes.
+__device__ __forceinline__ int node_lanes(int64_t nodes, int src) {
+  int lanes = src;
+  while (lanes > 1 && nodes * lanes > (int64_t)gridDim.x * blockDim.x) lanes >>= 1;
+  return lanes;
+}
+
+// Sum of SRC per-source partial sums of one grid node, computed by `lanes`
+// aligned lanes (a power of two dividing SRC): lane `sub` sums the sources
+// [sub * SRC / lanes, (sub + 1) * SRC / lanes) with part(s), then the lanes
+// exchange.  Both steps follow one binary tree over the sources
+// ((s0 + s1) + (s2 + s3)) + ..., so every lane count gives bitwise the same
+// total (on every lane of the group).
+template <int SRC, class Part>
+__device__ __forceinline__ float4 tree_sum(int lanes, int sub, Part part) {
+  const int per = SRC / lanes;  // uniform: every lane runs the same part() calls
+  float4 v[SRC];
+#pragma unroll
+  for (int k = 0; k < SRC; k++)
+    v[k] = k < per ? part(sub * per + k) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+  for (int w = 1; w < SRC; w <<= 1)
+#pragma unroll
+    for (int k = 0; k + w < SRC; k += 2 * w) {
+      v[k].x += v[k + w].x;
+      v[k].y += v[k + w].y;
+      v[k].z += v[k + w].z;
+      v[k].w += v[k + w].w;
+    }
+  float4 r = v[0];
+  for (int off = 1; off < lanes; off <<= 1) {
+    r.x += __shfl_xor(r.x, off);
+    r.y += __shfl_xor(r.y, off);
+    r.z += __shfl_xor(r.z, off);
+    r.w += __shfl_xor(r.w, off);
+  }
+  return r;
+}
+
 // Quadratic B-spline weights, mpm_solver.py:505.
 __device__ __forceinline__ void bspline(float f, float w[3]) {
   w[0] = 0.5f * (1.5f - f) * (1.5f - f);

@@ -482,14 +482,15 @@ __global__ __launch_bounds__(kThreads) void k_grid_op_g2p2g(
   constexpr int E8 = G2P2GGeo<D>::kEdge, TN8 = G2P2GGeo<D>::kNodes;
   // The tile of block q = b - d starts one node below q, so this node sits at
   // tile coordinate l + 1 + 4 d, inside the tile for d = 0 and, per axis,
-  // for d = -1 (l = 3) or d = +1 (l < 3): 2^D source blocks, one lane each,
-  // combined with a butterfly of lane exchanges as in k_grid_op.
+  // for d = -1 (l = 3) or d = +1 (l < 3): 2^D source blocks, summed by
+  // node_lanes lanes per node and combined in a fixed tree as in k_grid_op.
   constexpr int SRC = D == 3 ? 8 : 4;
-  const int64_t total = (int64_t)totals[kTotActive] * NPB * SRC;
-  for (int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x; t < total;
+  const int64_t nodes = (int64_t)totals[kTotActive] * NPB;
+  const int lanes = node_lanes(nodes, SRC);
+  for (int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x; t < nodes * lanes;
        t += (int64_t)gridDim.x * kThreads) {
-    const int64_t nt = t / SRC;
-    const int src = (int)(t % SRC);
+    const int64_t nt = t / lanes;
+    const int sub = (int)(t % lanes);
     const int blk = (int)(nt / NPB), node = (int)(nt % NPB);
     const int bid = active[blk];
     int lb[3];
@@ -497,8 +498,8 @@ __global__ __launch_bounds__(kThreads) void k_grid_op_g2p2g(
     const Params& P = scenes[scene_of<D>(lb, slab)];
     if (P.skip) continue;  // uniform over the group
     const int l[3] = {node & 3, (node >> 2) & 3, node >> 4};
-    float m = 0.0f, p[3] = {0.0f, 0.0f, 0.0f};
-    {
+    const float4 sum = tree_sum<SRC>(lanes, sub, [&](int src) {
+      float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       int q[3] = {lb[0], lb[1], lb[2]}, tl[3] = {0, 0, 0};
       bool ok = true;
 #pragma unroll
@@ -514,21 +515,16 @@ __global__ __launch_bounds__(kThreads) void k_grid_op_g2p2g(
         const int tn = D == 3 ? tl[0] + E8 * (tl[1] + E8 * tl[2]) : tl[0] + E8 * tl[1];
         for (int k = c0; k < c0 + cn; k++) {
           const float4 tv = tiles[(int64_t)k * TN8 + tn];
-          p[0] += tv.x;
-          p[1] += tv.y;
-          if (D == 3) p[2] += tv.z;
-          m += tv.w;
+          acc.x += tv.x;
+          acc.y += tv.y;
+          if (D == 3) acc.z += tv.z;
+          acc.w += tv.w;
         }
       }
-    }
-#pragma unroll
-    for (int off = 1; off < SRC; off <<= 1) {
-      p[0] += __shfl_xor(p[0], off);
-      p[1] += __shfl_xor(p[1], off);
-      if (D == 3) p[2] += __shfl_xor(p[2], off);
-      m += __shfl_xor(m, off);
-    }
-    if (src != 0) continue;
+      return acc;
+    });
+    if (sub != 0) continue;
+    float m = sum.w, p[3] = {sum.x, sum.y, sum.z};
     if (m > 0.0f) {  // mpm_solver.py:591-593
       const float inv = 1.0f / m;
 #pragma unroll

@@ -771,18 +771,19 @@ __global__ __launch_bounds__(kThreads) void k_grid_op(const Params* __restrict__
                                                       float4* __restrict__ grid, int raw,
                                                       float4* __restrict__ gbar) {
   constexpr int NPB = Geo<D>::kNodesPerBlock, TN = Geo<D>::kTileNodes;
-  // SRC lanes per node, one per particle block b - d (d in {0,1}^D) whose
-  // tiles reach the node; the partial sums are combined with a butterfly of
-  // lane exchanges (every lane of the group ends with the same, bitwise
-  // reproducible total).  A lane per source block gives this latency-bound
-  // kernel SRC times the loads in flight of a lane per node.
+  // Lanes per node (node_lanes), each summing the tiles of some of the SRC
+  // particle blocks b - d (d in {0,1}^D) whose tiles reach the node; the
+  // partial sums combine in a fixed tree (tree_sum), so the result does not
+  // depend on the lane count.  A sparse window gets a lane per source block:
+  // this latency-bound kernel then has SRC times the loads in flight.
   constexpr int SRC = D == 3 ? 8 : 4;
-  const int64_t total = (int64_t)totals[kTotActive] * NPB * SRC;
-  for (int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x; t < total;
+  const int64_t nodes = (int64_t)totals[kTotActive] * NPB;
+  const int lanes = node_lanes(nodes, SRC);
+  for (int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x; t < nodes * lanes;
        t += (int64_t)gridDim.x * kThreads) {
-    // groups of SRC lanes are aligned and enter or leave the loop together
-    const int64_t nt = t / SRC;
-    const int src = (int)(t % SRC);
+    // groups of `lanes` lanes are aligned and enter or leave the loop together
+    const int64_t nt = t / lanes;
+    const int sub = (int)(t % lanes);
     const int blk = (int)(nt / NPB), node = (int)(nt % NPB);
     const int bid = active[blk];
     int lb[3];
@@ -790,8 +791,8 @@ __global__ __launch_bounds__(kThreads) void k_grid_op(const Params* __restrict__
     const Params& P = scenes[scene_of<D>(lb, slab)];
     if (P.skip) continue;  // uniform over the group: one node, one scene
     const int l[3] = {node & 3, (node >> 2) & 3, node >> 4};
-    float m = 0.0f, p[3] = {0.0f, 0.0f, 0.0f};
-    {
+    const float4 sum = tree_sum<SRC>(lanes, sub, [&](int src) {
+      float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       const int d[3] = {src & 1, (src >> 1) & 1, src >> 2};
       int q[3], tl[3];
       bool ok = true;
@@ -807,21 +808,16 @@ __global__ __launch_bounds__(kThreads) void k_grid_op(const Params* __restrict__
         const int tn = D == 3 ? tl[0] + kTile * (tl[1] + kTile * tl[2]) : tl[0] + kTile * tl[1];
         for (int k = c0; k < c0 + cn; k++) {
           const float4 tv = tiles[(int64_t)k * TN + tn];
-          p[0] += tv.x;
-          p[1] += tv.y;
-          if (D == 3) p[2] += tv.z;
-          m += tv.w;
+          acc.x += tv.x;
+          acc.y += tv.y;
+          if (D == 3) acc.z += tv.z;
+          acc.w += tv.w;
         }
       }
-    }
-#pragma unroll
-    for (int off = 1; off < SRC; off <<= 1) {
-      p[0] += __shfl_xor(p[0], off);
-      p[1] += __shfl_xor(p[1], off);
-      if (D == 3) p[2] += __shfl_xor(p[2], off);
-      m += __shfl_xor(m, off);
-    }
-    if (src != 0) continue;
+      return acc;
+    });
+    if (sub != 0) continue;
+    float m = sum.w, p[3] = {sum.x, sum.y, sum.z};
     if (MODE == 1) {
       const float4 f = grid[(int64_t)bid * NPB + node];
       float pb[3] = {0.0f, 0.0f, 0.0f}, mb = 0.0f;
@@ -1344,6 +1340,11 @@ __global__ __launch_bounds__(kThreads) void k_fill_u32(unsigned* __restrict__ p,
 // launchers
 // ---------------------------------------------------------------------------
 static inline int div_up(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+// Workgroups of the grid operators (lanes per node x source block): enough
+// that a large window needs few grid-stride iterations per lane, while the
+// idle workgroups of a sparse window stay cheap.
+constexpr int kGridOpGroups = 16384;
+
 static inline int clamp_groups(int64_t g, int cap) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(g, cap));
 }
@@ -1456,7 +1457,7 @@ static void launch_g2p2g(const Params* P, const int* slab, SubstepBuffers& B, in
   {
     TimerScope ts(tm, "k_grid_op", st);
     const int64_t node_bound = (int64_t)B.total_blocks * Geo<D>::kNodesPerBlock;
-    k_grid_op_g2p2g<D><<<clamp_groups(div_up(node_bound * (D == 3 ? 8 : 4), kThreads), 2 * G), kThreads,
+    k_grid_op_g2p2g<D><<<clamp_groups(div_up(node_bound * (D == 3 ? 8 : 4), kThreads), kGridOpGroups), kThreads,
                          0, st>>>(
         P, slab, B.active, B.totals, B.pchunk_begin, B.pchunk_cnt, B.tiles, B.grid);
   }
@@ -1480,7 +1481,7 @@ void launch_transfer(const Params* P, const int* slab, SubstepBuffers& B, int ra
   {
     TimerScope ts(tm, "k_grid_op", st);
     const int64_t node_bound = (int64_t)B.total_blocks * Geo<D>::kNodesPerBlock;
-    k_grid_op<D, 0><<<clamp_groups(div_up(node_bound * (D == 3 ? 8 : 4), kThreads), 2 * G), kThreads, 0, st>>>(
+    k_grid_op<D, 0><<<clamp_groups(div_up(node_bound * (D == 3 ? 8 : 4), kThreads), kGridOpGroups), kThreads, 0, st>>>(
         P, slab, B.active, B.totals, B.pchunk_begin, B.pchunk_cnt, B.tiles, B.grid, raw, nullptr);
   }
 }
@@ -1540,7 +1541,7 @@ void launch_adjoint(const Params* P, const int* slab, SubstepBuffers& B, const f
     {
       TimerScope ts(tm, "k_grid_op_adj", st);
       const int64_t node_bound = (int64_t)B.total_blocks * Geo<D>::kNodesPerBlock;
-      k_grid_op<D, 1><<<clamp_groups(div_up(node_bound * (D == 3 ? 8 : 4), kThreads), 2 * G), kThreads, 0, st>>>(
+      k_grid_op<D, 1><<<clamp_groups(div_up(node_bound * (D == 3 ? 8 : 4), kThreads), kGridOpGroups), kThreads, 0, st>>>(
           P, slab, B.active, B.totals, B.pchunk_begin, B.pchunk_cnt, B.tiles, B.grid, 0, gbar);
     }
     {
