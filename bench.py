@@ -193,12 +193,13 @@ def batched_bench(steps, warmup, autotune=True):
     return out
 
 
-def fwd_bwd_bench(nsub=64, checkpoint_every=8, repeats=3, seed=4):
+def fwd_bwd_bench(nsub=64, checkpoint_every=None, repeats=3, seed=4):
     """Differentiable workload: 128^3 grid, an elastic cube of 100k particles
     with per-particle E ~ U[5e5, 2e6] and a water cube of 100k particles
     pressed against one of its faces (so that the elastic body's motion
     depends on its stiffness through the contact), v ~ U[-0.5, 0.5]^3,
-    64 substeps at default_dt with checkpoints every 8; loss = squared
+    64 substeps at default_dt (every substep's input kept: DiffMPMSolver.run's
+    default at this size; also timed with checkpoints every 8); loss = squared
     distance of the elastic body's centre of mass from a seeded target;
     gradients with respect to the initial velocities, E and nu.  Times the
     forward alone (no autograd) and forward + backward."""
@@ -251,16 +252,22 @@ def fwd_bwd_bench(nsub=64, checkpoint_every=8, repeats=3, seed=4):
 
     t_f, loss = timed(forward_only)
     t_fb, (loss2, gv, ge, gn) = timed(forward_backward)
+    chosen = ds.last_checkpoint_every  # 64 states of 200k particles: ~1.4 GB, all kept
+    # the same with checkpoints every 8 substeps (segments recomputed)
+    checkpoint_every = 8
+    t_fb8, (_, gv8, _, _) = timed(forward_backward)
     ds.check()
     n = n_el + n_wa
-    return dict(particles=n, substeps=nsub, checkpoint_every=checkpoint_every,
+    return dict(particles=n, substeps=nsub, checkpoint_every=chosen,
                 fwd_ms=1000 * t_f, fwd_bwd_ms=1000 * t_fb, ratio=t_fb / t_f,
                 fwd_value=n * nsub / t_f, fwd_bwd_value=n * nsub / t_fb,
+                checkpoints_every_8=dict(fwd_bwd_ms=1000 * t_fb8, ratio=t_fb8 / t_f,
+                                         same_gradients=bool(torch.equal(gv8, gv))),
                 unit="particle-substeps/s", loss=float(loss),
                 grad_norms=dict(v0=float(gv.norm()), E=float(ge.norm()), nu=float(gn)),
                 workload="128^3 grid, elastic cube 100k (E ~ U[5e5, 2e6]) + water cube 100k, "
-                         "64 substeps, checkpoints every 8, loss = |COM_elastic - target|^2, "
-                         "gradients w.r.t. v0, E, nu")
+                         "64 substeps, every substep's input kept (also: checkpoints every 8), "
+                         "loss = |COM_elastic - target|^2, gradients w.r.t. v0, E, nu")
 
 
 def variant_bench(x, v, m, steps, warmup, autotune, workload, **solver_kw):

@@ -54,6 +54,7 @@ struct SubstepBuffers {
   LBState* lbs;
   float4* tiles;    // p2g per-chunk node tiles (momentum xyz, mass)
   float4* grid;     // dense block-major grid [total_blocks][nodes] of (v, mass)
+  float4* raw_out;  // if set, the grid operator also stores the raw p2g sums here
   int* err;         // error bits
   unsigned* maxv;   // per-substep, per-g2p-workgroup max |v| (float bits)
   int* step;        // device substep counter selecting the maxv slot (null: slot 0);

@@ -769,7 +769,8 @@ __global__ __launch_bounds__(kThreads) void k_grid_op(const Params* __restrict__
                                                       const int* __restrict__ pchunk_cnt,
                                                       const float4* __restrict__ tiles,
                                                       float4* __restrict__ grid, int raw,
-                                                      float4* __restrict__ gbar) {
+                                                      float4* __restrict__ gbar,
+                                                      float4* __restrict__ raw_out) {
   constexpr int NPB = Geo<D>::kNodesPerBlock, TN = Geo<D>::kTileNodes;
   // Lanes per node (node_lanes), each summing the tiles of some of the SRC
   // particle blocks b - d (d in {0,1}^D) whose tiles reach the node; the
@@ -818,6 +819,7 @@ __global__ __launch_bounds__(kThreads) void k_grid_op(const Params* __restrict__
     });
     if (sub != 0) continue;
     float m = sum.w, p[3] = {sum.x, sum.y, sum.z};
+    if (MODE == 0 && raw_out) raw_out[(int64_t)bid * NPB + node] = sum;
     if (MODE == 1) {
       const float4 f = grid[(int64_t)bid * NPB + node];
       float pb[3] = {0.0f, 0.0f, 0.0f}, mb = 0.0f;
@@ -1482,7 +1484,8 @@ void launch_transfer(const Params* P, const int* slab, SubstepBuffers& B, int ra
     TimerScope ts(tm, "k_grid_op", st);
     const int64_t node_bound = (int64_t)B.total_blocks * Geo<D>::kNodesPerBlock;
     k_grid_op<D, 0><<<clamp_groups(div_up(node_bound * (D == 3 ? 8 : 4), kThreads), kGridOpGroups), kThreads, 0, st>>>(
-        P, slab, B.active, B.totals, B.pchunk_begin, B.pchunk_cnt, B.tiles, B.grid, raw, nullptr);
+        P, slab, B.active, B.totals, B.pchunk_begin, B.pchunk_cnt, B.tiles, B.grid, raw, nullptr,
+        B.raw_out);
   }
 }
 
@@ -1542,7 +1545,8 @@ void launch_adjoint(const Params* P, const int* slab, SubstepBuffers& B, const f
       TimerScope ts(tm, "k_grid_op_adj", st);
       const int64_t node_bound = (int64_t)B.total_blocks * Geo<D>::kNodesPerBlock;
       k_grid_op<D, 1><<<clamp_groups(div_up(node_bound * (D == 3 ? 8 : 4), kThreads), kGridOpGroups), kThreads, 0, st>>>(
-          P, slab, B.active, B.totals, B.pchunk_begin, B.pchunk_cnt, B.tiles, B.grid, 0, gbar);
+          P, slab, B.active, B.totals, B.pchunk_begin, B.pchunk_cnt, B.tiles, B.grid, 0, gbar,
+          nullptr);
     }
     {
       TimerScope ts(tm, "k_p2g_adj", st);

@@ -439,6 +439,7 @@ void fill_buffers(mpm_solver* s, SubstepBuffers& B) {
   B.totals = s->totals + cur * 16;
   B.tiles = s->tiles;
   B.grid = s->g2p2g ? s->grid_alt : s->grid;
+  B.raw_out = nullptr;
   B.err = s->err;
   B.maxv = s->maxv;
   B.step = s->d_step;
@@ -1273,8 +1274,12 @@ int diff_params(mpm_solver* s, float dt, const float* youngs, double nu, hipStre
 
 // Saved binning of a differentiable substep: perm, chunks, per-block chunk
 // ranges, active blocks and totals, each 256-byte aligned in one blob.
+// Per-substep record of the differentiable forward, reused by its backward:
+// the binning (sorted permutation, chunks, per-block chunk ranges, active
+// list, totals) and the raw p2g grid sums (momentum, mass) of the active
+// nodes, so that the backward needs neither a sort nor a p2g.
 struct BinLayout {
-  int64_t perm, chunks, pbeg, pcnt, active, totals, bytes;
+  int64_t perm, chunks, pbeg, pcnt, active, totals, grid, bytes;
 };
 
 BinLayout bin_layout(const mpm_solver* s) {
@@ -1287,7 +1292,8 @@ BinLayout bin_layout(const mpm_solver* s) {
   L.pcnt = L.pbeg + al(4 * tb);
   L.active = L.pcnt + al(4 * tb);
   L.totals = L.active + al(4 * tb);
-  L.bytes = L.totals + 256;
+  L.grid = L.totals + 256;
+  L.bytes = L.grid + 16 * tb * (s->D == 3 ? Geo<3>::kNodesPerBlock : Geo<2>::kNodesPerBlock);
   return L;
 }
 
@@ -1410,6 +1416,7 @@ int mpm_diff_substep(mpm_solver* s, float dt, const float* state_in, float* stat
   SubstepBuffers B;
   diff_buffers(s, B, const_cast<float*>(state_in));
   B.pf_out = state_out;  // p2g / g2p write the new state out of place
+  if (binning) B.raw_out = (float4*)((char*)binning + bin_layout(s).grid);
   if (s->D == 3)
     launch_substep<3>(s->d_params, s->d_slab, B, 0, 0, st, nullptr);
   else
@@ -1447,8 +1454,8 @@ int mpm_diff_substep_backward(mpm_solver* s, float dt, const float* state_in,
   hipStream_t st = (hipStream_t)stream;  // NULL: the null stream
   HIP_TRY(hipSetDevice(s->device));
   TRY(diff_params(s, dt, youngs, nu, st));
-  // the forward's raw grid sums recomputed from the old state (p2g writes
-  // its F / Jp into scratch), on the saved binning or a recomputed one
+  // the forward's binning and raw grid sums: from its record, or recomputed
+  // from the old state (p2g then writes its F / Jp into scratch)
   SubstepBuffers B;
   diff_buffers(s, B, const_cast<float*>(state_in));
   B.pf_out = s->diff_scratch;
@@ -1461,10 +1468,7 @@ int mpm_diff_substep_backward(mpm_solver* s, float dt, const float* state_in,
     B.pchunk_cnt = (int*)(b + L.pcnt);
     B.active = (int*)(b + L.active);
     B.totals = (int*)(b + L.totals);
-    if (s->D == 3)
-      launch_transfer<3>(s->d_params, s->d_slab, B, 1, st, nullptr);
-    else
-      launch_transfer<2>(s->d_params, s->d_slab, B, 1, st, nullptr);
+    B.grid = (float4*)(b + L.grid);
   } else {
     if (s->D == 3)
       launch_front<3>(s->d_params, s->d_slab, B, 1, st, nullptr);

@@ -173,10 +173,18 @@ class DiffMPMSolver:
             state, _ = _Substep.apply(state, youngs, nu, self, nu_value, dt)
         return state
 
-    def run(self, state, youngs, nu, nsub, dt=None, checkpoint_every=8, nu_value=None):
+    # Device memory that run() may fill with the inputs of every substep
+    # before it falls back to checkpointing (a few per cent of an MI355X's
+    # 288 GB of HBM).
+    keep_bytes = 8 << 30
+
+    def run(self, state, youngs, nu, nsub, dt=None, checkpoint_every=None, nu_value=None):
         """``nsub`` substeps; with gradients enabled and checkpoint_every > 1
         only every checkpoint_every-th state is kept and each segment is
-        recomputed in the backward pass."""
+        recomputed in the backward pass.  checkpoint_every=None keeps every
+        substep's input (no recomputation) when they fit in ``keep_bytes``,
+        else checkpoints every 8 substeps.  The gradients are the same
+        either way."""
         if dt is None:
             dt = self.default_dt
         if nu_value is None:
@@ -184,6 +192,10 @@ class DiffMPMSolver:
         if self._order is not None:
             state = state[:, self._order]
             youngs = youngs[self._order]
+        if checkpoint_every is None:
+            per = state.numel() * state.element_size() + self._binning_bytes
+            checkpoint_every = 1 if nsub * per <= self.keep_bytes else 8
+        self.last_checkpoint_every = int(checkpoint_every)
         k = max(1, int(checkpoint_every))
         for t0 in range(0, nsub, k):
             m = min(k, nsub - t0)
