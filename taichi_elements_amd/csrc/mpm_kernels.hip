@@ -1,17 +1,25 @@
 // gfx950 kernels of the MLS-MPM substep.
 //
 // One substep (mpm_solver.py:789-799) is:
-//   k_keys            block key of every particle + per-block counts   (build_pid, mpm:344-361)
-//   k_bscan_*         scan of the dense block window: particle offsets,
+//   k_keys            block key of every particle + per-block counts   (build_pid, mpm:344-361;
+//                     first substep, or positions set by the host)
+//   k_bscan*          scan of the dense block window: particle offsets,
 //                     transfer work items (chunks), active grid blocks  (replaces the pointer
 //                                                                        SNodes and deactivate_all)
-//   k_radix_*         stable LSD radix sort of slots by block key
+//   k_onesweep        stable LSD radix sort of slots by block key (with k_keys), or
+//   k_offsets,        the steady-state neighbour sort: the previous order
+//   k_nscatter        stably re-sorted from g2p's direction codes
 //   k_p2g             per chunk: constitutive update + scatter into an
 //                     LDS tile of (4+2)^d nodes, tile written to HBM     (p2g, mpm:487-584)
 //   k_grid_op         per active node: sum of the overlapping tiles,
 //                     normalisation, gravity, post-process              (mpm:586-692)
 //   k_g2p             per chunk: LDS tile of grid velocity, gather,
-//                     advection, max |v|                                (g2p, mpm:694-735)
+//                     advection, max |v|, next binning's hints          (g2p, mpm:694-735)
+//   k_reorder         every few substeps: particle state moved into the sorted order
+// use_g2p2g replaces p2g / grid op / g2p with k_g2p2g + k_grid_op_g2p2g
+// (mpm_g2p2g.h); the differentiable backward runs the adjoints of
+// mpm_adjoint.h.  Particles are read and written through Particles<D, Q>
+// (mpm_particles.h), instantiated for the float and the packed layout.
 #include "mpm_kernels.h"
 #include "mpm_adjoint.h"
 #include "mpm_g2p2g.h"
