@@ -55,6 +55,19 @@ class _Field0D:
         self._set(value)
 
 
+class _ParticleStorage:
+    """Stands in for the reference's particle SNode, whose per-particle size
+    the demos print (``mpm.particle._cell_size_bytes``,
+    demo/demo_3d_bunnies.py:116)."""
+
+    def __init__(self, solver):
+        self._solver = solver
+
+    @property
+    def _cell_size_bytes(self):
+        return self._solver.particle_bytes()
+
+
 def _autotune(obj, handle, dt, get_state, set_state, substeps, chunks, groups, verbose, dts):
     """Shared by MPMSolver and MPMSolverBatch: time each launch shape on
     the live particles (two untimed substeps re-bin after each change)."""
@@ -192,6 +205,7 @@ class MPMSolver:
         self._gravity = [0.0] * self.dim
         self._handle = None
         self.n_particles = _Field0D(self._num_particles)
+        self.particle = _ParticleStorage(self)
         self.gravity = _Field0D(lambda: tuple(self._gravity), self.set_gravity)
         self.grid_postprocess = []
         self.writers = []

@@ -76,6 +76,7 @@ def test_storage_rounds_like_the_oracle(oracle, dim, g2p2g, plastic):
     # insertion index, scene + the host-side colour
     words = 4 + (dim * dim + 1) // 2 + (0 if g2p2g else dim * dim) + (1 if plastic else 0)
     assert s.particle_bytes() == 4 * (words + 3) + 4
+    assert s.particle._cell_size_bytes == s.particle_bytes()  # as the demos print it
     # a fresh particle: F = I is stored as 7992 steps on the diagonal
     s2, _ = _make(dim, n=10, g2p2g=g2p2g, plastic=plastic)
     s2.add_particles(np.full((1, dim), 0.4, np.float32), 1)
