@@ -55,6 +55,13 @@ class BatchScene(MPMSolver):
         n1 = self._batch._total_particles()
         self._ids = np.concatenate([self._ids, np.arange(n0, n1, dtype=np.int64)])
 
+    def add_mesh(self, *args, **kwargs):
+        n0 = self._batch._total_particles()
+        added = super().add_mesh(*args, **kwargs)
+        n1 = self._batch._total_particles()
+        self._ids = np.concatenate([self._ids, np.arange(n0, n1, dtype=np.int64)])
+        return added
+
     # readout: the library returns the whole batch in insertion order
     def particle_info(self):
         full = self._batch._particle_info_all()
@@ -130,6 +137,14 @@ class MPMSolverBatch:
         quant, plastic = storage.pop()
         if quant or not plastic:
             check(L.mpm_set_storage(h, int(quant), int(plastic)))
+        modes = {(bool(s.use_g2p2g), bool(s.v_clamp_g2p2g), float(s.g2p2g_allowed_cfl))
+                 for s in self.scenes}
+        if len(modes) > 1:
+            raise ValueError("use_g2p2g, v_clamp_g2p2g and g2p2g_allowed_cfl must be the same "
+                             "for every scene of a batch")
+        g2p2g, clamp, cfl = modes.pop()
+        if g2p2g:
+            check(L.mpm_set_g2p2g(h, 1, cfl, int(clamp)))
         if kernel_profiler:
             check(L.mpm_set_timing(h, 1))
         for s in self.scenes:

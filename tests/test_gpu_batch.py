@@ -160,3 +160,46 @@ def test_batch_rejects_bad_time_steps():
     # a scene with no particles and a zero time step are both fine
     b.substeps(2, [1e-4, 0.0])
     assert b[0].total_substeps == 2 and b[1].total_substeps == 0
+
+
+def test_g2p2g_batch_matches_separate_solvers_bit_exactly():
+    """use_g2p2g scenes in one batch: bitwise their own solvers (mixed
+    scene sizes, colliders and gravity; the fused transfer and its grid
+    operator run once over all scenes)."""
+    from taichi_elements_amd import MPMSolver, MPMSolverBatch
+    dim = 3
+    specs = _scene_specs(dim)[:3]
+    states = []
+    for sp in specs:
+        st = _particles(sp, dim)
+        st["F"] = np.broadcast_to(np.eye(dim, dtype=np.float32), st["F"].shape).copy()
+        st["Jp"] = np.where(st["material"] == 3, 0.0, 1.0).astype(np.float32)
+        st["C"] = np.zeros_like(st["C"])
+        states.append(st)
+    b = MPMSolverBatch([dict(res=sp["res"], E_scale=sp["E_scale"], use_g2p2g=True)
+                        for sp in specs])
+    for k in (2, 0, 1):
+        _configure(b[k], specs[k], states[k])
+    nsub = 20
+    b.substeps(nsub)
+    for k, sp in enumerate(specs):
+        solo = MPMSolver(res=sp["res"], E_scale=sp["E_scale"], use_g2p2g=True)
+        _configure(solo, sp, states[k])
+        for _ in range(nsub):
+            solo.substep(solo.default_dt)
+        _assert_same(b[k].get_state(), solo.get_state(), f"g2p2g scene={k}")
+    with pytest.raises(ValueError):
+        MPMSolverBatch([dict(res=(32, ) * 3, use_g2p2g=True), dict(res=(32, ) * 3)])
+
+
+def test_batch_scene_add_mesh():
+    from taichi_elements_amd import MPMSolver, MPMSolverBatch
+    from tests.helpers import icosphere, triangles_of
+    tri = triangles_of(*icosphere(2), scale=0.15, center=(0.5, 0.5, 0.5))
+    b = MPMSolverBatch([dict(res=(32, ) * 3), dict(res=(48, ) * 3)], seed=5)
+    b[0].add_cube((0.2, 0.2, 0.2), (0.1, 0.1, 0.1), MPMSolver.material_water)
+    n = b[1].add_mesh(triangles=tri, material=MPMSolver.material_elastic, color=0x123456)
+    assert n > 0 and b[1].n_particles[None] == n
+    info = b.particle_info()[1]
+    assert len(info["position"]) == n and np.all(info["color"] == 0x123456)
+    assert np.all(np.linalg.norm(info["position"] - 0.5, axis=1) < 0.15 + b[1].dx)
