@@ -98,7 +98,10 @@ const char* mpm_last_error(void);
 int mpm_set_gravity(mpm_solver* s, const double* g);
 
 /* grid_postprocess list (mpm_solver.py:300, 642, 687-692), applied in the
- * order added after gravity.  unbounded must be 0. */
+ * order added after gravity.  The bounding box's walls sit `padding` cells
+ * inside [0, res), or, with unbounded = 1, inside [-grid_size/2,
+ * grid_size/2) (mpm_solver.py:604-616; grid_size as set by
+ * mpm_set_unbounded, else 4096). */
 int mpm_add_bounding_box(mpm_solver* s, int32_t unbounded);
 /* add_surface_collider (mpm_solver.py:647-687): normal is normalised here
  * as the reference does on the host; sticky with friction != 0 is
@@ -268,6 +271,16 @@ int mpm_quantize_particles(mpm_solver* s, float* ranges, uint32_t* x_and_v, uint
  * differentiable mode.  mpm_get_state / mpm_particle_info return the stored
  * (rounded) values. */
 int mpm_set_storage(mpm_solver* s, int32_t quant, int32_t support_plasticity);
+/* MPMSolver(unbounded=True) (mpm_solver.py:74, 143-148, 604-611) for a
+ * one-scene solver: grid_size = 4096 doubled until it exceeds twice the
+ * largest resolution, and the dense block window follows the particles --
+ * every 8 substeps the particles' bounding box (one small reduction and a
+ * host synchronisation) is checked against the window, which is moved and
+ * resized when a particle could otherwise leave it within the next 8
+ * substeps (the grid a use_g2p2g substep reads is carried over).  The
+ * window holds at most 2^24 blocks; a scene spread wider raises
+ * MPM_EDOMAIN.  Call before mpm_add_bounding_box. */
+int mpm_set_unbounded(mpm_solver* s, int32_t enabled);
 /* Bytes one particle occupies (device state plus its host-side colour) in
  * the current storage mode (the reference's figure is 40 B for quant with
  * use_g2p2g and no plasticity, README.md:60). */

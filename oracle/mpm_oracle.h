@@ -75,6 +75,10 @@ typedef struct {
   /* MPMSolver(support_plasticity=False) (mpm_solver.py:62, 135-136, 419-421,
    * 436-437, 456, 510-518, 533-534, 541-542, 553). */
   int32_t no_plasticity;
+  /* MPMSolver(unbounded=True) (mpm_solver.py:143-148, 604-611): walls at
+   * -grid_size/2 + padding and grid_size/2 - padding instead of padding and
+   * res - padding; this holds grid_size/2 (0: bounded). */
+  int32_t unbounded_half;
 } oracle_cfg;
 
 /* The packed particle formats, in place: the value each element reads back

@@ -42,7 +42,8 @@ class Cfg(ctypes.Structure):
                 ("post", PostOp * MAX_POST),
                 ("youngs", ctypes.POINTER(ctypes.c_double)), ("nu", ctypes.c_double),
                 ("v_clamp_cfl", ctypes.c_double),
-                ("quant", ctypes.c_int32), ("no_plasticity", ctypes.c_int32)]
+                ("quant", ctypes.c_int32), ("no_plasticity", ctypes.c_int32),
+                ("unbounded_half", ctypes.c_int32)]
 
 
 def build():

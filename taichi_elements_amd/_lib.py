@@ -74,6 +74,7 @@ SIGNATURES = [
     ("mpm_quantize_particles", ctypes.c_int, [_P, _FP, ctypes.POINTER(ctypes.c_uint32),
                                               ctypes.POINTER(ctypes.c_uint8)]),
     ("mpm_set_storage", ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),
+    ("mpm_set_unbounded", ctypes.c_int, [_P, ctypes.c_int32]),
     ("mpm_particle_bytes", ctypes.c_int, [_P, _I32P]),
     ("mpm_voxelize", ctypes.c_int, [ctypes.c_int32, _I32P, ctypes.c_double, ctypes.c_int32,
                                     ctypes.c_int32, ctypes.c_int64, _D3, _I32P, _I32P]),

@@ -123,6 +123,8 @@ class MPMSolverBatch:
         self.dim = self.scenes[0].dim
         if any(s.dim != self.dim for s in self.scenes):
             raise ValueError("all scenes of a batch must have the same dimension")
+        if any(s.unbounded for s in self.scenes):
+            raise NotImplementedError("unbounded scenes run in their own MPMSolver")
         cfgs = (_lib.MpmConfig * len(self.scenes))()
         for k, s in enumerate(self.scenes):
             cfgs[k] = s._make_config()

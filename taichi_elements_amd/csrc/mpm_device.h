@@ -39,6 +39,10 @@ struct PostOp {
   float point[3];
   float normal[3];
   float radius, friction;
+  // bounding box (mpm_solver.py:600-616): node index I_a < lo[a] stops a
+  // negative v_a, I_a >= hi[a] a positive one; [padding, res - padding), or
+  // [-grid_size/2 + padding, grid_size/2 - padding) for unbounded
+  int lo[3], hi[3];
 };
 
 // Constants of one substep, passed by value to every kernel.  Host-side
@@ -583,8 +587,8 @@ __device__ __forceinline__ void apply_post(const Params& P, const int I[3], floa
     if (op.kind == kPostBBox) {  // mpm_solver.py:613-616
 #pragma unroll
       for (int a = 0; a < D; a++) {
-        if (I[a] < P.padding && v[a] < 0.0f) v[a] = 0.0f;
-        if (I[a] >= P.res[a] - P.padding && v[a] > 0.0f) v[a] = 0.0f;
+        if (I[a] < op.lo[a] && v[a] < 0.0f) v[a] = 0.0f;
+        if (I[a] >= op.hi[a] && v[a] > 0.0f) v[a] = 0.0f;
       }
     } else if (op.kind == kPostPlane) {  // mpm_solver.py:660-685
       float dot = 0.0f;

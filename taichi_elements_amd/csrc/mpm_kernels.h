@@ -143,6 +143,16 @@ void launch_scatter_i(const int* src, int64_t cap, int ncomp, const int* orig, i
                       hipStream_t st);
 void launch_fill_u32(unsigned* p, unsigned v, int64_t n, hipStream_t st);
 
+// Bounding box of the particle positions: groups x 6 partial (min, max) per axis.
+void launch_aabb(int D, bool quant, const float* pf, int64_t cap, int n, float* partial,
+                 int groups, hipStream_t st);
+// A block window moved or resized: origin and extent (blocks, x fastest).
+struct WindowMap {
+  int old_blo[3], old_bn[3], new_blo[3], new_bn[3];
+};
+void launch_remap_grid(const float4* old_grid, float4* new_grid, const WindowMap& m,
+                       int64_t nodes, int npb, hipStream_t st);
+
 // Up to 8 int array copies in one launch.
 struct CopySegments {
   const int* src[8];

@@ -751,12 +751,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
 template <int D>
 __device__ __forceinline__ int wall_keep_mask(const Params& P, const int I[3], const float v[3]) {
   int keep = (1 << D) - 1;
-  bool bbox = false;
-  for (int k = 0; k < P.n_post; k++) bbox |= P.post[k].kind == kPostBBox;
-  if (bbox) {
+  for (int k = 0; k < P.n_post; k++) {
+    const PostOp& op = P.post[k];
+    if (op.kind != kPostBBox) continue;
 #pragma unroll
     for (int a = 0; a < D; a++) {
-      if ((I[a] < P.padding && v[a] < 0.0f) || (I[a] >= P.res[a] - P.padding && v[a] > 0.0f))
+      if ((I[a] < op.lo[a] && v[a] < 0.0f) || (I[a] >= op.hi[a] && v[a] > 0.0f))
         keep &= ~(1 << a);
     }
   }
@@ -1655,6 +1655,85 @@ void launch_copy_segments(const CopySegments& S, hipStream_t st) {
 void launch_fill_u32(unsigned* p, unsigned v, int64_t n, hipStream_t st) {
   if (n > 0)
     k_fill_u32<<<clamp_groups(div_up(n, kThreads), 4096), kThreads, 0, st>>>(p, v, n);
+}
+
+// Per-workgroup bounding box of the particle positions: partial[g * 6 + 2a]
+// = min, partial[g * 6 + 2a + 1] = max of x_a (workgroup g).
+template <int D, bool Q>
+__global__ __launch_bounds__(kThreads) void k_aabb(const float* __restrict__ pf, int64_t cap,
+                                                   int n, float* __restrict__ partial) {
+  __shared__ float red[kThreads / 64][6];
+  float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+  for (int p = blockIdx.x * kThreads + threadIdx.x; p < n; p += gridDim.x * kThreads) {
+    float x[3] = {0.0f, 0.0f, 0.0f};
+    Particles<D, Q>::load_x(pf, cap, p, x);
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+      lo[a] = fminf(lo[a], x[a]);
+      hi[a] = fmaxf(hi[a], x[a]);
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      lo[a] = fminf(lo[a], __shfl_xor(lo[a], off));
+      hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], off));
+    }
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      red[wave][2 * a] = lo[a];
+      red[wave][2 * a + 1] = hi[a];
+    }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int c = threadIdx.x;
+    float v = red[0][c];
+#pragma unroll
+    for (int w = 1; w < kThreads / 64; w++) v = (c & 1) ? fmaxf(v, red[w][c]) : fminf(v, red[w][c]);
+    partial[(int64_t)blockIdx.x * 6 + c] = v;
+  }
+}
+
+void launch_aabb(int D, bool quant, const float* pf, int64_t cap, int n, float* partial,
+                 int groups, hipStream_t st) {
+  if (D == 3)
+    (quant ? k_aabb<3, true> : k_aabb<3, false>)<<<groups, kThreads, 0, st>>>(pf, cap, n, partial);
+  else
+    (quant ? k_aabb<2, true> : k_aabb<2, false>)<<<groups, kThreads, 0, st>>>(pf, cap, n, partial);
+}
+
+// Grid contents carried into a moved block window: node i of new block b
+// takes node i of the same global block in the old window, or zero.
+__global__ __launch_bounds__(kThreads) void k_remap_grid(const float4* __restrict__ old_grid,
+                                                         float4* __restrict__ new_grid,
+                                                         WindowMap m, int64_t nodes, int npb) {
+  for (int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x; t < nodes;
+       t += (int64_t)gridDim.x * kThreads) {
+    const int64_t b = t / npb;
+    const int node = (int)(t % npb);
+    int64_t r = b;
+    int64_t ob = 0, stride = 1;
+    bool in = true;
+    for (int a = 0; a < 3; a++) {
+      const int c = (int)(r % m.new_bn[a]);
+      r /= m.new_bn[a];
+      const int oc = c + m.new_blo[a] - m.old_blo[a];
+      in &= (oc >= 0) & (oc < m.old_bn[a]);
+      ob += (int64_t)oc * stride;
+      stride *= m.old_bn[a];
+    }
+    new_grid[t] = in ? old_grid[ob * npb + node] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+}
+
+void launch_remap_grid(const float4* old_grid, float4* new_grid, const WindowMap& m,
+                       int64_t nodes, int npb, hipStream_t st) {
+  if (nodes > 0)
+    k_remap_grid<<<clamp_groups(div_up(nodes, kThreads), 8192), kThreads, 0, st>>>(
+        old_grid, new_grid, m, nodes, npb);
 }
 
 template void launch_transfer<2>(const Params*, const int*, SubstepBuffers&, int, hipStream_t,

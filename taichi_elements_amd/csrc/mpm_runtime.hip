@@ -201,6 +201,10 @@ struct mpm_solver {
   // is support_plasticity=False (no Jp in the packed layout)
   int quant = 0;
   int plastic = 1;
+  // unbounded (mpm_solver.py:143-148, 604-611): the walls sit at
+  // +-grid_half cells and the block window follows the particles
+  int unbounded = 0;
+  int grid_half = 2048;
 
   // 32-bit words per particle slot in the float buffers
   int nf() const {
@@ -630,6 +634,162 @@ extern "C" {
 
 const char* mpm_last_error(void) { return g_last_error.c_str(); }
 
+// Buffers sized by the combined block window (bn, total_blocks): block
+// counts, scans, chunk tables, active list, grid(s), slab table; zeroed.
+int alloc_window_buffers(mpm_solver* s) {
+  const int D = s->D;
+  const int64_t tb = s->total_blocks;
+  const int NPB = D == 3 ? Geo<3>::kNodesPerBlock : Geo<2>::kNodesPerBlock;
+  const int64_t nparts = (tb + kScanTile - 1) / kScanTile;
+  TRY(dalloc(&s->count, tb + 1));
+  TRY(dalloc(&s->count_next, tb + 1));
+  TRY(dalloc(&s->lb_status_scan, 4 * nparts));
+  TRY(dalloc(&s->pstart, tb));
+  TRY(dalloc(&s->pchunk_begin, 2 * tb));
+  TRY(dalloc(&s->pchunk_cnt, 2 * tb));
+  TRY(dalloc(&s->active, tb));
+  TRY(dalloc(&s->tile_prefix, nparts));
+  TRY(dalloc(&s->grid, tb * NPB));
+  if (s->g2p2g) TRY(dalloc(&s->grid_alt, tb * NPB));
+  TRY(dalloc(&s->d_slab, s->bn[D - 1]));
+  std::vector<int> slab(s->bn[D - 1], 0);
+  for (size_t k = 0; k < s->scenes.size(); k++)
+    for (int z = 0; z < s->scenes[k].bnl[D - 1]; z++) slab[s->scenes[k].zoff + z] = (int)k;
+  HIP_TRY(hipMemcpyAsync(s->d_slab, slab.data(), sizeof(int) * slab.size(), hipMemcpyHostToDevice,
+                         s->stream));
+  HIP_TRY(hipMemsetAsync(s->count, 0, sizeof(int) * (tb + 1), s->stream));
+  HIP_TRY(hipMemsetAsync(s->count_next, 0, sizeof(int) * (tb + 1), s->stream));
+  HIP_TRY(hipMemsetAsync(s->lb_status_scan, 0, sizeof(unsigned long long) * 4 * nparts,
+                         s->stream));
+  HIP_TRY(hipMemsetAsync(s->grid, 0, sizeof(float4) * tb * NPB, s->stream));
+  if (s->grid_alt) HIP_TRY(hipMemsetAsync(s->grid_alt, 0, sizeof(float4) * tb * NPB, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return MPM_OK;
+}
+
+void free_window_buffers(mpm_solver* s) {
+  dfree(s->count);
+  dfree(s->count_next);
+  dfree(s->lb_status_scan);
+  dfree(s->pstart);
+  dfree(s->pchunk_begin);
+  dfree(s->pchunk_cnt);
+  dfree(s->active);
+  dfree(s->tile_prefix);
+  dfree(s->grid);
+  dfree(s->grid_alt);
+  dfree(s->d_slab);
+}
+
+// Moves the block window of a one-scene solver to [blo, blo + bnl) (blocks).
+// The grid a use_g2p2g substep reads next is carried over; the next
+// substep bins from scratch.
+int set_window(mpm_solver* s, const int blo[3], const int bnl[3]) {
+  const int D = s->D;
+  int64_t tb = 1;
+  for (int a = 0; a < D; a++) tb *= bnl[a];
+  if (tb >= (1 << (8 * kMaxRadixPasses)) - 1)
+    return fail(MPM_EDOMAIN, "the particles of the unbounded scene spread over more than 2^24 "
+                             "grid blocks (the dense block window's limit)");
+  drop_graphs(s);
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  SceneHost& sc = s->scenes[0];
+  WindowMap m{};
+  for (int a = 0; a < 3; a++) {
+    m.old_blo[a] = sc.blo[a];
+    m.old_bn[a] = s->bn[a];
+  }
+  float4* keep = nullptr;  // use_g2p2g: the input grid of the next substep
+  if (s->g2p2g) {
+    keep = s->grid;
+    s->grid = nullptr;
+  }
+  free_window_buffers(s);
+  for (int a = 0; a < D; a++) {
+    sc.blo[a] = blo[a];
+    sc.bnl[a] = bnl[a];
+    s->bn[a] = bnl[a];
+  }
+  s->total_blocks = (int)tb;
+  s->key_bits = 1;
+  while ((1ll << s->key_bits) <= tb) s->key_bits++;
+  int r = alloc_window_buffers(s);
+  if (r == MPM_OK && keep) {
+    for (int a = 0; a < 3; a++) {
+      m.new_blo[a] = sc.blo[a];
+      m.new_bn[a] = s->bn[a];
+    }
+    const int NPB = D == 3 ? Geo<3>::kNodesPerBlock : Geo<2>::kNodesPerBlock;
+    launch_remap_grid(keep, s->grid, m, tb * NPB, NPB, s->stream);
+    if (hipStreamSynchronize(s->stream) != hipSuccess) r = fail(MPM_EHIP, "grid remap failed");
+  }
+  dfree(keep);
+  if (r != MPM_OK) return r;
+  if (s->cap > 0) TRY(alloc_chunk_buffers(s));
+  s->have_perm = false;
+  return MPM_OK;
+}
+
+// Unbounded scene: before the next `seg` substeps, make sure every particle
+// stays in the window for all of them (a particle moves less than one block
+// per substep): its block and the next one, `seg + 1` blocks of margin.
+// A window that does not cover that is moved to the particles' box with a
+// margin of twice that on each side, clamped to the walls' extent.
+int follow_particles(mpm_solver* s, int seg) {
+  if (!s->unbounded || s->n == 0) return MPM_OK;
+  const int D = s->D;
+  const int groups = (int)std::min<int64_t>((s->n + kThreads - 1) / kThreads, 1024);
+  float* part = nullptr;
+  TRY(dalloc(&part, (int64_t)groups * 6));
+  launch_aabb(D, s->quant != 0, s->pf, s->cap, (int)s->n, part, groups, s->stream);
+  std::vector<float> h((size_t)groups * 6);
+  hipError_t e = hipMemcpyAsync(h.data(), part, sizeof(float) * h.size(), hipMemcpyDeviceToHost,
+                                s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  dfree(part);
+  if (e != hipSuccess) return fail(MPM_EHIP, std::string("particle box: ") + hipGetErrorString(e));
+  const SceneHost& sc = s->scenes[0];
+  const float inv_dx = (float)sc.cfg.inv_dx;
+  const int margin = seg + 1;
+  const int wall_blocks = s->grid_half / kBlock + 2;  // beyond the walls nothing moves
+  int need_lo[3], need_hi[3], blo[3], bnl[3];
+  bool fits = true;
+  for (int a = 0; a < D; a++) {
+    float lo = h[2 * a], hi = h[2 * a + 1];
+    for (int g = 1; g < groups; g++) {
+      lo = std::min(lo, h[(size_t)g * 6 + 2 * a]);
+      hi = std::max(hi, h[(size_t)g * 6 + 2 * a + 1]);
+    }
+    if (!std::isfinite(lo) || !std::isfinite(hi))
+      return fail(MPM_EDOMAIN, "particle positions are not finite");
+    const auto block_of = [&](float x) {
+      const double c = std::floor((double)(x * inv_dx) - 0.5);  // base cell
+      return (int)std::floor(c / kBlock);
+    };
+    need_lo[a] = std::max(block_of(lo) - margin, -wall_blocks);
+    need_hi[a] = std::min(block_of(hi) + 1 + margin, wall_blocks);  // inclusive
+    fits &= need_lo[a] >= sc.blo[a] && need_hi[a] < sc.blo[a] + sc.bnl[a];
+    blo[a] = std::max(need_lo[a] - margin, -wall_blocks);
+    bnl[a] = std::min(need_hi[a] + margin, wall_blocks) - blo[a] + 1;
+  }
+  for (int a = D; a < 3; a++) {
+    blo[a] = 0;
+    bnl[a] = 1;
+  }
+  if (fits) return MPM_OK;
+  TRY(set_window(s, blo, bnl));
+  // the scene table carries the window
+  for (size_t k = 0; k < s->scenes.size(); k++) {
+    const float dt = s->h_params[k].dt;
+    const int skip = s->h_params[k].skip;
+    fill_params(s->h_params[k], s, s->scenes[k], dt, skip);
+  }
+  HIP_TRY(hipMemcpyAsync(s->d_params, s->h_params, sizeof(Params) * s->scenes.size(),
+                         hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return MPM_OK;
+}
+
 static int create_impl(const mpm_config* cfgs, int nscenes, mpm_solver** out) {
   if (!cfgs || !out || nscenes < 1) return fail(MPM_EINVAL, "null argument or no scene");
   *out = nullptr;
@@ -688,18 +848,11 @@ static int create_impl(const mpm_config* cfgs, int nscenes, mpm_solver** out) {
     delete s;
     return fail(MPM_EHIP, "cannot create a HIP stream");
   }
-  const int NPB = s->D == 3 ? Geo<3>::kNodesPerBlock : Geo<2>::kNodesPerBlock;
-  const int64_t nparts = (tb + kScanTile - 1) / kScanTile;
-  if ((r = dalloc(&s->count, tb + 1)) || (r = dalloc(&s->count_next, tb + 1)) ||
-      (r = dalloc(&s->lb_status_scan, 4 * nparts)) || (r = dalloc(&s->lbs, 1)) ||
-      (r = dalloc(&s->ghist, 2 * kMaxRadixPasses * kRadixBins)) ||
-      (r = dalloc(&s->pstart, tb)) || (r = dalloc(&s->pchunk_begin, 2 * tb)) ||
-      (r = dalloc(&s->pchunk_cnt, 2 * tb)) || (r = dalloc(&s->active, tb)) ||
-      (r = dalloc(&s->tile_prefix, nparts)) ||
-      (r = dalloc(&s->grid, tb * NPB)) || (r = dalloc(&s->totals, 32)) ||
-      (r = dalloc(&s->err, 1)) || (r = dalloc(&s->maxv, 16 * s->persistent_groups)) ||
-      (r = dalloc(&s->d_params, nscenes)) || (r = dalloc(&s->d_slab, s->bn[D - 1])) ||
-      (r = dalloc(&s->d_step, 1))) {
+  if ((r = dalloc(&s->lbs, 1)) || (r = dalloc(&s->ghist, 2 * kMaxRadixPasses * kRadixBins)) ||
+      (r = dalloc(&s->totals, 32)) || (r = dalloc(&s->err, 1)) ||
+      (r = dalloc(&s->maxv, 16 * s->persistent_groups)) ||
+      (r = dalloc(&s->d_params, nscenes)) || (r = dalloc(&s->d_step, 1)) ||
+      (r = alloc_window_buffers(s))) {
     mpm_destroy(s);
     return r;
   }
@@ -709,20 +862,9 @@ static int create_impl(const mpm_config* cfgs, int nscenes, mpm_solver** out) {
     mpm_destroy(s);
     return fail(MPM_ENOMEM, "pinned host allocation failed");
   }
-  {
-    std::vector<int> slab(s->bn[D - 1], 0);
-    for (int k = 0; k < nscenes; k++)
-      for (int z = 0; z < s->scenes[k].bnl[D - 1]; z++) slab[s->scenes[k].zoff + z] = k;
-    HIP_TRY(hipMemcpy(s->d_slab, slab.data(), sizeof(int) * slab.size(), hipMemcpyHostToDevice));
-  }
-  if (hipMemsetAsync(s->count, 0, sizeof(int) * (tb + 1), s->stream) != hipSuccess ||
-      hipMemsetAsync(s->count_next, 0, sizeof(int) * (tb + 1), s->stream) != hipSuccess ||
-      hipMemsetAsync(s->lb_status_scan, 0, sizeof(unsigned long long) * 4 * nparts, s->stream) !=
-          hipSuccess ||
-      hipMemsetAsync(s->lbs, 0, sizeof(LBState), s->stream) != hipSuccess ||
+  if (hipMemsetAsync(s->lbs, 0, sizeof(LBState), s->stream) != hipSuccess ||
       hipMemsetAsync(s->ghist, 0, sizeof(int) * 2 * kMaxRadixPasses * kRadixBins, s->stream) !=
           hipSuccess ||
-      hipMemsetAsync(s->grid, 0, sizeof(float4) * tb * NPB, s->stream) != hipSuccess ||
       hipMemsetAsync(s->totals, 0, sizeof(int) * 32, s->stream) != hipSuccess ||
       hipMemsetAsync(s->err, 0, sizeof(int), s->stream) != hipSuccess ||
       hipStreamSynchronize(s->stream) != hipSuccess) {
@@ -790,11 +932,16 @@ int mpm_set_gravity(mpm_solver* s, const double* g) {
 
 int mpm_add_bounding_box(mpm_solver* s, int32_t unbounded) {
   if (!s) return fail(MPM_EINVAL, "null argument");
-  if (unbounded) return fail(MPM_EINVAL, "unbounded simulations are not supported");
-  std::vector<PostOp>& post = s->scenes[s->active_scene].post;
+  SceneHost& sc = s->scenes[s->active_scene];
+  std::vector<PostOp>& post = sc.post;
   if (post.size() >= (size_t)kMaxPost) return fail(MPM_EINVAL, "too many grid post-processes");
   PostOp op{};
   op.kind = kPostBBox;
+  for (int a = 0; a < 3; a++) {  // mpm_solver.py:604-616
+    const int res = a < s->D ? sc.cfg.res[a] : 1;
+    op.lo[a] = unbounded ? -s->grid_half + sc.cfg.padding : sc.cfg.padding;
+    op.hi[a] = unbounded ? s->grid_half - sc.cfg.padding : res - sc.cfg.padding;
+  }
   post.push_back(op);
   return MPM_OK;
 }
@@ -914,8 +1061,16 @@ static int substeps_impl(mpm_solver* s, int32_t nsub, const float* dts, bool bat
                          hipMemcpyHostToDevice, s->stream));
   Timer* tm = s->timing ? &s->timer : nullptr;
   launch_fill_u32((unsigned*)s->d_step, 0xFFFFFFFFu, 1, s->stream);  // slot k for substep k
+  // unbounded: the window is checked against the particles every kSegment substeps
+  constexpr int kSegment = 8;
+  int next_check = 0;
   for (int k = 0; k < nsub;) {
-    if (steady_pair(s, nsub - k, tm)) {
+    if (s->unbounded && k == next_check) {
+      next_check = std::min(k + kSegment, nsub);
+      TRY(follow_particles(s, next_check - k));
+    }
+    const int left = s->unbounded ? next_check - k : nsub - k;
+    if (steady_pair(s, left, tm)) {
       TRY(run_graph_pair(s));
       k += 2;
       continue;
@@ -1217,6 +1372,7 @@ int mpm_diff_setup(mpm_solver* s, int64_t n, const int32_t* material) {
     if (op.kind != kPostBBox)
       return fail(MPM_EINVAL, "differentiable mode supports the bounding box only");
   if (s->g2p2g) return fail(MPM_EINVAL, "differentiable mode uses the p2g / g2p transfers");
+  if (s->unbounded) return fail(MPM_EINVAL, "differentiable mode keeps its block window");
   if (s->quant || !s->plastic)
     return fail(MPM_EINVAL, "differentiable mode uses the float particle layout with plasticity");
   HIP_TRY(hipSetDevice(s->device));
@@ -1354,6 +1510,25 @@ int mpm_particle_bytes(mpm_solver* s, int32_t* bytes) {
   if (!s || !bytes) return fail(MPM_EINVAL, "null argument");
   const int ni = s->D == 3 ? Layout<3>::NI : Layout<2>::NI;
   *bytes = 4 * (s->nf() + ni) + 4;  // device words + the host-side colour
+  return MPM_OK;
+}
+
+int mpm_set_unbounded(mpm_solver* s, int32_t enabled) {
+  if (!s) return fail(MPM_EINVAL, "null argument");
+  if (enabled && s->scenes.size() != 1)
+    return fail(MPM_EINVAL, "unbounded scenes are supported for one-scene solvers");
+  if (enabled && s->diff_n != 0)
+    return fail(MPM_EINVAL, "the differentiable mode keeps its block window");
+  s->unbounded = enabled ? 1 : 0;
+  // grid_size: 4096, doubled until it exceeds twice the largest resolution
+  // (mpm_solver.py:74, 143-148); the walls sit at +-grid_size/2
+  const mpm_config& c = s->scenes[0].cfg;
+  int64_t grid_size = 4096;
+  int rmax = 0;
+  for (int a = 0; a < s->D; a++) rmax = std::max(rmax, c.res[a]);
+  if (enabled)
+    while (grid_size <= 2 * (int64_t)rmax) grid_size *= 2;
+  s->grid_half = (int)(grid_size / 2);
   return MPM_OK;
 }
 

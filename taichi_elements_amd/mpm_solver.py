@@ -16,9 +16,13 @@ Differences from the reference, all explicit:
     4096^d sparse grid; particles that leave it raise ``DomainError``;
   * ``use_g2p2g=True`` runs the fused g2p2g transfer of the reference
     (mpm_solver.py:363-485) with its grid velocity clamp;
-  * ``unbounded``, ``quant``, ``use_adaptive_dt`` and
-    ``support_plasticity=False`` raise ``NotImplementedError``, as do the
-    mesh/texture/polygon seeders;
+  * ``unbounded=True`` keeps the reference's walls at +-grid_size/2
+    (mpm_solver.py:143-148, 604-611); the dense window then follows the
+    particles, checked every 8 substeps;
+  * ``quant=True`` stores particles in packed words of the same precision
+    as the reference's quantised fields (mpm_solver.py:106-136);
+  * ``use_adaptive_dt`` and the texture/polygon seeders raise
+    ``NotImplementedError``;
   * ``grid_postprocess`` lists descriptions of the built-in post-processes
     (bounding box and colliders); arbitrary Taichi callables cannot be added.
 """
@@ -153,9 +157,8 @@ class MPMSolver:
                  kernel_profiler=False):
         self.dim = len(res)
         assert self.dim in (2, 3), "MPM solver supports only 2D and 3D simulations."
-        for flag, name in ((unbounded, "unbounded"), (use_adaptive_dt, "use_adaptive_dt")):
-            if flag:
-                raise NotImplementedError(f"{name}=True is not supported by this engine")
+        if use_adaptive_dt:
+            raise NotImplementedError("use_adaptive_dt=True is not supported by this engine")
         self.quant = quant
         self.use_g2p2g = use_g2p2g
         self.v_clamp_g2p2g = v_clamp_g2p2g
@@ -251,6 +254,8 @@ class MPMSolver:
         h = _lib._P()
         check(L.mpm_create(cfg, h))
         self._handle = h
+        if self.unbounded:  # mpm_solver.py:143-148
+            check(L.mpm_set_unbounded(h, 1))
         if self.quant or not self.support_plasticity:  # mpm_solver.py:106-136, 216-262
             check(L.mpm_set_storage(h, int(bool(self.quant)), int(bool(self.support_plasticity))))
         if self.use_g2p2g:
@@ -282,10 +287,8 @@ class MPMSolver:
         check(_lib.lib().mpm_set_gravity(self._h(), _lib.dvec(self._gravity)))
 
     def add_bounding_box(self, unbounded):
-        if unbounded:
-            raise NotImplementedError("unbounded simulations are not supported")
-        check(_lib.lib().mpm_add_bounding_box(self._h(), 0))
-        self.grid_postprocess.append(("bounding_box", {}))
+        check(_lib.lib().mpm_add_bounding_box(self._h(), int(bool(unbounded))))
+        self.grid_postprocess.append(("bounding_box", {"unbounded": bool(unbounded)}))
 
     def add_sphere_collider(self, center, radius, surface=surface_sticky):
         center = list(center)
@@ -420,7 +423,7 @@ class MPMSolver:
         self.grid_postprocess = []
         for kind, kw in post:
             if kind == "bounding_box":
-                self.add_bounding_box(False)
+                self.add_bounding_box(kw.get("unbounded", False))
             elif kind == "sphere":
                 self.add_sphere_collider(**kw)
             else:

@@ -65,7 +65,19 @@ def oracle_cfg_for(O, solver, post=None, gravity=None):
         setattr(cfg, name, getattr(solver, name))
     cfg.quant = 1 if getattr(solver, "quant", False) else 0
     cfg.no_plasticity = 0 if getattr(solver, "support_plasticity", True) else 1
+    if any(kind == "bounding_box" and args.get("unbounded") for kind, args in
+           solver.grid_postprocess):
+        cfg.unbounded_half = unbounded_half(solver.res)
     return cfg
+
+
+def unbounded_half(res):
+    """grid_size / 2 of MPMSolver(unbounded=True): 4096 doubled while it is
+    at most twice the largest resolution (mpm_solver.py:74, 143-148)."""
+    g = 4096
+    while g <= 2 * max(res):
+        g *= 2
+    return g // 2
 
 
 # ---------------------------------------------------------------------------
