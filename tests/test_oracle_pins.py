@@ -497,3 +497,38 @@ def test_substep_f32_tracks_f64(oracle, dim):
     for k in ("x", "v", "C", "F", "Jp"):
         err = np.linalg.norm(a[k] - b[k]) / max(np.linalg.norm(b[k]), 1e-6)
         assert err < 1e-5, (k, err)
+
+
+@pytest.mark.parametrize("side", ["low", "high"])
+def test_unbounded_walls_are_the_bounded_walls_moved(oracle, side):
+    """unbounded=True walls at -grid_size/2 + padding and grid_size/2 -
+    padding (mpm_solver.py:604-611) are the bounded walls at padding and
+    res - padding moved by -grid_size/2 and grid_size/2 - res cells: an
+    elastic blob thrown against either wall moves the same (float64) once
+    translated by that many cells.  A wall one cell off changes the motion
+    by ~1e-2."""
+    res, half = (32, 32), 2048
+    shift = -half if side == "low" else half - res[0]  # cells
+    rng = np.random.default_rng(18)
+    n = 600
+    lo = 4 if side == "low" else 23
+    x = np.stack([rng.uniform(lo, lo + 5, n), rng.uniform(12.8, 19.2, n)], 1) / res[0]
+    v = np.zeros((n, 2))
+    v[:, 0] = -20.0 if side == "low" else 20.0
+    st = oracle.rest_state(x, v, np.ones(n, np.int32))
+    runs = []
+    for h, lo_x in ((0, -8), (half, -8 + shift)):
+        cfg = oracle.make_cfg(res, post=[{"kind": "bbox"}], grid_lo=[lo_x, -8], grid_n=[48, 48])
+        cfg.unbounded_half = h
+        s = {k: (np.asarray(a, np.float64) if k != "material" else a) for k, a in st.items()}
+        s["x"] = s["x"] + np.array([shift / res[0] if h else 0.0, 0.0])
+        run = oracle.Substepper(cfg, s, F64)
+        for _ in range(80):
+            run.substep(cfg.dx * 2e-2)
+        runs.append(run.s)
+    bounded, unbounded = runs
+    moved = unbounded["x"] - np.array([shift / res[0], 0.0])
+    assert np.abs(bounded["x"][:, 0] * res[0] - lo).max() > 0  # it moved
+    assert np.abs(bounded["x"][:, 0] * res[0]).min() > 1 and (bounded["v"][:, 0] != v[:, 0]).all()
+    np.testing.assert_allclose(moved, bounded["x"], rtol=0, atol=1e-9)
+    np.testing.assert_allclose(unbounded["v"], bounded["v"], rtol=0, atol=1e-6)
