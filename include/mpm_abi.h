@@ -140,6 +140,12 @@ int mpm_substeps_batch(mpm_solver* s, int32_t n, const float* dts);
  * mpm_solver.py:726-735) for the substeps of the last mpm_substeps call;
  * out has room for that call's n entries.  Synchronises. */
 int mpm_read_max_velocity(mpm_solver* s, int32_t n, float* out);
+/* compute_max_grid_velocity (mpm_solver.py:737-746), which step() calls
+ * before every substep when use_adaptive_dt (mpm_solver.py:762-770): max
+ * over grid nodes of max_i |v_i| on the grid the last substep left (the
+ * g2p2g input grid of the next substep with use_g2p2g); 0 before the first
+ * substep and without particles.  One-scene solvers.  Synchronises. */
+int mpm_max_grid_velocity(mpm_solver* s, float* out);
 
 /* particle_info (mpm_solver.py:1172-1191): position (n,dim), velocity
  * (n,dim), material (n,), color (n,) in insertion order.  Any pointer may

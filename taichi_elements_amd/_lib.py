@@ -48,6 +48,7 @@ SIGNATURES = [
     ("mpm_num_particles", ctypes.c_int, [_P, _I64P]),
     ("mpm_substeps", ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_float, ctypes.c_double]),
     ("mpm_read_max_velocity", ctypes.c_int, [_P, ctypes.c_int32, _FP]),
+    ("mpm_max_grid_velocity", ctypes.c_int, [_P, _FP]),
     ("mpm_particle_info", ctypes.c_int, [_P, _FP, _FP, _I32P, _I32P]),
     ("mpm_get_state", ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_void_p]),
     ("mpm_set_state", ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_void_p]),

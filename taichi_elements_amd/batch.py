@@ -123,8 +123,9 @@ class MPMSolverBatch:
         self.dim = self.scenes[0].dim
         if any(s.dim != self.dim for s in self.scenes):
             raise ValueError("all scenes of a batch must have the same dimension")
-        if any(s.unbounded for s in self.scenes):
-            raise NotImplementedError("unbounded scenes run in their own MPMSolver")
+        if any(s.unbounded or s.use_adaptive_dt for s in self.scenes):
+            raise NotImplementedError("unbounded and use_adaptive_dt scenes run in their own "
+                                      "MPMSolver")
         cfgs = (_lib.MpmConfig * len(self.scenes))()
         for k, s in enumerate(self.scenes):
             cfgs[k] = s._make_config()

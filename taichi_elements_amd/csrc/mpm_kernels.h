@@ -152,6 +152,10 @@ struct WindowMap {
 };
 void launch_remap_grid(const float4* old_grid, float4* new_grid, const WindowMap& m,
                        int64_t nodes, int npb, hipStream_t st);
+// out[0] = max over the nodes of the active blocks (count at totals[kTotActive])
+// of max_a |v_a|, as float bits; out is cleared first.
+void launch_grid_vmax(int D, const float4* grid, const int* active, const int* totals,
+                      int64_t total_blocks, unsigned* out, hipStream_t st);
 
 // Up to 8 int array copies in one launch.
 struct CopySegments {

@@ -1736,6 +1736,40 @@ void launch_remap_grid(const float4* old_grid, float4* new_grid, const WindowMap
         old_grid, new_grid, m, nodes, npb);
 }
 
+// compute_max_grid_velocity (mpm_solver.py:737-746): the largest velocity
+// component magnitude on the grid after the last grid operator.  Nodes of
+// active blocks without mass hold zero velocity, as the reference's
+// freshly activated nodes do.  Non-negative floats order like their bits.
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_grid_vmax(const float4* __restrict__ grid,
+                                                        const int* __restrict__ active,
+                                                        const int* __restrict__ totals,
+                                                        unsigned* __restrict__ out) {
+  constexpr int NPB = Geo<D>::kNodesPerBlock;
+  const int64_t nodes = (int64_t)totals[kTotActive] * NPB;
+  float m = 0.0f;
+  for (int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x; t < nodes;
+       t += (int64_t)gridDim.x * kThreads) {
+    const float4 g = grid[(int64_t)active[t / NPB] * NPB + t % NPB];
+    m = fmaxf(m, fmaxf(fabsf(g.x), fabsf(g.y)));
+    if (D == 3) m = fmaxf(m, fabsf(g.z));
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
+}
+
+void launch_grid_vmax(int D, const float4* grid, const int* active, const int* totals,
+                      int64_t total_blocks, unsigned* out, hipStream_t st) {
+  launch_fill_u32(out, 0u, 1, st);
+  const int npb = D == 3 ? Geo<3>::kNodesPerBlock : Geo<2>::kNodesPerBlock;
+  const int groups = clamp_groups(div_up(total_blocks * npb, kThreads), 2048);
+  if (D == 3)
+    k_grid_vmax<3><<<groups, kThreads, 0, st>>>(grid, active, totals, out);
+  else
+    k_grid_vmax<2><<<groups, kThreads, 0, st>>>(grid, active, totals, out);
+}
+
 template void launch_transfer<2>(const Params*, const int*, SubstepBuffers&, int, hipStream_t,
                                  Timer*);
 template void launch_transfer<3>(const Params*, const int*, SubstepBuffers&, int, hipStream_t,

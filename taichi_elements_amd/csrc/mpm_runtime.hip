@@ -160,6 +160,7 @@ struct mpm_solver {
   int since_reorder = 1 << 30;  // reorder at the first opportunity
   int64_t substep_counter = 0;
   int* d_step = nullptr;  // device substep counter (max-velocity slot)
+  unsigned* d_gvmax = nullptr;  // compute_max_grid_velocity result
   // HIP graphs of two steady-state substeps, keyed by the buffer pointers
   // and sizes their kernels were captured with
   bool use_graphs = true;
@@ -852,6 +853,7 @@ static int create_impl(const mpm_config* cfgs, int nscenes, mpm_solver** out) {
       (r = dalloc(&s->totals, 32)) || (r = dalloc(&s->err, 1)) ||
       (r = dalloc(&s->maxv, 16 * s->persistent_groups)) ||
       (r = dalloc(&s->d_params, nscenes)) || (r = dalloc(&s->d_step, 1)) ||
+      (r = dalloc(&s->d_gvmax, 1)) ||
       (r = alloc_window_buffers(s))) {
     mpm_destroy(s);
     return r;
@@ -906,6 +908,7 @@ int mpm_destroy(mpm_solver* s) {
   for (auto& g : s->graphs) hipGraphExecDestroy(g.exec);
   s->graphs.clear();
   dfree(s->d_step);
+  dfree(s->d_gvmax);
   dfree(s->grid);
   dfree(s->grid_alt);
   dfree(s->totals);
@@ -1510,6 +1513,21 @@ int mpm_particle_bytes(mpm_solver* s, int32_t* bytes) {
   if (!s || !bytes) return fail(MPM_EINVAL, "null argument");
   const int ni = s->D == 3 ? Layout<3>::NI : Layout<2>::NI;
   *bytes = 4 * (s->nf() + ni) + 4;  // device words + the host-side colour
+  return MPM_OK;
+}
+
+int mpm_max_grid_velocity(mpm_solver* s, float* out) {
+  if (!s || !out) return fail(MPM_EINVAL, "null argument");
+  if (s->scenes.size() != 1) return fail(MPM_EINVAL, "the grid of a batch holds many scenes");
+  HIP_TRY(hipSetDevice(s->device));
+  *out = 0.0f;
+  if (s->n == 0) return MPM_OK;  // an empty grid, as before the first substep
+  launch_grid_vmax(s->D, s->grid, s->active, s->totals + s->last_parity * 16, s->total_blocks,
+                   s->d_gvmax, s->stream);
+  unsigned bits = 0;
+  HIP_TRY(hipMemcpyAsync(&bits, s->d_gvmax, sizeof(unsigned), hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  std::memcpy(out, &bits, sizeof(float));
   return MPM_OK;
 }
 

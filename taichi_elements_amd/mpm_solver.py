@@ -21,8 +21,9 @@ Differences from the reference, all explicit:
     particles, checked every 8 substeps;
   * ``quant=True`` stores particles in packed words of the same precision
     as the reference's quantised fields (mpm_solver.py:106-136);
-  * ``use_adaptive_dt`` and the texture/polygon seeders raise
-    ``NotImplementedError``;
+  * ``use_adaptive_dt=True`` reads the grid's largest velocity back before
+    every substep, as the reference does (one host synchronisation per
+    substep); the texture/polygon seeders raise ``NotImplementedError``;
   * ``grid_postprocess`` lists descriptions of the built-in post-processes
     (bounding box and colliders); arbitrary Taichi callables cannot be added.
 """
@@ -157,8 +158,10 @@ class MPMSolver:
                  kernel_profiler=False):
         self.dim = len(res)
         assert self.dim in (2, 3), "MPM solver supports only 2D and 3D simulations."
-        if use_adaptive_dt:
-            raise NotImplementedError("use_adaptive_dt=True is not supported by this engine")
+        if use_adaptive_dt and not g2p2g_allowed_cfl > 0:
+            # dt = min(dt, cfl dx / (v + 1e-6)) would be 0 and step() would
+            # never finish (mpm_solver.py:762-771)
+            raise ValueError("use_adaptive_dt needs g2p2g_allowed_cfl > 0")
         self.quant = quant
         self.use_g2p2g = use_g2p2g
         self.v_clamp_g2p2g = v_clamp_g2p2g
@@ -498,6 +501,11 @@ class MPMSolver:
         frame_time_left = frame_dt
         if print_stat:
             print(f'needed substeps: {substeps}')
+        if self.use_adaptive_dt:
+            count, dt, maxv = self._adaptive_substeps(frame_dt, dt, smry_writer)
+            if print_stat:
+                self._print_step_stat(maxv, count, dt, begin_t, begin_substep)
+            return
         count = 0
         while frame_time_left > 0:  # float64 loop; may run substeps + 1 times
             count += 1
@@ -515,12 +523,41 @@ class MPMSolver:
             self.all_time_max_velocity = max(self.all_time_max_velocity, cur_frame_velocity)
 
         if print_stat:
-            self.print_kernel_profiler_info()
-            cur_frame_velocity = float(maxv[-1]) if count else 0.0
-            print(f'CFL: {cur_frame_velocity * dt / self.dx}')
-            print(f'num particles={self.n_particles[None]}')
-            print(f'  frame time {time.time() - begin_t:.3f} s')
-            print(f'  substep time {1000 * (time.time() - begin_t) / max(1, self.total_substeps - begin_substep):.3f} ms')
+            self._print_step_stat(maxv, count, dt, begin_t, begin_substep)
+
+    def _print_step_stat(self, maxv, count, dt, begin_t, begin_substep):
+        self.print_kernel_profiler_info()
+        cur_frame_velocity = float(maxv[-1]) if count else 0.0
+        print(f'CFL: {cur_frame_velocity * dt / self.dx}')
+        print(f'num particles={self.n_particles[None]}')
+        print(f'  frame time {time.time() - begin_t:.3f} s')
+        print(f'  substep time {1000 * (time.time() - begin_t) / max(1, self.total_substeps - begin_substep):.3f} ms')
+
+    def _adaptive_substeps(self, frame_dt, dt, smry_writer):
+        """step()'s loop with use_adaptive_dt (mpm_solver.py:758-808): before
+        each substep dt = min(dt, cfl dx / (max grid velocity + 1e-6), time
+        left in the frame), in float64 as the reference's host loop; the
+        substep itself runs with dt rounded to float32."""
+        frame_time_left = frame_dt
+        maxv = []
+        while frame_time_left > 0:
+            if PRINT_PROGRESS:
+                print('.', end='', flush=True)
+            self.total_substeps += 1
+            max_grid_v = self.compute_max_grid_velocity()
+            cfl_dt = self.g2p2g_allowed_cfl * self.dx / (max_grid_v + 1e-6)
+            dt = min(dt, cfl_dt, frame_time_left)
+            frame_time_left -= dt
+            cur_frame_velocity = float(self._run_substeps(1, dt)[0])
+            self.t += dt
+            maxv.append(cur_frame_velocity)
+            if smry_writer is not None:
+                smry_writer.add_scalar("substep_max_CFL", cur_frame_velocity * dt / self.dx,
+                                       self.total_substeps)
+            self.all_time_max_velocity = max(self.all_time_max_velocity, cur_frame_velocity)
+        if PRINT_PROGRESS:
+            print(flush=True)
+        return len(maxv), dt, maxv
 
     def _run_substeps(self, count, dt):
         L = _lib.lib()
@@ -537,6 +574,15 @@ class MPMSolver:
         self.t += dt
         self.all_time_max_velocity = max(self.all_time_max_velocity, float(maxv[0]))
         return float(maxv[0])
+
+    def compute_max_grid_velocity(self, grid_v=None):
+        """Largest velocity component magnitude on the grid the last substep
+        left (mpm_solver.py:737-746); ``grid_v`` is accepted for the
+        reference's signature and ignored (the engine holds one current
+        grid)."""
+        out = np.zeros(1, np.float32)
+        check(_lib.lib().mpm_max_grid_velocity(self._h(), _lib.fptr(out)))
+        return float(out[0])
 
     def compute_max_velocity(self):
         v = self.get_state()["v"]

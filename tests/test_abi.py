@@ -44,8 +44,8 @@ def test_config_struct_layout_matches_header():
 
 def test_unsupported_options_fail_before_touching_the_gpu():
     from taichi_elements_amd import MPMSolver
-    with pytest.raises(NotImplementedError):
-        MPMSolver(res=(16, 16, 16), use_adaptive_dt=True)
+    with pytest.raises(ValueError):  # its dt would be 0 and step() would never end
+        MPMSolver(res=(16, 16, 16), use_adaptive_dt=True, g2p2g_allowed_cfl=0.0)
     with pytest.raises(AssertionError):
         MPMSolver(res=(16, ))
 

@@ -1,0 +1,117 @@
+"""use_adaptive_dt=True on the GPU against the oracle.
+
+step() with use_adaptive_dt (mpm_solver.py:758-771) reads the largest grid
+velocity component before every substep (compute_max_grid_velocity,
+mpm_solver.py:737-746) and shrinks dt to g2p2g_allowed_cfl dx / (v + 1e-6)
+when that is smaller.  Checked:
+  * compute_max_grid_velocity after a substep equals the max over the
+    oracle's grid of |v_i| (relative 1e-5: float32 grids summed in different
+    orders) and is 0 before the first substep;
+  * a blob thrown for two frames, its dt limited by the CFL rule to
+    0.81x (p2g, 60 units/s) and 0.88x (use_g2p2g without its velocity
+    clamp, 30 units/s, CFL number 0.5) of the frame's uniform dt: step() runs the same
+    number of substeps with the same dts (relative 1e-5) as the reference's
+    loop driven by the oracle's grid, and ends within 1e-4 of the oracle's
+    particle state (x, v).
+"""
+import numpy as np
+import pytest
+
+from tests.helpers import cube_particles, oracle_cfg_for, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def _quiet():
+    from taichi_elements_amd import mpm_solver
+    mpm_solver.PRINT_PROGRESS = False
+
+
+def _scene(dim, g2p2g, seed=0, clamp=True, cfl=0.9):
+    """A water / elastic blob thrown along x: at 60 units/s (1.2 cells per
+    default_dt substep; the p2g path allows up to a 4-cell block), or at 30
+    units/s with use_g2p2g, whose particles may move at most one cell per
+    substep."""
+    from taichi_elements_amd import MPMSolver
+    rng = np.random.default_rng(900 + 10 * dim + seed)
+    n = 4000
+    s = MPMSolver(res=(32, ) * dim, use_adaptive_dt=True, use_g2p2g=g2p2g, v_clamp_g2p2g=clamp,
+                  g2p2g_allowed_cfl=cfl)
+    x = cube_particles(rng, n, [0.1] + [0.35] * (dim - 1), [0.2] * dim, dim)
+    v = rng.uniform(-1, 1, (n, dim)).astype(np.float32)
+    v[:, 0] += 30.0 if g2p2g else 60.0
+    mat = rng.integers(0, 2, n).astype(np.int32)  # water and elastic
+    s.read_restart(n, x, v, mat, np.zeros(n, np.int32))
+    return s, x, v, mat
+
+
+@pytest.mark.parametrize("g2p2g", [False, True])
+def test_max_grid_velocity_matches_the_oracle_grid(oracle, g2p2g):
+    s, x, v, mat = _scene(3, g2p2g)
+    assert s.compute_max_grid_velocity() == 0.0
+    dt = s.default_dt
+    cfg = oracle_cfg_for(oracle, s)
+    st = oracle.rest_state(x, v, mat)
+    if g2p2g:
+        ref = oracle.G2P2GStepper(oracle.with_clamp(cfg, s.g2p2g_allowed_cfl), st)
+    else:
+        ref = oracle.Substepper(cfg, st, np.float32)
+    for k in range(3):
+        s.substep(dt)
+        ref.substep(dt)
+        grid_v = ref.gv_in if g2p2g else ref.gv
+        want = float(np.abs(grid_v).max())
+        got = s.compute_max_grid_velocity()
+        print(f"g2p2g={g2p2g} substep {k + 1}: max grid v {got:.7g} oracle {want:.7g}", flush=True)
+        assert abs(got - want) <= 1e-5 * want
+
+
+@pytest.mark.parametrize("g2p2g", [False, True])
+def test_adaptive_step_follows_the_reference_loop(oracle, g2p2g):
+    # use_g2p2g at 30 units/s with a CFL number of 0.5 and without its
+    # velocity clamp (which would hold the grid at cfl dx / dt and so keep
+    # the CFL limit from shrinking dt)
+    s, x, v, mat = _scene(3, g2p2g, seed=1, clamp=False, cfl=0.5 if g2p2g else 0.9)
+    dts = []
+    run = s._run_substeps
+
+    def recording(count, dt):
+        dts.extend([dt] * count)
+        return run(count, dt)
+
+    s._run_substeps = recording
+    cfg = oracle_cfg_for(oracle, s)
+    st = oracle.rest_state(x, v, mat)
+    if g2p2g:
+        ref = oracle.G2P2GStepper(oracle.with_clamp(cfg, 0.0), st)
+    else:
+        ref = oracle.Substepper(cfg, st, np.float32)
+    frame_dt = 4e-3
+    ref_dts = []
+    grid_max = 0.0  # no grid before the first substep
+    for _ in range(2):
+        s.step(frame_dt)
+        # the reference's loop (mpm_solver.py:752-771) on the oracle's grid
+        substeps = int(frame_dt / s.default_dt) + 1
+        dt = frame_dt / substeps
+        left = frame_dt
+        while left > 0:
+            cfl_dt = s.g2p2g_allowed_cfl * s.dx / (grid_max + 1e-6)
+            dt = min(dt, cfl_dt, left)
+            left -= dt
+            ref.substep(float(np.float32(dt)))
+            ref_dts.append(dt)
+            grid_max = float(np.abs(ref.gv_in if g2p2g else ref.gv).max())
+    uniform = frame_dt / (int(frame_dt / s.default_dt) + 1)
+    print(f"g2p2g={g2p2g}: {len(dts)} substeps, dt / uniform dt:",
+          " ".join(f"{d / uniform:.3f}" for d in dts), flush=True)
+    assert len(dts) == len(ref_dts) == s.total_substeps
+    assert dts[1] < 0.9 * uniform  # the CFL limit is active from the second substep
+    np.testing.assert_allclose(dts, ref_dts, rtol=1e-5)
+    got = s.get_state()
+    for f in ("x", "v"):
+        e = rel_err(got[f], ref.s[f])
+        print(f"g2p2g={g2p2g} {f}: {e:.2e}", flush=True)
+        assert e <= 1e-4, f
+    assert abs(s.t - 2 * frame_dt) < 1e-12
