@@ -23,7 +23,7 @@ Differences from the reference, all explicit:
     as the reference's quantised fields (mpm_solver.py:106-136);
   * ``use_adaptive_dt=True`` reads the grid's largest velocity back before
     every substep, as the reference does (one host synchronisation per
-    substep); the texture/polygon seeders raise ``NotImplementedError``;
+    substep);
   * ``grid_postprocess`` lists descriptions of the built-in post-processes
     (bounding box and colliders); arbitrary Taichi callables cannot be added.
 """
@@ -469,11 +469,62 @@ class MPMSolver:
             [self._emitter_ids, np.full(int(added.value), emmiter_id, np.int32)])
         return int(added.value)
 
-    def add_ngon(self, *args, **kwargs):
-        raise NotImplementedError("add_ngon is not part of this engine")
+    def _random_point_in_unit_polygon(self, n, sides, angle):
+        """Uniform points in the regular polygon of circumradius 1 with a
+        vertex at ``angle`` from +x: rejection from [-1, 1]^2 by the
+        reference's test (mpm_solver.py:918-931) -- the point's angle past
+        the last vertex direction, theta = (atan2(y, x) - angle) mod (2 pi /
+        sides), is inside when |p| < cos(phi) / cos(phi - theta), phi = pi /
+        sides (the edge's distance along that direction)."""
+        central = 2 * math.pi / sides
+        phi = central / 2
+        out = np.empty((n, 2), np.float32)
+        got = 0
+        while got < n:
+            m = max(64, int((n - got) * 2.5))
+            cand = self._rng.random((m, 2), dtype=np.float32) * 2 - 1
+            c = cand.astype(np.float64)
+            theta = np.mod(np.arctan2(c[:, 1], c[:, 0]) - angle, central)
+            cand = cand[np.hypot(c[:, 0], c[:, 1]) < math.cos(phi) / np.cos(phi - theta)]
+            k = min(len(cand), n - got)
+            out[got:got + k] = cand[:k]
+            got += k
+        return out
 
-    def add_texture_2d(self, *args, **kwargs):
-        raise NotImplementedError("add_texture_2d is not part of this engine")
+    def add_ngon(self, sides, center, radius, angle, material, color=0xFFFFFF,
+                 sample_density=None, velocity=None):
+        """A regular polygon of particles (2D; mpm_solver.py:886-916):
+        ceil(sample_density * sides/2 sin(2 pi / sides) (radius inv_dx)^2)
+        particles, uniform in the polygon of circumradius ``radius`` around
+        ``center`` with a vertex at ``angle`` radians from +x."""
+        if self.dim != 2:
+            raise ValueError("Add Ngon only works for 2D simulations")
+        if sample_density is None:
+            sample_density = 2**self.dim
+        num_particles = 0.5 * (radius * self.inv_dx)**2 * math.sin(2 * math.pi / sides) * sides
+        num_particles = int(math.ceil(num_particles * sample_density))
+        self.set_source_velocity(velocity=velocity)
+        assert self.n_particles[None] + num_particles <= self.max_num_particles
+        pts = self._random_point_in_unit_polygon(num_particles, sides, angle)
+        x = np.asarray(center[:2], np.float32) + pts * np.float32(radius)
+        self._append(x, self._uniform_velocity(num_particles), material, color)
+
+    def add_texture_2d(self, offset_x, offset_y, texture, new_material, color=0xFFFFFF):
+        """One particle per texel above 0.1 (2D; mpm_solver.py:943-957): texel
+        (i, j) at (offset_x + i dx, offset_y + j dx), with the velocity of
+        the last seeding call.  Particles are stored in texel order (i major),
+        where the reference's atomic counter leaves the order open."""
+        if self.dim != 2:
+            raise ValueError("add_texture_2d works for 2D simulations")
+        tex = np.asarray(texture)
+        if tex.ndim != 2:
+            raise ValueError("texture must be a 2D array")
+        i, j = np.nonzero(tex > 0.1)
+        n = len(i)
+        assert self.n_particles[None] + n <= self.max_num_particles
+        x = np.stack([np.float32(offset_x) + i.astype(np.float32) * np.float32(self.dx),
+                      np.float32(offset_y) + j.astype(np.float32) * np.float32(self.dx)], 1)
+        self._append(x.astype(np.float32), self._uniform_velocity(n), new_material, color)
 
     def write_particles(self, fn, slice_size=1000000):
         """Quantised particle file (mpm_solver.py:1198-1200); the
