@@ -20,6 +20,11 @@
  *   g2p                            mpm_solver.py:694-724
  *   compute_max_velocity           mpm_solver.py:726-735
  *
+ * Pinned by tests/test_oracle_pins.py (closed forms, invariants, LAPACK),
+ * tests/test_oracle_g2p2g.py, tests/test_oracle_quant.py and
+ * tests/test_golden.py: whole substeps against the golden fixtures of an
+ * independent float64 brute-force restatement (tests/golden/make_golden.py).
+ *
  * The grid is a dense node window [grid_lo, grid_lo + grid_n) per axis,
  * x fastest.  Particle arrays are AoS: x[p*dim + a], C[p*dim*dim + a*dim + b].
  */
