@@ -12,6 +12,7 @@ rotations, the sign of det F on the last singular value).  It restates:
   grid operator      mpm_solver.py:586-598 (no g2p2g clamp), bounding box
                      walls 600-616
   g2p                mpm_solver.py:694-724
+  g2p2g (use_g2p2g)  mpm_solver.py:363-485, clamp 589, 596-598, loop 773-787
 
 Each fixture holds the inputs (x, v, C, F, Jp, material, dt, gravity, res)
 and the outputs of one substep (x, v, C, F, Jp and the grid nodes the
@@ -20,7 +21,8 @@ operator).  Run from the repository root:
 
     python tests/golden/make_golden.py
 
-which rewrites tests/golden/substep_{2,3}d.npz.  The states are
+which rewrites tests/golden/substep_{2,3}d.npz and g2p2g_{2,3}d.npz (three
+use_g2p2g substeps).  The states are
 pre-deformed (F = R diag(1 +- 0.05) Q^T, C ~ N(0, 5^2), snow Jp in
 [0.9, 1.1], sand Jp in [-0.05, 0], water Jp in [0.95, 1.05]) so that every
 constitutive branch does work, and the particles sit next to the low walls
@@ -157,13 +159,113 @@ def substep(st, res, dt, gravity):
     return dict(x=x, v=v, C=C, F=F, Jp=Jp), nodes, gm, gvel
 
 
+def constitutive_g2p2g(Fp, Jp, mat, C, dt, k, dim):
+    """The constitutive half of g2p2g (mpm_solver.py:411-473): F is not
+    rebuilt for water, the hardening applies to every material but elastic,
+    and Jp of water is untouched (its volume stays in F[0][0])."""
+    Fp = (np.eye(dim) + dt * C) @ Fp
+    h = math.exp(10 * (1.0 - Jp))
+    if mat == ELASTIC:
+        h = 0.3
+    mu, la = k["mu_0"] * h, k["lambda_0"] * h
+    if mat == WATER:
+        mu = 0.0
+    U, sig, V = svd_rotations(Fp)
+    J = 1.0
+    if mat != SAND:
+        for d in range(dim):
+            ns = sig[d]
+            if mat == SNOW:
+                ns = min(max(sig[d], 1 - 2.5e-2), 1 + 4.5e-3)
+            Jp *= sig[d] / ns
+            sig[d] = ns
+            J *= ns
+    if mat == WATER:
+        Fp = np.eye(dim)
+        Fp[0, 0] = J
+    elif mat == SNOW:
+        Fp = U @ np.diag(sig) @ V.T
+    if mat != SAND:
+        stress = 2 * mu * (Fp - U @ V.T) @ Fp.T + np.eye(dim) * la * J * (J - 1)
+    else:
+        sig, Jp = sand_projection(sig, Jp, k, dim)
+        Fp = U @ np.diag(sig) @ V.T
+        ls = np.log(sig)
+        center = np.diag(2.0 * k["mu_0"] * ls / sig + k["lambda_0"] * ls.sum() / sig)
+        stress = U @ center @ V.T @ Fp.T
+    return Fp, Jp, stress
+
+
+def g2p2g_substeps(st, res, dt, gravity, nsub, cfl=0.9):
+    """use_g2p2g substeps (mpm_solver.py:363-485, 586-598, 773-787): g2p from
+    the input grid (none before the first substep: every particle is new and
+    keeps its v with C = 0), advection, constitutive update and p2g into the
+    output grid; the grid operator clamps the velocity to +-cfl dx / dt.
+    Returns the state and output grid after each substep."""
+    dim = len(res)
+    k = constants(res)
+    x, v, F, Jp, mat = (st[f].copy() for f in ("x", "v", "F", "Jp", "material"))
+    inv_dx, dx = k["inv_dx"], k["dx"]
+    offsets = [np.array(o) for o in np.ndindex(*(3, ) * dim)]
+    grid_in = {}
+    n_final = 0
+    out = []
+    for _ in range(nsub):
+        grid = {}
+        for p in range(len(x)):
+            base = np.floor(x[p] * inv_dx - 0.5).astype(int)
+            fx = x[p] * inv_dx - base
+            w = [0.5 * (1.5 - fx)**2, 0.75 - (fx - 1.0)**2, 0.5 * (fx - 0.5)**2]
+            nv, C = np.zeros(dim), np.zeros((dim, dim))
+            for o in offsets:
+                weight = np.prod([w[o[d]][d] for d in range(dim)])
+                g = grid_in.get(tuple(base + o), np.zeros(dim))
+                nv += weight * g
+                C += 4 * inv_dx * weight * np.outer(g, o - fx)
+            if p >= n_final:
+                nv, C = v[p].copy(), np.zeros((dim, dim))
+            if mat[p] != STATIONARY:
+                v[p] = nv
+                x[p] = x[p] + dt * v[p]
+            base = np.floor(x[p] * inv_dx - 0.5).astype(int)
+            fx = x[p] * inv_dx - base
+            w2 = [0.5 * (1.5 - fx)**2, 0.75 - (fx - 1)**2, 0.5 * (fx - 0.5)**2]
+            F[p], Jp[p], stress = constitutive_g2p2g(F[p], Jp[p], mat[p], C, dt, k, dim)
+            affine = (-dt * k["p_vol"] * 4 * inv_dx**2) * stress + k["p_mass"] * C
+            for o in offsets:
+                weight = np.prod([w2[o[d]][d] for d in range(dim)])
+                node = tuple(base + o)
+                m0, p0 = grid.get(node, (0.0, np.zeros(dim)))
+                grid[node] = (m0 + weight * k["p_mass"],
+                              p0 + weight * (k["p_mass"] * v[p] + affine @ ((o - fx) * dx)))
+        v_allowed = dx * cfl / dt
+        grid_in = {}
+        for node, (m, mom) in grid.items():
+            vel = mom.copy()
+            if m > 0:
+                vel = mom / m + dt * np.asarray(gravity, np.float64)
+            vel = np.minimum(np.maximum(vel, -v_allowed), v_allowed)
+            for d in range(dim):
+                if node[d] < k["padding"] and vel[d] < 0:
+                    vel[d] = 0.0
+                if node[d] >= res[d] - k["padding"] and vel[d] > 0:
+                    vel[d] = 0.0
+            grid_in[node] = vel
+        n_final = len(x)
+        nodes = np.array(sorted(grid), np.int64)
+        out.append((dict(x=x.copy(), v=v.copy(), F=F.copy(), Jp=Jp.copy()), nodes,
+                    np.array([grid[tuple(q)][0] for q in nodes]),
+                    np.array([grid_in[tuple(q)] for q in nodes])))
+    return out
+
+
 def rotations(rng, n, dim):
     Q = np.linalg.qr(rng.normal(size=(n, dim, dim)))[0]
     Q[np.linalg.det(Q) < 0, :, 0] *= -1
     return Q
 
 
-def make(dim, seed, n=64):
+def initial_state(dim, seed, n=64):
     rng = np.random.default_rng(seed)
     res = (32, ) * dim if dim == 3 else (64, 64)
     # float32-representable inputs, so a float32 engine reads the same state
@@ -181,8 +283,11 @@ def make(dim, seed, n=64):
     st = {f: a.astype(np.float32).astype(np.float64) for f, a in
           dict(x=x, v=v, C=C, F=F, Jp=Jp).items()}
     st["material"] = mat.astype(np.int32)
-    dt = 2e-2 / res[0]
-    gravity = [0.0, -9.8, 0.0][:dim]
+    return st, res, 2e-2 / res[0], [0.0, -9.8, 0.0][:dim]
+
+
+def make(dim, seed):
+    st, res, dt, gravity = initial_state(dim, seed)
     out, nodes, gm, gvel = substep(st, res, dt, gravity)
     return dict(res=np.array(res), dt=np.float64(dt), gravity=np.array(gravity),
                 **{f"in_{f}": a for f, a in st.items()},
@@ -190,10 +295,26 @@ def make(dim, seed, n=64):
                 grid_nodes=nodes, grid_m=gm, grid_v=gvel)
 
 
+def make_g2p2g(dim, seed, nsub=3):
+    """Three use_g2p2g substeps; the state after each, and the output grid of
+    the last.  The input velocities are kept below 0.6 cells per substep (a
+    g2p2g particle may move at most one cell)."""
+    st, res, dt, gravity = initial_state(dim, seed)
+    st["v"] = (st["v"] * 0.3).astype(np.float32).astype(np.float64)
+    out = g2p2g_substeps(st, res, dt, gravity, nsub)
+    d = dict(res=np.array(res), dt=np.float64(dt), gravity=np.array(gravity), nsub=np.int64(nsub),
+             **{f"in_{f}": a for f, a in st.items()})
+    for i, (state, nodes, gm, gvel) in enumerate(out):
+        d.update({f"out{i + 1}_{f}": a for f, a in state.items()})
+    d.update(grid_nodes=nodes, grid_m=gm, grid_v=gvel)
+    return d
+
+
 def main():
     here = os.path.dirname(os.path.abspath(__file__))
     for dim, seed in ((2, 20), (3, 30)):
         np.savez(os.path.join(here, f"substep_{dim}d.npz"), **make(dim, seed))
+        np.savez(os.path.join(here, f"g2p2g_{dim}d.npz"), **make_g2p2g(dim, seed + 1))
 
 
 if __name__ == "__main__":

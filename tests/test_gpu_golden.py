@@ -37,3 +37,27 @@ def test_hip_substep_matches_golden(dim):
     for k, e in errs.items():
         assert e <= (3e-5 if k in ("C", "grid_v") else 1e-5), (k, e)
     assert abs(m.sum() - g["grid_m"].sum()) <= 1e-5 * g["grid_m"].sum()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_hip_g2p2g_matches_golden(dim):
+    """Three use_g2p2g substeps against the brute-force fixture after every
+    substep (bar 3e-5: the velocity is gathered from a grid that the
+    stressed states' p2g makes sensitive to rounding, as for C above)."""
+    from taichi_elements_amd import MPMSolver, mpm_solver
+    from tests.test_golden import G2P2G_FIELDS, load_g2p2g
+    mpm_solver.PRINT_PROGRESS = False
+    g = load_g2p2g(dim)
+    res = tuple(int(r) for r in g["res"])
+    s = MPMSolver(res=res, use_g2p2g=True)
+    s.set_gravity(tuple(float(a) for a in g["gravity"]))
+    n = len(g["in_x"])
+    s.read_restart(n, g["in_x"], g["in_v"], g["in_material"], np.zeros(n, np.int32))
+    s.set_state(F=g["in_F"], Jp=g["in_Jp"])
+    for i in range(int(g["nsub"])):
+        s.substep(float(g["dt"]))
+        got = s.get_state()
+        errs = {f: rel_err(got[f], g[f"out{i + 1}_{f}"]) for f in G2P2G_FIELDS}
+        print(dim, i + 1, {k: f"{e:.1e}" for k, e in errs.items()}, flush=True)
+        for k, e in errs.items():
+            assert e <= 3e-5, (i, k, e)
