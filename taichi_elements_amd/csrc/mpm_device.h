@@ -189,6 +189,97 @@ __device__ __forceinline__ void bspline(float f, float w[3]) {
 
 __device__ __forceinline__ int floor_div4(int b) { return b >= 0 ? (b >> 2) : -((-b + 3) >> 2); }
 
+// One pass of the cell-major scatter of p2g and g2p2g.  The lane owns cell
+// lc of the block (its particles share their 27 (9) stencil nodes) and adds
+// the contributions w_o (q + Ad o, mass) of NB of them -- slots k0, k0 +
+// SUBS, ... of its cell's ccnt particles, listed from order[cbeg]; an empty
+// slot adds nothing -- to its private LDS tile tp ((4+2)^d nodes) with one
+// plain read-modify-write per node.  stage[r][pt] holds particle pt's record
+// (3D: Ad row-major, q, fx, mass; 2D: Ad, q, fx, mass).
+template <int D, int NB, int SUBS, int ROW>
+__device__ __forceinline__ void cell_scatter_pass(int k0, int ccnt, int cbeg,
+                                                  const unsigned char* order,
+                                                  const float4 (*stage)[ROW], const int lc[3],
+                                                  float4* tp) {
+  constexpr int kT = kBlock + 2;
+  float Ad[NB][3][3], q[NB][3], mass[NB], w[NB][3][3];
+#pragma unroll
+  for (int bi = 0; bi < NB; bi++) {
+    const int k = k0 + bi * SUBS;
+    float4 s0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), s1 = s0, s2 = s0, s3 = s0;
+    if (k < ccnt) {
+      const int pt = order[cbeg + k];
+      s0 = stage[0][pt];
+      s1 = stage[1][pt];
+      s2 = stage[2][pt];
+      s3 = stage[3][pt];
+    }
+    float fx[3];
+    if (D == 3) {
+      Ad[bi][0][0] = s0.x; Ad[bi][0][1] = s0.y; Ad[bi][0][2] = s0.z;
+      Ad[bi][1][0] = s0.w; Ad[bi][1][1] = s1.x; Ad[bi][1][2] = s1.y;
+      Ad[bi][2][0] = s1.z; Ad[bi][2][1] = s1.w; Ad[bi][2][2] = s2.x;
+      q[bi][0] = s2.y; q[bi][1] = s2.z; q[bi][2] = s2.w;
+      fx[0] = s3.x; fx[1] = s3.y; fx[2] = s3.z;
+      mass[bi] = s3.w;
+    } else {
+      Ad[bi][0][0] = s0.x; Ad[bi][0][1] = s0.y; Ad[bi][1][0] = s0.z; Ad[bi][1][1] = s0.w;
+      Ad[bi][0][2] = Ad[bi][1][2] = Ad[bi][2][0] = Ad[bi][2][1] = Ad[bi][2][2] = 0.0f;
+      q[bi][0] = s1.x; q[bi][1] = s1.y; q[bi][2] = 0.0f;
+      fx[0] = s1.z; fx[1] = s1.w; fx[2] = 0.0f;
+      mass[bi] = s2.x;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) bspline(fx[a], w[bi][a]);
+    if (k >= ccnt) {  // an empty slot contributes exactly nothing
+#pragma unroll
+      for (int a = 0; a < 3; a++) w[bi][a][0] = w[bi][a][1] = w[bi][a][2] = 0.0f;
+    }
+  }
+#pragma unroll
+  for (int oz = 0; oz < (D == 3 ? 3 : 1); oz++)
+#pragma unroll
+    for (int oy = 0; oy < 3; oy++) {
+      float4 line[3];
+#pragma unroll
+      for (int ox = 0; ox < 3; ox++) line[ox] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+      for (int bi = 0; bi < NB; bi++) line_contrib<D>(oy, oz, Ad[bi], q[bi], mass[bi], w[bi], line);
+#pragma unroll
+      for (int ox = 0; ox < 3; ox++) {
+        const float4 add = line[ox];
+        const int node = D == 3 ? (lc[0] + ox) + kT * ((lc[1] + oy) + kT * (lc[2] + oz))
+                                : (lc[0] + ox) + kT * (lc[1] + oy);
+        float4 acc = tp[node];
+        acc.x += add.x;
+        acc.y += add.y;
+        acc.z += add.z;
+        acc.w += add.w;
+        tp[node] = acc;
+        // Lanes of a wave update distinct nodes in one instruction, but a
+        // later node of one lane can be an earlier node of another: keep
+        // every LDS access in program order (LDS serves a wave in order).
+        __asm__ volatile("" ::: "memory");
+      }
+    }
+}
+
+// The scatter of one lane's cell: passes of up to two particles per lane;
+// a pass in which no lane of the wave has a second particle evaluates one
+// slot only (sparse chunks, and the last pass of most dense ones).  The
+// lanes still in the loop are the active ones, so the test is wave-uniform.
+template <int D, int SUBS, int ROW>
+__device__ __forceinline__ void cell_scatter(int sub, int ccnt, int cbeg, const unsigned char* order,
+                                             const float4 (*stage)[ROW], const int lc[3],
+                                             float4* tp) {
+  for (int k0 = sub; k0 < ccnt; k0 += 2 * SUBS) {
+    if (__any(k0 + SUBS < ccnt))
+      cell_scatter_pass<D, 2, SUBS>(k0, ccnt, cbeg, order, stage, lc, tp);
+    else
+      cell_scatter_pass<D, 1, SUBS>(k0, ccnt, cbeg, order, stage, lc, tp);
+  }
+}
+
 // --------------------------------------------------------------------------
 // SVD F = U diag(s) V^T, U, V in SO(d), s decreasing, s[d-1] signed.
 // Replaces ti.svd (mpm_solver.py:525).  3x3: Jacobi eigen-analysis of F^T F

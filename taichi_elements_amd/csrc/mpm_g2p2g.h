@@ -43,7 +43,6 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(
   constexpr int GROUPS = 64 / CELLS;
   constexpr int WAVES = kThreads / 64;
   constexpr int SUBS = WAVES * GROUPS;
-  constexpr int kBatch = 2;
   __shared__ float tin_x[TN], tin_y[TN], tin_z[TN];  // input grid velocity (as k_g2p)
   __shared__ float4 stage[4][kChunkMax];  // [record part][particle]: lane-contiguous
   __shared__ float4 ptile[SUBS][TN];  // inner particles: nodes 1..6 of the output tile
@@ -292,65 +291,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(
       const int lc[3] = {my_cell & 3, (my_cell >> 2) & 3, my_cell >> 4};
       const int cbeg = cstart[my_cell], ccnt = ccount[my_cell];
       float4* tp = ptile[sub];
-      for (int k0 = sub; k0 < ccnt; k0 += SUBS * kBatch) {
-        float Ad[kBatch][3][3], q[kBatch][3], mass[kBatch], w[kBatch][3][3];
-#pragma unroll
-        for (int bi = 0; bi < kBatch; bi++) {
-          const int k = k0 + bi * SUBS;
-          float4 s0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), s1 = s0, s2 = s0, s3 = s0;
-          if (k < ccnt) {
-            const int pt = order[cbeg + k];
-            s0 = stage[0][pt];
-            s1 = stage[1][pt];
-            s2 = stage[2][pt];
-            s3 = stage[3][pt];
-          }
-          float f3[3];
-          if (D == 3) {
-            Ad[bi][0][0] = s0.x; Ad[bi][0][1] = s0.y; Ad[bi][0][2] = s0.z;
-            Ad[bi][1][0] = s0.w; Ad[bi][1][1] = s1.x; Ad[bi][1][2] = s1.y;
-            Ad[bi][2][0] = s1.z; Ad[bi][2][1] = s1.w; Ad[bi][2][2] = s2.x;
-            q[bi][0] = s2.y; q[bi][1] = s2.z; q[bi][2] = s2.w;
-            f3[0] = s3.x; f3[1] = s3.y; f3[2] = s3.z;
-            mass[bi] = s3.w;
-          } else {
-            Ad[bi][0][0] = s0.x; Ad[bi][0][1] = s0.y; Ad[bi][1][0] = s0.z; Ad[bi][1][1] = s0.w;
-            Ad[bi][0][2] = Ad[bi][1][2] = Ad[bi][2][0] = Ad[bi][2][1] = Ad[bi][2][2] = 0.0f;
-            q[bi][0] = s1.x; q[bi][1] = s1.y; q[bi][2] = 0.0f;
-            f3[0] = s1.z; f3[1] = s1.w; f3[2] = 0.0f;
-            mass[bi] = s2.x;
-          }
-#pragma unroll
-          for (int a = 0; a < 3; a++) bspline(f3[a], w[bi][a]);
-          if (k >= ccnt) {
-#pragma unroll
-            for (int a = 0; a < 3; a++) w[bi][a][0] = w[bi][a][1] = w[bi][a][2] = 0.0f;
-          }
-        }
-#pragma unroll
-        for (int oz = 0; oz < (D == 3 ? 3 : 1); oz++)
-#pragma unroll
-          for (int oy = 0; oy < 3; oy++) {
-            float4 line[3];
-#pragma unroll
-            for (int ox = 0; ox < 3; ox++) line[ox] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-#pragma unroll
-            for (int bi = 0; bi < kBatch; bi++) line_contrib<D>(oy, oz, Ad[bi], q[bi], mass[bi], w[bi], line);
-#pragma unroll
-            for (int ox = 0; ox < 3; ox++) {
-              const float4 add = line[ox];
-              const int node = D == 3 ? (lc[0] + ox) + kTile * ((lc[1] + oy) + kTile * (lc[2] + oz))
-                                      : (lc[0] + ox) + kTile * (lc[1] + oy);
-              float4 acc = tp[node];
-              acc.x += add.x;
-              acc.y += add.y;
-              acc.z += add.z;
-              acc.w += add.w;
-              tp[node] = acc;
-              __asm__ volatile("" ::: "memory");  // keep the LDS RMWs in program order
-            }
-          }
-      }
+      cell_scatter<D, SUBS>(sub, ccnt, cbeg, order, stage, lc, tp);
     }
     __syncthreads();
 

@@ -495,7 +495,6 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
   constexpr int GROUPS = 64 / CELLS;             // lane groups per wave
   constexpr int WAVES = kThreads / 64;
   constexpr int SUBS = WAVES * GROUPS;           // private tiles
-  constexpr int kBatch = 2;                      // particles per lane per scatter pass
   __shared__ float4 stage[4][kChunkMax];  // [record part][particle]: lane-contiguous
   __shared__ float4 ptile[SUBS][TN];
   __shared__ int wcnt[WAVES][CELLS];
@@ -657,70 +656,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
       const int lc[3] = {my_cell & 3, (my_cell >> 2) & 3, my_cell >> 4};
       const int cbeg = cstart[my_cell], ccnt = ccount[my_cell];
       float4* tp = ptile[sub];
-      // A lane's particles share its cell, hence its 27 (9) nodes: take them
-      // kBatch at a time and do one read-modify-write per node per batch.
-      for (int k0 = sub; k0 < ccnt; k0 += SUBS * kBatch) {
-        float Ad[kBatch][3][3], q[kBatch][3], mass[kBatch], w[kBatch][3][3];
-#pragma unroll
-        for (int bi = 0; bi < kBatch; bi++) {
-          const int k = k0 + bi * SUBS;
-          float4 s0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), s1 = s0, s2 = s0, s3 = s0;
-          if (k < ccnt) {
-            const int pt = order[cbeg + k];
-            s0 = stage[0][pt];
-            s1 = stage[1][pt];
-            s2 = stage[2][pt];
-            s3 = stage[3][pt];
-          }
-          float fx[3];
-          if (D == 3) {
-            Ad[bi][0][0] = s0.x; Ad[bi][0][1] = s0.y; Ad[bi][0][2] = s0.z;
-            Ad[bi][1][0] = s0.w; Ad[bi][1][1] = s1.x; Ad[bi][1][2] = s1.y;
-            Ad[bi][2][0] = s1.z; Ad[bi][2][1] = s1.w; Ad[bi][2][2] = s2.x;
-            q[bi][0] = s2.y; q[bi][1] = s2.z; q[bi][2] = s2.w;
-            fx[0] = s3.x; fx[1] = s3.y; fx[2] = s3.z;
-            mass[bi] = s3.w;
-          } else {
-            Ad[bi][0][0] = s0.x; Ad[bi][0][1] = s0.y; Ad[bi][1][0] = s0.z; Ad[bi][1][1] = s0.w;
-            Ad[bi][0][2] = Ad[bi][1][2] = Ad[bi][2][0] = Ad[bi][2][1] = Ad[bi][2][2] = 0.0f;
-            q[bi][0] = s1.x; q[bi][1] = s1.y; q[bi][2] = 0.0f;
-            fx[0] = s1.z; fx[1] = s1.w; fx[2] = 0.0f;
-            mass[bi] = s2.x;
-          }
-#pragma unroll
-          for (int a = 0; a < 3; a++) bspline(fx[a], w[bi][a]);
-          if (k >= ccnt) {  // an empty slot contributes exactly nothing
-#pragma unroll
-            for (int a = 0; a < 3; a++) w[bi][a][0] = w[bi][a][1] = w[bi][a][2] = 0.0f;
-          }
-        }
-#pragma unroll
-        for (int oz = 0; oz < (D == 3 ? 3 : 1); oz++)
-#pragma unroll
-          for (int oy = 0; oy < 3; oy++) {
-            float4 line[3];
-#pragma unroll
-            for (int ox = 0; ox < 3; ox++) line[ox] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-#pragma unroll
-            for (int bi = 0; bi < kBatch; bi++) line_contrib<D>(oy, oz, Ad[bi], q[bi], mass[bi], w[bi], line);
-#pragma unroll
-            for (int ox = 0; ox < 3; ox++) {
-              const float4 add = line[ox];
-              const int node = D == 3 ? (lc[0] + ox) + kTile * ((lc[1] + oy) + kTile * (lc[2] + oz))
-                                      : (lc[0] + ox) + kTile * (lc[1] + oy);
-              float4 acc = tp[node];
-              acc.x += add.x;
-              acc.y += add.y;
-              acc.z += add.z;
-              acc.w += add.w;
-              tp[node] = acc;
-              // Lanes of a wave update distinct nodes in one instruction, but a
-              // later node of one lane can be an earlier node of another: keep
-              // every LDS access in program order (LDS serves a wave in order).
-              __asm__ volatile("" ::: "memory");
-            }
-          }
-      }
+      cell_scatter<D, SUBS>(sub, ccnt, cbeg, order, stage, lc, tp);
     }
     lds_barrier();
 
