@@ -12,7 +12,10 @@ particles instead.  Checked:
     without HIP graph replay; the grid read back at the blob's final place
     holds all of its mass;
   * the same flight with use_g2p2g: the grid a g2p2g substep reads is
-    carried into the moved window;
+    carried into the moved window; and with quant=True (the window follows
+    positions decoded from the packed words), x and v within the same bound
+    (measured 2e-5 and 6e-5; F, whose 16-bit codes flip at rounding
+    boundaries, is printed);
   * particles seeded beside the far wall at -grid_size/2 + padding are
     stopped by it as in the oracle (x within 1e-4; v, which the elastic
     bounce makes sensitive to rounding, at most 3x as far from the float64
@@ -117,6 +120,35 @@ def test_blob_leaves_the_domain_with_g2p2g(oracle):
     for f in ("x", "v"):
         assert errs[f] <= DRIFT_TOL, (f, errs[f])
     assert got["x"][:, 0].min() > 3.0
+
+
+def test_packed_blob_leaves_the_domain(oracle):
+    """quant=True: the window follows positions read from the packed x words
+    (21-bit fixed point, |x| < 2).  A 3D blob thrown at 8 units/s leaves
+    [0, 1) and ends near x = 1.6 after 240 substeps; against the oracle with
+    the same storage flags (stored state as input) within the drift bound."""
+    from taichi_elements_amd import MPMSolver
+    dim = 3
+    s = MPMSolver(res=(32, ) * dim, unbounded=True, quant=True)
+    x, vel, mat = _blob(dim, 6000, [8.0, 2.0, -3.0], seed=2)
+    s.read_restart(len(x), x, vel, mat, np.zeros(len(x), np.int32))
+    st = s.get_state()  # the stored (rounded) state is the common input
+    cfg = oracle_cfg_for(oracle, s)
+    assert cfg.quant == 1
+    _set_window(cfg, [-8, -8, -32], [72, 56, 64])
+    ref = oracle.Substepper(cfg, st, np.float32)
+    dt = s.default_dt
+    for _ in range(4):
+        s._run_substeps(60, dt)
+        for _ in range(60):
+            ref.substep(dt)
+    got = s.get_state()
+    errs = {f: rel_err(got[f], ref.s[f]) for f in ("x", "v", "F")}
+    print("unbounded packed vs f32 oracle", {f: f"{e:.2e}" for f, e in errs.items()},
+          "x range", got["x"].min(0), got["x"].max(0), flush=True)
+    assert got["x"][:, 0].min() > 1.2
+    for f in ("x", "v"):
+        assert errs[f] <= DRIFT_TOL, (f, errs[f])
 
 
 @pytest.mark.parametrize("dim", [2, 3])
