@@ -189,6 +189,18 @@ __device__ __forceinline__ void bspline(float f, float w[3]) {
 
 __device__ __forceinline__ int floor_div4(int b) { return b >= 0 ? (b >> 2) : -((-b + 3) >> 2); }
 
+// The cell a lane owns in the cell-major scatter.  3D: lane bits
+// (x, x, z, z, y, y), so the 16 lanes of each quarter-wave hold x = 0..3 and
+// z = 0..3 at one y: their float4 nodes x + 36 z (+ a common offset) of the
+// 6-wide LDS tile are distinct modulo 16 and modulo 8 (36 z = 0, 4, 8, 12
+// mod 16), which keeps the 128-bit tile accesses free of bank conflicts
+// (the x-fastest order put nodes 0..3 and 6..9 in one group: 2-way
+// conflicts).  2D: x fastest, 16 cells per lane group.
+template <int D>
+__device__ __forceinline__ int scatter_cell_of_lane(int lane) {
+  return D == 3 ? (lane & 3) + 4 * ((lane >> 4) + 4 * ((lane >> 2) & 3)) : lane % 16;
+}
+
 // One pass of the cell-major scatter of p2g and g2p2g.  The lane owns cell
 // lc of the block (its particles share their 27 (9) stencil nodes) and adds
 // the contributions w_o (q + Ad o, mass) of NB of them -- slots k0, k0 +

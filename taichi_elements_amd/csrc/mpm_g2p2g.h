@@ -286,7 +286,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(
 
     // ---- phase 3: cell-major scatter of the inner particles ----
     {
-      const int my_cell = lane % CELLS;
+      const int my_cell = scatter_cell_of_lane<D>(lane);
       const int sub = wave * GROUPS + lane / CELLS;
       const int lc[3] = {my_cell & 3, (my_cell >> 2) & 3, my_cell >> 4};
       const int cbeg = cstart[my_cell], ccnt = ccount[my_cell];

@@ -651,7 +651,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
 
     // ---- phase 3: cell-major scatter into private tiles ----
     {
-      const int my_cell = lane % CELLS;
+      const int my_cell = scatter_cell_of_lane<D>(lane);
       const int sub = wave * GROUPS + lane / CELLS;
       const int lc[3] = {my_cell & 3, (my_cell >> 2) & 3, my_cell >> 4};
       const int cbeg = cstart[my_cell], ccnt = ccount[my_cell];
