@@ -7,9 +7,9 @@ when that is smaller.  Checked:
   * compute_max_grid_velocity after a substep equals the max over the
     oracle's grid of |v_i| (relative 1e-5: float32 grids summed in different
     orders) and is 0 before the first substep;
-  * a blob thrown for two frames, its dt limited by the CFL rule to
-    0.81x (p2g, 60 units/s) and 0.88x (use_g2p2g without its velocity
-    clamp, 30 units/s, CFL number 0.5) of the frame's uniform dt: step() runs the same
+  * a blob thrown for two frames (2D and 3D), its dt limited by the CFL
+    rule to 0.81x (p2g, 60 units/s) and 0.88-0.90x (use_g2p2g without its
+    velocity clamp, 30 units/s, CFL number 0.5) of the frame's uniform dt: step() runs the same
     number of substeps with the same dts (relative 1e-5) as the reference's
     loop driven by the oracle's grid, and ends within 1e-4 of the oracle's
     particle state (x, v).
@@ -46,9 +46,9 @@ def _scene(dim, g2p2g, seed=0, clamp=True, cfl=0.9):
     return s, x, v, mat
 
 
-@pytest.mark.parametrize("g2p2g", [False, True])
-def test_max_grid_velocity_matches_the_oracle_grid(oracle, g2p2g):
-    s, x, v, mat = _scene(3, g2p2g)
+@pytest.mark.parametrize("dim,g2p2g", [(2, False), (3, False), (3, True)])
+def test_max_grid_velocity_matches_the_oracle_grid(oracle, dim, g2p2g):
+    s, x, v, mat = _scene(dim, g2p2g)
     assert s.compute_max_grid_velocity() == 0.0
     dt = s.default_dt
     cfg = oracle_cfg_for(oracle, s)
@@ -67,12 +67,12 @@ def test_max_grid_velocity_matches_the_oracle_grid(oracle, g2p2g):
         assert abs(got - want) <= 1e-5 * want
 
 
-@pytest.mark.parametrize("g2p2g", [False, True])
-def test_adaptive_step_follows_the_reference_loop(oracle, g2p2g):
+@pytest.mark.parametrize("dim,g2p2g", [(2, False), (2, True), (3, False), (3, True)])
+def test_adaptive_step_follows_the_reference_loop(oracle, dim, g2p2g):
     # use_g2p2g at 30 units/s with a CFL number of 0.5 and without its
     # velocity clamp (which would hold the grid at cfl dx / dt and so keep
     # the CFL limit from shrinking dt)
-    s, x, v, mat = _scene(3, g2p2g, seed=1, clamp=False, cfl=0.5 if g2p2g else 0.9)
+    s, x, v, mat = _scene(dim, g2p2g, seed=1, clamp=False, cfl=0.5 if g2p2g else 0.9)
     dts = []
     run = s._run_substeps
 
@@ -107,7 +107,7 @@ def test_adaptive_step_follows_the_reference_loop(oracle, g2p2g):
     print(f"g2p2g={g2p2g}: {len(dts)} substeps, dt / uniform dt:",
           " ".join(f"{d / uniform:.3f}" for d in dts), flush=True)
     assert len(dts) == len(ref_dts) == s.total_substeps
-    assert dts[1] < 0.9 * uniform  # the CFL limit is active from the second substep
+    assert dts[1] < 0.95 * uniform  # the CFL limit is active from the second substep
     np.testing.assert_allclose(dts, ref_dts, rtol=1e-5)
     got = s.get_state()
     for f in ("x", "v"):
