@@ -60,6 +60,33 @@ class _Field0D:
         self._set(value)
 
 
+class _ParticleField:
+    """Stands in for one of the reference's per-particle fields (``mpm.x``,
+    ``mpm.v``, ``mpm.C``, ``mpm.F``, ``mpm.Jp``, ``mpm.material``,
+    ``mpm.color``; mpm_solver.py:100-141) for code that reads or writes them
+    directly: ``to_numpy()`` / ``from_numpy()`` over the solver's particles
+    in insertion order (one device transfer each), ``shape``, and
+    ``field[i]`` (a copy of particle i's value)."""
+
+    def __init__(self, solver, name):
+        self._solver = solver
+        self._name = name
+
+    @property
+    def shape(self):
+        _, rank = self._solver._FIELDS[self._name]
+        return self._solver._shape(rank, self._solver.n_particles[None])
+
+    def to_numpy(self):
+        return self._solver.get_state()[self._name]
+
+    def from_numpy(self, array):
+        self._solver.set_state(**{self._name: array})
+
+    def __getitem__(self, i):
+        return self.to_numpy()[i]
+
+
 class _ParticleStorage:
     """Stands in for the reference's particle SNode, whose per-particle size
     the demos print (``mpm.particle._cell_size_bytes``,
@@ -212,6 +239,8 @@ class MPMSolver:
         self._handle = None
         self.n_particles = _Field0D(self._num_particles)
         self.particle = _ParticleStorage(self)
+        for name in self._FIELDS:  # mpm.x, mpm.v, ... as in the reference
+            setattr(self, name, _ParticleField(self, name))
         self.gravity = _Field0D(lambda: tuple(self._gravity), self.set_gravity)
         self.grid_postprocess = []
         self.writers = []

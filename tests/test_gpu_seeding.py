@@ -82,3 +82,23 @@ def test_add_texture_2d():
     assert (info["material"][n0:] == s.material_snow).all()
     assert (info["color"][n0:] == 0xABCDEF).all()
     s.step(1e-3)
+
+
+def test_particle_field_proxies():
+    """mpm.x, mpm.v, mpm.F, ... read and write the particles as the
+    reference's fields do (to_numpy / from_numpy / shape / [i])."""
+    from taichi_elements_amd import MPMSolver
+    s = MPMSolver(res=(32, 32, 32))
+    s.add_cube((0.3, 0.3, 0.3), (0.1, 0.1, 0.1), s.material_snow, color=0x112233,
+               velocity=[1.0, 0.0, -1.0])
+    n = s.n_particles[None]
+    st = s.get_state()
+    assert s.x.shape == (n, 3) and s.F.shape == (n, 3, 3) and s.Jp.shape == (n, )
+    np.testing.assert_array_equal(s.x.to_numpy(), st["x"])
+    np.testing.assert_array_equal(s.v[5], [1.0, 0.0, -1.0])
+    assert (s.material.to_numpy() == s.material_snow).all()
+    assert (s.color.to_numpy() == 0x112233).all()
+    F = np.tile(np.diag([1.01, 0.99, 1.0]).astype(np.float32), (n, 1, 1))
+    s.F.from_numpy(F)
+    np.testing.assert_array_equal(s.F.to_numpy(), F)
+    s.step(1e-3)
