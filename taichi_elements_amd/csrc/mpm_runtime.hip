@@ -161,6 +161,8 @@ struct mpm_solver {
   int64_t substep_counter = 0;
   int* d_step = nullptr;  // device substep counter (max-velocity slot)
   unsigned* d_gvmax = nullptr;  // compute_max_grid_velocity result
+  float* d_aabb = nullptr;      // unbounded: per-workgroup particle boxes (kAabbGroups x 6)
+  std::vector<float> h_aabb;
   // HIP graphs of two steady-state substeps, keyed by the buffer pointers
   // and sizes their kernels were captured with
   bool use_graphs = true;
@@ -736,18 +738,19 @@ int set_window(mpm_solver* s, const int blo[3], const int bnl[3]) {
 // per substep): its block and the next one, `seg + 1` blocks of margin.
 // A window that does not cover that is moved to the particles' box with a
 // margin of twice that on each side, clamped to the walls' extent.
+constexpr int kAabbGroups = 1024;
+
 int follow_particles(mpm_solver* s, int seg) {
   if (!s->unbounded || s->n == 0) return MPM_OK;
   const int D = s->D;
-  const int groups = (int)std::min<int64_t>((s->n + kThreads - 1) / kThreads, 1024);
-  float* part = nullptr;
-  TRY(dalloc(&part, (int64_t)groups * 6));
-  launch_aabb(D, s->quant != 0, s->pf, s->cap, (int)s->n, part, groups, s->stream);
-  std::vector<float> h((size_t)groups * 6);
-  hipError_t e = hipMemcpyAsync(h.data(), part, sizeof(float) * h.size(), hipMemcpyDeviceToHost,
-                                s->stream);
+  const int groups = (int)std::min<int64_t>((s->n + kThreads - 1) / kThreads, kAabbGroups);
+  if (!s->d_aabb) TRY(dalloc(&s->d_aabb, (int64_t)kAabbGroups * 6));  // kept until destroy
+  launch_aabb(D, s->quant != 0, s->pf, s->cap, (int)s->n, s->d_aabb, groups, s->stream);
+  std::vector<float>& h = s->h_aabb;
+  h.resize((size_t)groups * 6);
+  hipError_t e = hipMemcpyAsync(h.data(), s->d_aabb, sizeof(float) * h.size(),
+                                hipMemcpyDeviceToHost, s->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-  dfree(part);
   if (e != hipSuccess) return fail(MPM_EHIP, std::string("particle box: ") + hipGetErrorString(e));
   const SceneHost& sc = s->scenes[0];
   const float inv_dx = (float)sc.cfg.inv_dx;
@@ -909,6 +912,7 @@ int mpm_destroy(mpm_solver* s) {
   s->graphs.clear();
   dfree(s->d_step);
   dfree(s->d_gvmax);
+  dfree(s->d_aabb);
   dfree(s->grid);
   dfree(s->grid_alt);
   dfree(s->totals);
