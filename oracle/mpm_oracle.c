@@ -5,6 +5,7 @@
  */
 #include "mpm_oracle.h"
 
+#include <float.h>
 #include <stdlib.h>
 #include <string.h>
 #include <tgmath.h>
@@ -84,15 +85,19 @@ void oracle_quant_v(int dim, int64_t n, float* v) {
 }
 
 #define REAL float
+#define REAL_MIN_NORMAL FLT_MIN
 #define SUF(x) x##_f32
 #include "mpm_oracle_impl.inc"
 #undef REAL
+#undef REAL_MIN_NORMAL
 #undef SUF
 
 #define REAL double
+#define REAL_MIN_NORMAL DBL_MIN
 #define SUF(x) x##_f64
 #include "mpm_oracle_impl.inc"
 #undef REAL
+#undef REAL_MIN_NORMAL
 #undef SUF
 
 /* build_pid's block of a particle, mpm_solver.py:357-361. */

@@ -223,7 +223,7 @@ __global__ __launch_bounds__(kThreads) void k_p2g_adj(
       const int node = D == 3 ? ln[0] + 4 * (ln[1] + 4 * ln[2]) : ln[0] + 4 * ln[1];
       const float4 f = grid[(int64_t)gb * NPB + node];
       float v[3] = {f.x, f.y, f.z};
-      if (f.w > 0.0f) {
+      if (node_has_mass(f.w)) {
         const float inv = 1.0f / f.w;
 #pragma unroll
         for (int a = 0; a < D; a++) v[a] = v[a] * inv + P.dt * P.gravity[a];

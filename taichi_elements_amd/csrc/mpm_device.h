@@ -201,6 +201,15 @@ __device__ __forceinline__ int scatter_cell_of_lane(int lane) {
   return D == 3 ? (lane & 3) + 4 * ((lane >> 4) + 4 * ((lane >> 2) & 3)) : lane % 16;
 }
 
+// Whether a grid node's mass takes part in the normalisation
+// (mpm_solver.py:591 tests m > 0).  A mass below the smallest normal float
+// comes only from stencil weights of ~1e-15 per axis (a particle within a
+// float step of a (k + 1/2) dx plane, with nothing else on the node); 1/m
+// overflows there and the node's velocity would be infinite.  The reference
+// on GPUs, built with flush-to-zero, reads such a mass as 0 and skips the
+// node; so does this engine.
+__device__ __forceinline__ bool node_has_mass(float m) { return m >= 1.17549435e-38f; }
+
 // One pass of the cell-major scatter of p2g and g2p2g.  The lane owns cell
 // lc of the block (its particles share their 27 (9) stencil nodes) and adds
 // the contributions w_o (q + Ad o, mass) of NB of them -- slots k0, k0 +

@@ -466,7 +466,7 @@ __global__ __launch_bounds__(kThreads) void k_grid_op_g2p2g(
     });
     if (sub != 0) continue;
     float m = sum.w, p[3] = {sum.x, sum.y, sum.z};
-    if (m > 0.0f) {  // mpm_solver.py:591-593
+    if (node_has_mass(m)) {  // mpm_solver.py:591-593
       const float inv = 1.0f / m;
 #pragma unroll
       for (int a = 0; a < D; a++) p[a] = inv * p[a];

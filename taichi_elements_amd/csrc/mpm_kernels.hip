@@ -767,7 +767,7 @@ __global__ __launch_bounds__(kThreads) void k_grid_op(const Params* __restrict__
     if (MODE == 1) {
       const float4 f = grid[(int64_t)bid * NPB + node];
       float pb[3] = {0.0f, 0.0f, 0.0f}, mb = 0.0f;
-      if (f.w > 0.0f) {
+      if (node_has_mass(f.w)) {
         const float inv = 1.0f / f.w;
         const float pf4[3] = {f.x, f.y, f.z};
         float v[3];
@@ -788,7 +788,7 @@ __global__ __launch_bounds__(kThreads) void k_grid_op(const Params* __restrict__
       continue;
     }
     if (!raw) {
-      if (m > 0.0f) {  // mpm_solver.py:591-593
+      if (node_has_mass(m)) {  // mpm_solver.py:591-593
         const float inv = 1.0f / m;
 #pragma unroll
         for (int a = 0; a < D; a++) p[a] = inv * p[a];

@@ -532,3 +532,27 @@ def test_unbounded_walls_are_the_bounded_walls_moved(oracle, side):
     assert np.abs(bounded["x"][:, 0] * res[0]).min() > 1 and (bounded["v"][:, 0] != v[:, 0]).all()
     np.testing.assert_allclose(moved, bounded["x"], rtol=0, atol=1e-9)
     np.testing.assert_allclose(unbounded["v"], bounded["v"], rtol=0, atol=1e-6)
+
+
+def test_subnormal_node_mass_is_an_empty_node(oracle):
+    """mpm_solver.py:591-593 normalises nodes with m > 0 as (1 / m) * p.  A
+    float32 mass below the smallest normal float overflows 1/m; the
+    reference on GPUs (flush-to-zero) reads it as 0 and leaves the node
+    alone, and so do the oracle and the engine: such a node keeps its
+    (subnormal) momentum and gets no gravity, a node of the smallest normal
+    mass is normalised.  float64 normalises both (they are normal there)."""
+    cfg = oracle.make_cfg((16, 16), padding=0, gravity=(0.0, -9.8), post=[])
+    shape = oracle.grid_shape(cfg)
+    dt = 1e-3
+    for dtype in (np.float32, np.float64):
+        gm = np.zeros(shape, dtype)
+        gp = np.zeros(shape + (2, ), dtype)
+        gm[8, 8], gp[8, 8] = 1e-40, (2e-40, -1e-40)        # subnormal in float32
+        gm[8, 9], gp[8, 9] = 1.1754944e-38, (2.3509887e-38, 0)  # smallest normal
+        m2, v2 = oracle.grid_op(cfg, gm, gp, dt, 0.0, dtype)
+        assert np.isfinite(v2).all()
+        np.testing.assert_allclose(v2[8, 9], [2.0, -9.8e-3], rtol=1e-6)
+        if dtype == np.float32:
+            np.testing.assert_array_equal(v2[8, 8], gp[8, 8])
+        else:
+            np.testing.assert_allclose(v2[8, 8], [2.0, -1.0 - 9.8e-3], rtol=1e-12)
