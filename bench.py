@@ -375,9 +375,27 @@ def cpu_baseline(x, v, m, dt, seconds):
         el = time.perf_counter() - t0
         if el >= seconds or k >= 100:
             break
+    # the same restatement on one thread, over a few substeps
+    one = O.Substepper(cfg, st, np.float32, nthreads=1)
+    t1 = time.perf_counter()
+    k1 = 0
+    while True:
+        one.substep(dt)
+        k1 += 1
+        el1 = time.perf_counter() - t1
+        if el1 >= seconds / 4 or k1 >= 10:
+            break
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        pass
     return dict(value=len(x) * k / el, unit="particle-substeps/s", cores=threads, kind="port",
                 sample=f"{k} substeps of the full 1M-particle 128^3 workload "
-                       f"(oracle C restatement, OpenMP atomics, float32), {el:.1f} s")
+                       f"(oracle C restatement, OpenMP atomics, float32), {el:.1f} s",
+                single_thread=dict(value=len(x) * k1 / el1, substeps=k1, seconds=round(el1, 2)),
+                host=dict(cpu_model=model, nproc=os.cpu_count()))
 
 
 def copy_calibration(nbytes=256 << 20):
