@@ -115,7 +115,11 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(
       float nv[3] = {0.0f, 0.0f, 0.0f}, nC[3][3] = {{0.0f}};
       const bool fresh = pi[(int64_t)L::ORIG * cap + p] >= n_final;  // mpm:396-399
       if (!fresh) {  // G2P, mpm:386-394 (tensor-product form of k_g2p)
-        float pe[3][3] = {{0.0f}};
+        float pe[3][3] = {{0.0f}}, wd[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+          for (int k = 0; k < 3; k++) wd[a][k] = a < D ? ((float)k - fx[a]) * w[a][k] : 0.0f;
 #pragma unroll
         for (int oz = 0; oz < (D == 3 ? 3 : 1); oz++) {
           float b0[3] = {0, 0, 0}, bx[3] = {0, 0, 0}, by[3] = {0, 0, 0};
@@ -130,14 +134,14 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(
 #pragma unroll
               for (int cc = 0; cc < D; cc++) {
                 a0[cc] += w[0][ox] * gg[cc];
-                if (ox) ax[cc] += ((float)ox * w[0][ox]) * gg[cc];
+                ax[cc] += wd[0][ox] * gg[cc];
               }
             }
 #pragma unroll
             for (int cc = 0; cc < D; cc++) {
               b0[cc] += w[1][oy] * a0[cc];
               bx[cc] += w[1][oy] * ax[cc];
-              if (oy) by[cc] += ((float)oy * w[1][oy]) * a0[cc];
+              by[cc] += wd[1][oy] * a0[cc];
             }
           }
           const float wz = D == 3 ? w[2][oz] : 1.0f;
@@ -146,13 +150,13 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(
             nv[cc] += wz * b0[cc];
             pe[cc][0] += wz * bx[cc];
             pe[cc][1] += wz * by[cc];
-            if (D == 3 && oz) pe[cc][2] += ((float)oz * wz) * b0[cc];
+            if (D == 3) pe[cc][2] += wd[2][oz] * b0[cc];
           }
         }
 #pragma unroll
         for (int cc = 0; cc < D; cc++)
 #pragma unroll
-          for (int e = 0; e < D; e++) nC[cc][e] = P.c4 * (pe[cc][e] - nv[cc] * fx[e]);
+          for (int e = 0; e < D; e++) nC[cc][e] = P.c4 * pe[cc][e];  // no cancellation (see k_g2p)
       } else {
 #pragma unroll
         for (int a = 0; a < D; a++) nv[a] = vp[a];

@@ -889,10 +889,17 @@ __global__ __launch_bounds__(kThreads) void k_g2p(const Params* __restrict__ sce
       }
       // v = sum_o w_o g_o and C = 4 inv_dx sum_o w_o g_o (o - fx)^T
       // (mpm_solver.py:714-721), evaluated through the tensor-product
-      // weights w_o = wx[ox] wy[oy] wz[oz]: with P_e = sum_o w_o g_o o_e,
-      // C = 4 inv_dx (P - v fx^T).  One pass over the 27 nodes, per-axis
-      // partial sums instead of a 3x3 outer product per node.
+      // weights w_o = wx[ox] wy[oy] wz[oz]: P_e = sum_o w_o g_o (o_e - fx_e)
+      // with the factor (o_e - fx_e) folded into axis e's weights (wd), so
+      // C = 4 inv_dx P needs no subtraction of two nearly equal sums.  One
+      // pass over the 27 nodes, per-axis partial sums instead of a 3x3 outer
+      // product per node.
       float nv[3] = {0.0f, 0.0f, 0.0f}, pe[3][3] = {{0.0f}}, nC[3][3];
+      float wd[3][3];
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) wd[a][k] = a < D ? ((float)k - fx[a]) * w[a][k] : 0.0f;
       if (inside && !skip) {
 #pragma unroll
         for (int oz = 0; oz < (D == 3 ? 3 : 1); oz++) {
@@ -909,14 +916,14 @@ __global__ __launch_bounds__(kThreads) void k_g2p(const Params* __restrict__ sce
               for (int c = 0; c < D; c++) {
                 const float g = gg[c];
                 a0[c] += w[0][ox] * g;
-                if (ox) ax[c] += ((float)ox * w[0][ox]) * g;
+                ax[c] += wd[0][ox] * g;
               }
             }
 #pragma unroll
             for (int c = 0; c < D; c++) {
               b0[c] += w[1][oy] * a0[c];
               bx[c] += w[1][oy] * ax[c];
-              if (oy) by[c] += ((float)oy * w[1][oy]) * a0[c];
+              by[c] += wd[1][oy] * a0[c];
             }
           }
           const float wz = D == 3 ? w[2][oz] : 1.0f;
@@ -925,14 +932,14 @@ __global__ __launch_bounds__(kThreads) void k_g2p(const Params* __restrict__ sce
             nv[c] += wz * b0[c];
             pe[c][0] += wz * bx[c];
             pe[c][1] += wz * by[c];
-            if (D == 3 && oz) pe[c][2] += ((float)oz * wz) * b0[c];
+            if (D == 3) pe[c][2] += wd[2][oz] * b0[c];
           }
         }
       }
 #pragma unroll
       for (int c = 0; c < D; c++)
 #pragma unroll
-        for (int e = 0; e < D; e++) nC[c][e] = P.c4 * (pe[c][e] - nv[c] * fx[e]);
+        for (int e = 0; e < D; e++) nC[c][e] = P.c4 * pe[c][e];
       float xn[3] = {xp[0], xp[1], xp[2]};
       if (skip) {
       } else if (mat != kStationary) {  // mpm_solver.py:722-724
