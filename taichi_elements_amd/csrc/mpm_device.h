@@ -383,7 +383,11 @@ __host__ __device__ __forceinline__ void givens_qr(float B[3][3], float U[3][3])
   float r2 = a * a + b * b;
   float c = 1.0f, s = 0.0f;
   if (r2 > 1e-30f) {
-    float ir = fast_rsqrt(r2);
+    // a correctly rounded 1 / sqrt: the hardware estimate's last-bit error
+    // leaves c^2 + s^2 off 1 in a biased way, so U and R = U V^T shrink or
+    // grow by ~1e-7, which 2 mu (F - R) F^T turns into stress at F ~ I
+    // (2D snow falling freely drifted 4e-4 from the float64 oracle with it)
+    float ir = 1.0f / sqrtf(r2);
     c = a * ir;
     s = b * ir;
   }
