@@ -48,12 +48,14 @@ def make_scene(seed=1):
     return np.concatenate(xs), np.concatenate(vs), np.concatenate(ms)
 
 
-def make_sparse_scene(seed=2, total=16_000_000, res=512):
+def make_sparse_scene(seed=2, total=16_000_000, res=512, parts=False):
     """64 ellipsoids, centres U[0.1, 0.9]^3, semi-axes U[0.02, 0.08]^3, random
     material in {water, elastic, snow, sand}; per-ellipsoid counts from
     add_ellipsoid's rule at 8 particles per cell (mpm_solver.py:997-1005),
     scaled so that the total is exactly `total`; uniform interior sampling by
-    rejection in the unit ball (mpm_solver.py:959-967); zero velocity."""
+    rejection in the unit ball (mpm_solver.py:959-967); zero velocity.
+    parts=True also returns the ellipsoids (centres, radii, materials,
+    particle counts, in the order their particles appear)."""
     rng = np.random.default_rng(seed)
     centres = rng.uniform(0.1, 0.9, (64, 3))
     radii = rng.uniform(0.02, 0.08, (64, 3))
@@ -73,6 +75,9 @@ def make_sparse_scene(seed=2, total=16_000_000, res=512):
             got += len(cand)
         m[at:at + k] = mat
         at += k
+    if parts:
+        return x, np.zeros_like(x), m, dict(centres=centres, radii=radii, materials=mats,
+                                            counts=counts)
     return x, np.zeros_like(x), m
 
 
@@ -193,6 +198,25 @@ def batched_bench(steps, warmup, autotune=True):
     return out
 
 
+def make_diff_scene(seed=4, n_el=100_000, n_wa=100_000):
+    """The differentiable workload's particles: an elastic cube (lower corner
+    (0.25, 0.2, 0.25), side 0.23) with E ~ U[5e5, 2e6] per particle and a
+    water cube beside it (lower corner (0.48, 0.2, 0.25)), v ~ U[-0.5, 0.5]^3;
+    the target of the loss is the elastic body's centre of mass moved by
+    U[-0.05, 0.05]^3.  Returns x, v, material, E, target, n_el."""
+    rng = np.random.default_rng(seed)
+    x_el = (np.array([0.25, 0.2, 0.25], np.float32)
+            + rng.random((n_el, 3), dtype=np.float32) * np.float32(0.23))
+    x_wa = (np.array([0.48, 0.2, 0.25], np.float32)
+            + rng.random((n_wa, 3), dtype=np.float32) * np.float32(0.23))
+    x = np.concatenate([x_el, x_wa])
+    mat = np.concatenate([np.full(n_el, 1, np.int32), np.zeros(n_wa, np.int32)])
+    v = rng.uniform(-0.5, 0.5, x.shape).astype(np.float32)
+    E = np.concatenate([rng.uniform(5e5, 2e6, n_el), np.full(n_wa, 1e6)]).astype(np.float32)
+    target = x_el.mean(0) + rng.uniform(-0.05, 0.05, 3).astype(np.float32)
+    return x, v, mat, E, target, n_el
+
+
 def fwd_bwd_bench(nsub=64, checkpoint_every=None, repeats=3, seed=4):
     """Differentiable workload: 128^3 grid, an elastic cube of 100k particles
     with per-particle E ~ U[5e5, 2e6] and a water cube of 100k particles
@@ -205,18 +229,8 @@ def fwd_bwd_bench(nsub=64, checkpoint_every=None, repeats=3, seed=4):
     forward alone (no autograd) and forward + backward."""
     import torch  # before the engine library loads: one HIP runtime
     from taichi_elements_amd.diff import DiffMPMSolver
-    rng = np.random.default_rng(seed)
-    n_el = n_wa = 100_000
-    x_el = (np.array([0.25, 0.2, 0.25], np.float32)
-            + rng.random((n_el, 3), dtype=np.float32) * np.float32(0.23))
-    x_wa = (np.array([0.48, 0.2, 0.25], np.float32)
-            + rng.random((n_wa, 3), dtype=np.float32) * np.float32(0.23))
-    x = np.concatenate([x_el, x_wa])
-    mat = np.concatenate([np.full(n_el, 1, np.int32), np.zeros(n_wa, np.int32)])
-    v = rng.uniform(-0.5, 0.5, x.shape).astype(np.float32)
-    E = np.concatenate([rng.uniform(5e5, 2e6, n_el), np.full(n_wa, 1e6)]).astype(np.float32)
-    target = torch.tensor(x_el.mean(0) + rng.uniform(-0.05, 0.05, 3).astype(np.float32),
-                          device="cuda")
+    x, v, mat, E, target, n_el = make_diff_scene(seed)
+    target = torch.tensor(target, device="cuda")
     ds = DiffMPMSolver((128, 128, 128))
     ds.setup(mat, x)
     s0 = ds.state(x, v)
@@ -413,13 +427,93 @@ def copy_calibration(nbytes=256 << 20):
     return nbytes
 
 
-def measured_traffic():
-    """HBM bytes per p2g launch from the committed rocprofv3 counter summary."""
+def measured_traffic(config):
+    """HBM bytes per p2g launch of the workload `config` (dense1m or
+    sparse16m) from the committed rocprofv3 counter summary, or None."""
     path = os.path.join(ROOT, "profiles", "p2g_3d_1M_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("p2g_hbm_bytes_per_launch")
+        prof = json.load(f)
+    if config == "sparse16m":
+        prof = prof.get("sparse16m", {})
+    return prof.get("p2g_hbm_bytes_per_launch")
+
+
+def plain_leg(config, x, v, m, steps, warmup, autotune, reorder_interval=0, chunk=0):
+    """The plain substep (binning, p2g, grid operator, g2p) of one workload:
+    dense1m (128^3, 1M particles in four cubes) or sparse16m (512^3 window,
+    16M particles in 64 ellipsoids, one slip plane at y = 0.05).  Wall-clock
+    particle-substeps/s over `steps` substeps, then the same substeps with
+    HIP events around each kernel on the solver's stream for the roofline."""
+    from taichi_elements_amd import MPMSolver
+    from taichi_elements_amd._lib import check
+    from taichi_elements_amd._lib import lib as _l
+    L = _l()
+    res = 512 if config == "sparse16m" else RES
+    n = len(x)
+    s = MPMSolver(res=(res, res, res), reorder_interval=reorder_interval, chunk=chunk)
+    if config == "sparse16m":
+        s.add_surface_collider((0.0, 0.05, 0.0), (0.0, 1.0, 0.0), surface=s.surface_slip)
+    s.read_restart(n, x, v, m, np.zeros(n, np.int32))
+    dt = s.default_dt
+    tuning = None
+    if autotune and not chunk:
+        cand = s.autotune(substeps=3)
+        tuning = dict(chosen=dict(zip(("chunk", "groups"), s.tuning())),
+                      ms_per_substep={f"{c}/{g}": round(t, 4) for (c, g), t in cand.items()})
+        print(f"[bench] {config}: autotuned launch shape {tuning['chosen']}", file=sys.stderr,
+              flush=True)
+
+    def run(k):
+        check(L.mpm_substeps(s._handle, k, dt, 0.0))
+        check(L.mpm_sync(s._handle))
+
+    print(f"[bench] {config}: {n} particles; warmup {warmup}", file=sys.stderr, flush=True)
+    run(warmup)
+    t0 = time.perf_counter()
+    run(steps)
+    t1 = time.perf_counter()
+    ms_per_step = 1000 * (t1 - t0) / steps
+    counts = s.last_counts()
+    n_nodes = 64 * counts["active_blocks"]
+    print(f"[bench] {config}: timed {steps} substeps: {ms_per_step:.4f} ms/substep",
+          file=sys.stderr, flush=True)
+    check(L.mpm_set_timing(s._handle, 1))
+    t2 = time.perf_counter()
+    run(steps)
+    t3 = time.perf_counter()
+    names, calls, ms = s.kernel_times()
+    check(L.mpm_set_timing(s._handle, 0))
+    kern = {nm: dict(calls=c, avg_us=1000 * t / max(c, 1)) for nm, c, t in zip(names, calls, ms)}
+    ab = algorithmic_bytes(counts["valid"], n_nodes)
+    achieved = ab["p2g"] / (kern["k_p2g"]["avg_us"] * 1e-6) / 1e9
+    substep_gbs = ab["substep"] / (ms_per_step * 1e-3) / 1e9
+    if config == "sparse16m":
+        data = "synthetic (seeded numpy: 64 ellipsoids, 16M particles, at rest)"
+        workload = ("3D sparse large domain, 512^3 window of 4^3 blocks, 16M particles in 64 "
+                    "random ellipsoids (8 per cell), plane collider, dt=default_dt")
+    else:
+        data = "synthetic (seeded numpy: 4 cubes x 250k particles, v ~ U[-1,1]^3)"
+        workload = ("3D four-material plain substep, 128^3 dense grid, dx=1/128, 1M particles "
+                    "(water/elastic/snow/sand cubes of side 0.2), dt=default_dt")
+    out = dict(
+        value=n * steps / (t1 - t0), unit="particle-substeps/s", ms_per_substep=ms_per_step,
+        steps=steps, dt=dt, data=data,
+        config={"workload": workload, "res": res, "particles": n,
+                "active_grid_blocks": counts["active_blocks"],
+                "particle_blocks": counts["particle_blocks"], "chunks": counts["chunks"]},
+        roofline={"bound": "hbm", "kernel": "k_p2g", "achieved": achieved,
+                  "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                  "traffic": measured_traffic(config),
+                  "algorithmic_bytes_per_launch": ab["p2g"]},
+        substep_hbm={"algorithmic_bytes": ab["substep"], "achieved_GBs": substep_gbs,
+                     "frac": substep_gbs / HBM_PEAK_GBS},
+        kernels_us={k: round(v_["avg_us"], 2) for k, v_ in kern.items()},
+        timed_with_events_ms_per_step=1000 * (t3 - t2) / steps,
+        tuning=tuning)
+    del s
+    return out
 
 
 def main():
@@ -450,6 +544,10 @@ def main():
                     help="skip the use_g2p2g measurement")
     ap.add_argument("--no-fwd-bwd", action="store_true",
                     help="skip the differentiable (forward + backward) measurement")
+    ap.add_argument("--no-sparse", action="store_true",
+                    help="skip the sparse 512^3 / 16M-particle measurement")
+    ap.add_argument("--sparse-steps", type=int, default=50,
+                    help="timed substeps of the sparse leg (its configuration runs 50)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device-to-device copy (hipMemcpy) so that "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE can be scaled by a kernel of "
@@ -494,92 +592,35 @@ def main():
         # torch (used by the differentiable leg) must be imported before the
         # engine library loads, so that the process runs one HIP runtime
         import torch  # noqa: F401
-    from taichi_elements_amd import MPMSolver, mpm_solver
+    from taichi_elements_amd import mpm_solver
     mpm_solver.PRINT_PROGRESS = False
-    res = RES
     if args.config == "sparse16m":
-        res = 512
         x, v, m = make_sparse_scene()
     else:
         x, v, m = make_scene()
-    n = len(x)
-    s = MPMSolver(res=(res, res, res), reorder_interval=args.reorder_interval, chunk=args.chunk)
-    if args.config == "sparse16m":
-        s.add_surface_collider((0.0, 0.05, 0.0), (0.0, 1.0, 0.0), surface=s.surface_slip)
-    s.read_restart(n, x, v, m, np.zeros(n, np.int32))
-    dt = s.default_dt
-    L = __import__("taichi_elements_amd._lib", fromlist=["lib"]).lib()
-    tuning = None
-    if not args.no_autotune and not args.chunk:
-        cand = s.autotune(substeps=3)
-        tuning = dict(chosen=dict(zip(("chunk", "groups"), s.tuning())),
-                      ms_per_substep={f"{c}/{g}": round(v, 4) for (c, g), v in cand.items()})
-        print(f"[bench] autotuned launch shape: {tuning['chosen']}", file=sys.stderr, flush=True)
-
-    def run(k):
-        from taichi_elements_amd._lib import check
-        check(L.mpm_substeps(s._handle, k, dt, 0.0))
-        check(L.mpm_sync(s._handle))
-
-    print(f"[bench] scene ready: {n} particles; warmup {args.warmup}", file=sys.stderr, flush=True)
-    run(args.warmup)
-    t0 = time.perf_counter()
-    run(args.steps)
-    t1 = time.perf_counter()
-    ms_per_step = 1000 * (t1 - t0) / args.steps
-    value = n * args.steps / (t1 - t0)
-    counts = s.last_counts()
-    n_nodes = 64 * counts["active_blocks"]
-    print(f"[bench] timed {args.steps} substeps: {ms_per_step:.4f} ms/substep", file=sys.stderr,
-          flush=True)
-
-    # per-kernel durations with HIP events on the solver's stream
-    from taichi_elements_amd._lib import check
-    check(L.mpm_set_timing(s._handle, 1))
-    t2 = time.perf_counter()
-    run(args.steps)
-    t3 = time.perf_counter()
-    names, calls, ms = s.kernel_times()
-    check(L.mpm_set_timing(s._handle, 0))
-    kern = {nm: dict(calls=c, avg_us=1000 * t / max(c, 1)) for nm, c, t in zip(names, calls, ms)}
-    ab = algorithmic_bytes(counts["valid"], n_nodes)
-    p2g_s = kern["k_p2g"]["avg_us"] * 1e-6
-    achieved = ab["p2g"] / p2g_s / 1e9
-    substep_gbs = ab["substep"] / (ms_per_step * 1e-3) / 1e9
-
+    leg = plain_leg(args.config, x, v, m, args.steps, args.warmup, not args.no_autotune,
+                    args.reorder_interval, args.chunk)
+    dt = leg.pop("dt")
     out = {
         "metric": "particle-substeps/s (3D 1M, 4-material): plain/batched/fwd+bwd; % HBM peak",
-        "value": value,
+        "value": leg["value"],
         "unit": "particle-substeps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": leg["ms_per_substep"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": ("synthetic (seeded numpy: 4 cubes x 250k particles, v ~ U[-1,1]^3)"
-                 if args.config == "dense1m" else
-                 "synthetic (seeded numpy: 64 ellipsoids, 16M particles, at rest)"),
-        "config": {"workload": ("3D four-material plain substep, 128^3 dense grid, dx=1/128, "
-                                "1M particles (water/elastic/snow/sand cubes of side 0.2), "
-                                "dt=default_dt") if args.config == "dense1m" else
-                               ("3D sparse large domain, 512^3 window of 4^3 blocks, 16M "
-                                "particles in 64 random ellipsoids (8 per cell), plane "
-                                "collider, dt=default_dt"),
-                   "res": res, "particles": n,
-                   "active_grid_blocks": counts["active_blocks"],
-                   "particle_blocks": counts["particle_blocks"], "chunks": counts["chunks"]},
-        "roofline": {"bound": "hbm", "kernel": "k_p2g", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(),
-                     "algorithmic_bytes_per_launch": ab["p2g"]},
-        "substep_hbm": {"algorithmic_bytes": ab["substep"], "achieved_GBs": substep_gbs,
-                        "frac": substep_gbs / HBM_PEAK_GBS},
-        "kernels_us": {k: round(v_["avg_us"], 2) for k, v_ in kern.items()},
-        "timed_with_events_ms_per_step": 1000 * (t3 - t2) / args.steps,
-        "tuning": tuning,
+        "data": leg.pop("data"),
+        "config": leg.pop("config"),
+        "roofline": leg.pop("roofline"),
+        "substep_hbm": leg.pop("substep_hbm"),
+        "kernels_us": leg.pop("kernels_us"),
+        "timed_with_events_ms_per_step": leg.pop("timed_with_events_ms_per_step"),
+        "tuning": leg.pop("tuning"),
+        "sparse16m": None,
         "g2p2g": None,
         "quant": None,
         "small2d": None,
@@ -591,7 +632,6 @@ def main():
     if args.config == "dense1m" and not args.no_small2d:
         out["small2d"] = small2d_bench()
     if not args.no_g2p2g and args.config == "dense1m":
-        del s
         out["g2p2g"] = g2p2g_bench(x, v, m, args.steps, args.warmup, not args.no_autotune)
     if not args.no_quant and args.config == "dense1m":
         out["quant"] = quant_bench(x, v, m, args.steps, args.warmup, not args.no_autotune)
@@ -599,6 +639,17 @@ def main():
         out["batched"] = batched_bench(args.steps, args.warmup, not args.no_autotune)
     if not args.no_fwd_bwd and args.config == "dense1m":
         out["fwd_bwd"] = fwd_bwd_bench()
+    if not args.no_sparse and args.config == "dense1m":
+        # the DRAM-honest workload: its 2.5 GB of particle and grid traffic per
+        # substep does not fit in the 256 MB last-level cache as the 1M one does
+        print("[bench] sparse16m", file=sys.stderr, flush=True)
+        xs, vs, ms = make_sparse_scene()
+        sp = plain_leg("sparse16m", xs, vs, ms, args.sparse_steps, args.warmup,
+                       not args.no_autotune, args.reorder_interval, args.chunk)
+        sp.pop("dt")
+        sp.pop("data")
+        out["sparse16m"] = sp
+        del xs, vs, ms
     if not args.no_cpu_baseline and args.config == "dense1m":
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(x, v, m, dt, args.cpu_seconds)

@@ -191,6 +191,11 @@ int mpm_kernel_times(mpm_solver* s, char* names, int32_t names_cap, int32_t* cou
  * evaluated on the host CPU with the same code the kernels inline, for tests:
  * A = U diag(sig) V^T, row-major dim x dim. */
 int mpm_host_svd(int32_t dim, const float* A, float* U, float* sig, float* V);
+/* The same SVD as the kernels run it on the GPU of `device` (where the
+ * Jacobi sweeps use the hardware reciprocal square root), for n matrices:
+ * A, U, V (n, dim, dim) and sig (n, dim), host arrays.  Synchronous. */
+int mpm_device_svd(int32_t device, int32_t dim, int64_t n, const float* A, float* U, float* sig,
+                   float* V);
 
 /* use_g2p2g mode (MPMSolver(use_g2p2g=True, v_clamp_g2p2g, g2p2g_allowed_cfl),
  * mpm_solver.py:363-485, 586-598, 773-787): every substep is one fused
@@ -219,6 +224,20 @@ int mpm_set_graphs(mpm_solver* s, int32_t enabled);
 int mpm_set_tuning(mpm_solver* s, int32_t chunk, int32_t groups);
 int mpm_get_tuning(mpm_solver* s, int32_t* chunk, int32_t* groups);
 
+/* Device-side snapshot of the simulated state (no reference counterpart;
+ * MPMSolver.autotune uses it to put the scene back after timing launch
+ * shapes).  op = MPM_SNAPSHOT_SAVE copies, on the device, every particle
+ * slot and, in use_g2p2g mode, the grid the next substep's G2P reads and
+ * which particles are new; MPM_SNAPSHOT_RESTORE puts them back (the next
+ * substep re-bins from scratch), and fails with MPM_EINVAL when particles
+ * were added or the storage mode changed since the save; an unbounded
+ * scene's block window is moved back first.  MPM_SNAPSHOT_RELEASE frees the
+ * copy.  Colours, gravity, post-processes and tuning are not part of it. */
+#define MPM_SNAPSHOT_SAVE 0
+#define MPM_SNAPSHOT_RESTORE 1
+#define MPM_SNAPSHOT_RELEASE 2
+int mpm_snapshot(mpm_solver* s, int32_t op);
+
 /* Bookkeeping of the last substep: number of valid particles, transfer
  * work items (chunks), active grid blocks, and particle blocks. */
 int mpm_last_counts(mpm_solver* s, int64_t* out4);
@@ -231,8 +250,13 @@ int mpm_last_counts(mpm_solver* s, int64_t* out4);
  * the per-particle Young's modulus E_p (device, n floats) and nu the global
  * Poisson ratio: mu_0 = E_p / (2 (1 + nu)), lambda_0 = E_p nu / ((1 + nu)(1 -
  * 2 nu)), scaled by h = 0.3 for elastic particles as the reference does.
- * Materials are water and elastic; the post-processes must be the bounding
- * box only.  stream: the hipStream_t to enqueue on (NULL: the null stream,
+ * Materials are water, elastic, snow (clamp and hardening) and sand
+ * (Drucker-Prager return map, whose coefficient (d lambda_0 + 2 mu_0) /
+ * (2 mu_0) follows nu); stationary particles are refused.  Every
+ * post-process (bounding box, surface and sphere colliders) has its adjoint;
+ * the derivative is taken on the branch the forward took (the clamp, the
+ * cone and the colliders' contact tests are kinks).  stream: the
+ * hipStream_t to enqueue on (NULL: the null stream,
  * e.g. torch's default stream).  The handle must hold no particles of its
  * own.
  * mpm_diff_setup: n particles with these host materials.

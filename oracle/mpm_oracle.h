@@ -105,6 +105,10 @@ void oracle_sand_projection_f32(int dim, float* sig, float* Jp, double mu_0,
 void oracle_constitutive_f32(const oracle_cfg* cfg, int material, float dt,
                              float* F, const float* C, float* Jp,
                              float* stress);
+/* The same update inside g2p2g (mpm:411-473). */
+void oracle_constitutive_g2p2g_f32(const oracle_cfg* cfg, int material, float dt,
+                                    float* F, const float* C, float* Jp,
+                                    float* stress);
 void oracle_p2g_f32(const oracle_cfg* cfg, int64_t n, const float* x,
                     const float* v, const float* C, float* F, float* Jp,
                     const int32_t* material, float dt, float* grid_m,
@@ -133,6 +137,10 @@ void oracle_sand_projection_f64(int dim, double* sig, double* Jp, double mu_0,
 void oracle_constitutive_f64(const oracle_cfg* cfg, int material, double dt,
                              double* F, const double* C, double* Jp,
                              double* stress);
+/* The same update inside g2p2g (mpm:411-473). */
+void oracle_constitutive_g2p2g_f64(const oracle_cfg* cfg, int material, double dt,
+                                    double* F, const double* C, double* Jp,
+                                    double* stress);
 void oracle_p2g_f64(const oracle_cfg* cfg, int64_t n, const double* x,
                     const double* v, const double* C, double* F, double* Jp,
                     const int32_t* material, double dt, double* grid_m,

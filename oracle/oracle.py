@@ -180,14 +180,16 @@ def sand_projection(sig, Jp, mu_0, lambda_0, alpha, dtype=np.float32):
     return s, float(j[0])
 
 
-def constitutive(cfg, material, dt, F, C, Jp, dtype=np.float32):
+def constitutive(cfg, material, dt, F, C, Jp, dtype=np.float32, g2p2g=0):
+    """Per-particle constitutive update of p2g (mpm:507-568), or, with
+    g2p2g=1, as g2p2g performs it (mpm:411-473)."""
     npt, ct, suf = _types(dtype)
     d = cfg.dim
     F = np.ascontiguousarray(F, dtype=npt).copy()
     C = np.ascontiguousarray(C, dtype=npt)
     j = np.array([Jp], dtype=npt)
     st = np.zeros((d, d), npt)
-    fn = getattr(lib(), "oracle_constitutive_" + suf)
+    fn = getattr(lib(), ("oracle_constitutive_g2p2g_" if g2p2g else "oracle_constitutive_") + suf)
     fn(ctypes.byref(cfg), ctypes.c_int(material), ct(dt), _ptr(F, ct), _ptr(C, ct),
        _ptr(j, ct), _ptr(st, ct))
     return F, float(j[0]), st

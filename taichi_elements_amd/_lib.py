@@ -65,6 +65,9 @@ SIGNATURES = [
     ("mpm_set_graphs", ctypes.c_int, [_P, ctypes.c_int32]),
     ("mpm_set_tuning", ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),
     ("mpm_get_tuning", ctypes.c_int, [_P, _I32P, _I32P]),
+    ("mpm_snapshot", ctypes.c_int, [_P, ctypes.c_int32]),
+    ("mpm_device_svd", ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, _FP, _FP,
+                                      _FP, _FP]),
     ("mpm_diff_setup", ctypes.c_int, [_P, ctypes.c_int64, _I32P]),
     ("mpm_diff_binning_bytes", ctypes.c_int, [_P, _I64P]),
     ("mpm_diff_substep", ctypes.c_int, [_P, ctypes.c_float, _P, _P, _P, ctypes.c_double, _P,

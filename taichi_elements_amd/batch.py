@@ -306,12 +306,8 @@ class MPMSolverBatch:
         default_dt); the state of every scene is restored afterwards."""
         from .mpm_solver import _autotune
 
-        def restore(st):
-            for name, a in st.items():
-                self._set_state_all(name, a)
-
-        return _autotune(self, self._handle, None, self._get_state_all, restore, substeps,
-                         chunks, groups, verbose, self.default_dts)
+        return _autotune(self, self._handle, None, substeps, chunks, groups, verbose,
+                         self.default_dts)
 
     def kernel_times(self):
         return MPMSolver.kernel_times(self)

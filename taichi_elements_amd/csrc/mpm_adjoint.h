@@ -28,6 +28,23 @@
 //            Jpbar = Jbar det K, Cbar += Jbar Jp dt cof(K), Fbar = 0.
 // Material parameters: mu_0 = E kmu(nu), lambda_0 = E kla(nu), times
 // h = 0.3 for elastic (mpm_solver.py:519-524), giving Ebar and nubar.
+//
+// Snow and sand (plastic_adjoint below).  Both store F = U diag(sh) V^T
+// with sh a function of the singular values sig of F' = K F (snow: clamped,
+// mpm:527-545; sand: the return map, mpm:321-342, 552-556), and both
+// stresses are tau = U diag(t) U^T (snow: t_i = 2 mu (sh_i - 1) sh_i +
+// la J (J - 1), J = prod sh; sand: t_i = 2 mu_0 l_i + lambda_0 sum l,
+// l = ln sh).  With M = U^T Fbar_out V and N = sym(U^T taubar U), the
+// adjoint of F' is U X V^T where, for i != j,
+//   X_ij = sym(M)_ij Dg_ij + skew(M)_ij (sh_i + sh_j) / (sig_i + sig_j)
+//          + 2 N_ij sig_j Dt_ij / (sig_i + sig_j),
+// Dg_ij = (sh_j - sh_i) / (sig_j - sig_i) and Dt_ij = (t_j - t_i) /
+// (sig_j - sig_i) being divided differences taken in closed form (no
+// 1 / (sig_i^2 - sig_j^2) of the SVD's own derivative, so repeated singular
+// values, F' = I included, are harmless), and X_ii = sigbar_i, the adjoint
+// of the singular values through sh, t and Jp.  These were checked against
+// float64 finite differences of the forward in every branch (snow inside /
+// at its clamp; sand at the cone tip, inside the cone, projected).
 #pragma once
 
 #include "mpm_device.h"
@@ -134,6 +151,218 @@ __device__ __forceinline__ void recompute(const Params& P, int mat, const float 
   }
 }
 
+// expm1(a d) / expm1(d), continuous through d = 0 (value a)
+__device__ __forceinline__ float ratio_expm1(float a, float d) {
+  if (fabsf(d) < 1e-4f) return a * (1.0f + 0.5f * (a - 1.0f) * d);
+  return expm1f(a * d) / expm1f(d);
+}
+
+// d / expm1(d), continuous through d = 0 (value 1)
+__device__ __forceinline__ float ratio_lin_expm1(float d) {
+  if (fabsf(d) < 1e-4f) return 1.0f - 0.5f * d;
+  return d / expm1f(d);
+}
+
+// Constitutive adjoint of one snow or sand particle (see the header): given
+// F' = K F, Jp, E and the adjoints G = taubar, Fbo = Fbar_out, Jpbo =
+// Jpbar_out, returns Fpbar (the adjoint of F'), Jpbar, and the adjoints of
+// mu_0 and lambda_0 (and, for sand, of the return map's coefficient
+// c = sand_coef alpha, through cbar).
+template <int D>
+__device__ __forceinline__ void plastic_adjoint(const Params& P, int mat, const float Fp[3][3],
+                                                float Jp, float mu0, float la0,
+                                                const float G[3][3], const float Fbo[3][3],
+                                                float Jpbo, float Fpb[3][3], float& Jpbar,
+                                                float& mu0b, float& la0b, float& cbar) {
+  float U[3][3], V[3][3], sig[3] = {1.0f, 1.0f, 1.0f};
+  if (D == 3)
+    svd3(Fp, U, sig, V);
+  else
+    svd2(Fp, U, sig, V);
+  // N = sym(U^T G U), M = U^T Fbo V
+  float N[3][3] = {{0.0f}}, M[3][3] = {{0.0f}};
+#pragma unroll
+  for (int i = 0; i < D; i++)
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+      float n = 0.0f, m = 0.0f;
+#pragma unroll
+      for (int a = 0; a < D; a++)
+#pragma unroll
+        for (int e = 0; e < D; e++) {
+          n += U[a][i] * G[a][e] * U[e][j];
+          m += U[a][i] * Fbo[a][e] * V[e][j];
+        }
+      N[i][j] = n;
+      M[i][j] = m;
+    }
+#pragma unroll
+  for (int i = 0; i < D; i++)
+#pragma unroll
+    for (int j = 0; j < i; j++) N[i][j] = N[j][i] = 0.5f * (N[i][j] + N[j][i]);
+  float sh[3] = {1.0f, 1.0f, 1.0f}, sb[3] = {0.0f, 0.0f, 0.0f};
+  float Dg[3][3] = {{0.0f}}, Dt[3][3] = {{0.0f}};
+  float trN = 0.0f;
+#pragma unroll
+  for (int i = 0; i < D; i++) trN += N[i][i];
+  cbar = 0.0f;
+  if (mat == kSnow) {
+    const float lo = 1.0f - 2.5e-2f, hi = 1.0f + 4.5e-3f;  // mpm_solver.py:530-531
+    const float h = __expf(10.0f * (1.0f - Jp));            // mpm_solver.py:515-518
+    const float mu = mu0 * h, la = la0 * h;
+    float gp[3] = {0.0f, 0.0f, 0.0f}, J = 1.0f, r = 1.0f;
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+      gp[i] = (sig[i] > lo && sig[i] < hi) ? 1.0f : 0.0f;
+      sh[i] = fminf(fmaxf(sig[i], lo), hi);
+      J *= sh[i];
+      r *= sig[i] / sh[i];
+    }
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+      for (int j = 0; j < D; j++) {
+        if (i == j) continue;
+        float dg;
+        if (gp[i] != 0.0f && gp[j] != 0.0f)
+          dg = 1.0f;
+        else if (fabsf(sig[j] - sig[i]) > 1e-6f)
+          dg = (sh[j] - sh[i]) / (sig[j] - sig[i]);
+        else
+          dg = 0.5f * (gp[i] + gp[j]);
+        Dg[i][j] = dg;
+        Dt[i][j] = 2.0f * mu * (sh[i] + sh[j] - 1.0f) * dg;
+      }
+    const float Jpn = Jp * r;
+    float mub = 0.0f;
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+      float jo = 1.0f;
+#pragma unroll
+      for (int k = 0; k < D; k++)
+        if (k != i) jo *= sh[k];
+      const float shb = M[i][i] + 2.0f * mu * (2.0f * sh[i] - 1.0f) * N[i][i] +
+                        la * (2.0f * J - 1.0f) * jo * trN;
+      sb[i] = gp[i] * shb + Jpbo * Jpn * (1.0f / sig[i] - gp[i] / sh[i]);
+      mub += N[i][i] * 2.0f * (sh[i] - 1.0f) * sh[i];
+    }
+    const float lab = trN * J * (J - 1.0f);
+    const float hb = mu0 * mub + la0 * lab;
+    Jpbar = Jpbo * r - 10.0f * h * hb;
+    mu0b = h * mub;
+    la0b = h * lab;
+  } else {  // sand
+    const float c = P.sand_coef * P.alpha;
+    float eps[3] = {0.0f, 0.0f, 0.0f}, epsd[3] = {0.0f, 0.0f, 0.0f}, tr = Jp;
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+      eps[i] = logf(fmaxf(fabsf(sig[i]), 1e-4f));
+      epsd[i] = fabsf(sig[i]) > 1e-4f ? 1.0f / sig[i] : 0.0f;
+      tr += eps[i];
+    }
+    float hat[3] = {0.0f, 0.0f, 0.0f}, n2 = 0.0f;
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+      hat[i] = eps[i] - tr / (float)D;
+      n2 += hat[i] * hat[i];
+    }
+    const float n = sqrtf(n2) + 1e-20f;
+    // l = a eps + (common shift): tip a = 0 (sh = 1), inside the cone
+    // a = 1 (sh = |sig|), projected a = 1 - k
+    int branch = 0;  // 0 tip, 1 inside, 2 projected
+    float a = 0.0f, ell[3] = {0.0f, 0.0f, 0.0f};
+    if (tr < 0.0f) {
+      const float dg = n + c * tr;
+      if (dg <= 0.0f) {
+        branch = 1;
+        a = 1.0f;
+#pragma unroll
+        for (int i = 0; i < D; i++) ell[i] = eps[i];
+      } else {
+        branch = 2;
+        const float k = dg / n;
+        a = 1.0f - k;
+#pragma unroll
+        for (int i = 0; i < D; i++) ell[i] = eps[i] - k * hat[i];
+      }
+#pragma unroll
+      for (int i = 0; i < D; i++) sh[i] = expf(ell[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+      for (int j = 0; j < D; j++) {
+        if (i == j) continue;
+        const float de = eps[j] - eps[i];
+        Dg[i][j] = sh[i] / sig[i] * ratio_expm1(a, de);
+        Dt[i][j] = 2.0f * mu0 * a / sig[i] * ratio_lin_expm1(de);
+      }
+    float ellb[3] = {0.0f, 0.0f, 0.0f}, Sl = 0.0f, lsum = 0.0f;
+    mu0b = 0.0f;
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+      ellb[i] = M[i][i] * sh[i] + 2.0f * mu0 * N[i][i] + la0 * trN;
+      Sl += ellb[i];
+      lsum += ell[i];
+      mu0b += N[i][i] * 2.0f * ell[i];
+    }
+    la0b = trN * lsum;
+    float epsb[3] = {0.0f, 0.0f, 0.0f};
+    if (branch == 0) {  // Jp' = tr, sh = 1
+#pragma unroll
+      for (int i = 0; i < D; i++) epsb[i] = Jpbo;
+      Jpbar = Jpbo;
+    } else if (branch == 1) {  // Jp' = 0, l = eps
+#pragma unroll
+      for (int i = 0; i < D; i++) epsb[i] = ellb[i];
+      Jpbar = 0.0f;
+    } else {  // Jp' = 0, l_i = tr / d - c tr u_i with u = hat / |hat|
+      float u[3] = {0.0f, 0.0f, 0.0f}, lu = 0.0f, su = 0.0f;
+#pragma unroll
+      for (int i = 0; i < D; i++) {
+        u[i] = hat[i] / n;
+        lu += ellb[i] * u[i];
+        su += u[i];
+      }
+      const float ctn = c * tr / n, invd = 1.0f / (float)D;
+#pragma unroll
+      for (int i = 0; i < D; i++)
+        epsb[i] = Sl * invd - c * lu - ctn * (ellb[i] - Sl * invd - lu * (u[i] - su * invd));
+      Jpbar = Sl * invd - c * lu + ctn * invd * (Sl - lu * su);
+      cbar = -tr * lu;
+    }
+#pragma unroll
+    for (int i = 0; i < D; i++) sb[i] = epsb[i] * epsd[i];
+  }
+  // X in the (U, V) frame, then Fpbar = U X V^T
+  float X[3][3] = {{0.0f}};
+#pragma unroll
+  for (int i = 0; i < D; i++)
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+      if (i == j) {
+        X[i][i] = sb[i];
+        continue;
+      }
+      float den = sig[i] + sig[j];
+      if (fabsf(den) < 1e-6f) den = den < 0.0f ? -1e-6f : 1e-6f;
+      X[i][j] = 0.5f * (M[i][j] + M[j][i]) * Dg[i][j] +
+                (0.5f * (M[i][j] - M[j][i]) * (sh[i] + sh[j]) + 2.0f * N[i][j] * sig[j] * Dt[i][j]) /
+                    den;
+    }
+#pragma unroll
+  for (int a = 0; a < D; a++)
+#pragma unroll
+    for (int e = 0; e < D; e++) {
+      float s = 0.0f;
+#pragma unroll
+      for (int i = 0; i < D; i++)
+#pragma unroll
+        for (int j = 0; j < D; j++) s += U[a][i] * X[i][j] * V[e][j];
+      Fpb[a][e] = s;
+    }
+}
+
 template <int D>
 __device__ __forceinline__ void load_cf(const float* __restrict__ pf, int64_t cap, int p,
                                         float C[3][3], float F[3][3]) {
@@ -166,6 +395,18 @@ __global__ __launch_bounds__(kThreads) void k_adj_affine(
     const int mat = pi[(int64_t)L::MAT * cap + p];
     float C[3][3], F[3][3];
     load_cf<D>(pf, cap, p, C, F);
+    if (mat == kSnow || mat == kSand) {  // the forward's own constitutive update
+      float Jc = pf[(int64_t)L::JP * cap + p], stress[3][3];
+      const float E = P.youngs ? P.youngs[p] : 0.0f;
+      constitutive<D>(mat, P, P.youngs ? E * P.kmu : P.mu_0, P.youngs ? E * P.kla : P.lambda_0, F,
+                      C, Jc, stress);
+#pragma unroll
+      for (int a = 0; a < D; a++)
+#pragma unroll
+        for (int e = 0; e < D; e++)
+          abuf[(int64_t)(a * D + e) * cap + p] = P.stress_scale * stress[a][e] + P.p_mass * C[a][e];
+      continue;
+    }
     Recompute<D> r;
     recompute<D>(P, mat, C, F, pf[(int64_t)L::JP * cap + p], P.youngs ? P.youngs[p] : 0.0f, r);
     const float iso = r.la * r.J * (r.J - 1.0f);
@@ -367,6 +608,52 @@ __global__ __launch_bounds__(kThreads) void k_adj_constitutive(
     load_cf<D>(pf, cap, p, C, F);
     const float Jp = pf[(int64_t)L::JP * cap + p];
     const float E = P.youngs ? P.youngs[p] : 0.0f;
+    if (mat == kSnow || mat == kSand) {
+      float K[3][3] = {{0.0f}}, Fp[3][3] = {{0.0f}}, G[3][3] = {{0.0f}}, Fbo[3][3] = {{0.0f}};
+      float Cb[3][3] = {{0.0f}};
+#pragma unroll
+      for (int a = 0; a < D; a++)
+#pragma unroll
+        for (int e = 0; e < D; e++) {
+          K[a][e] = (a == e ? 1.0f : 0.0f) + P.dt * C[a][e];
+          const float abar = gin[(int64_t)(L::C + a * D + e) * cap + p];
+          Cb[a][e] = P.p_mass * abar;
+          G[a][e] = P.stress_scale * abar;
+          Fbo[a][e] = gf[(int64_t)(L::F + a * D + e) * cap + p];
+        }
+#pragma unroll
+      for (int a = 0; a < D; a++)
+#pragma unroll
+        for (int e = 0; e < D; e++) {
+          float s = 0.0f;
+#pragma unroll
+          for (int k = 0; k < D; k++) s += K[a][k] * F[k][e];
+          Fp[a][e] = s;
+        }
+      float Fpb[3][3], Jpbar, mu0b, la0b, cb;
+      plastic_adjoint<D>(P, mat, Fp, Jp, P.youngs ? E * P.kmu : P.mu_0,
+                         P.youngs ? E * P.kla : P.lambda_0, G, Fbo,
+                         gf[(int64_t)L::JP * cap + p], Fpb, Jpbar, mu0b, la0b, cb);
+      // F' = K F: Fbar = K^T F'bar, Cbar += dt F'bar F^T
+#pragma unroll
+      for (int a = 0; a < D; a++)
+#pragma unroll
+        for (int e = 0; e < D; e++) {
+          float fb = 0.0f, t = 0.0f;
+#pragma unroll
+          for (int k = 0; k < D; k++) {
+            fb += K[k][a] * Fpb[k][e];
+            t += Fpb[a][k] * F[e][k];
+          }
+          gin[(int64_t)(L::C + a * D + e) * cap + p] = Cb[a][e] + P.dt * t;
+          gin[(int64_t)(L::F + a * D + e) * cap + p] = fb;
+        }
+      gin[(int64_t)L::JP * cap + p] = Jpbar;
+      // mu_0 = E kmu, lambda_0 = E kla; sand's coefficient depends on nu
+      if (gE) gE[p] = P.kmu * mu0b + P.kla * la0b;
+      nu_acc += E * (P.dkmu * mu0b + P.dkla * la0b) + cb * P.dsand_coef * P.alpha;
+      continue;
+    }
     Recompute<D> r;
     recompute<D>(P, mat, C, F, Jp, E, r);
     float Cbar[3][3] = {{0}}, G[3][3] = {{0}}, Fbo[3][3] = {{0}};

@@ -67,6 +67,7 @@ struct Params {
   // their derivatives in nu.  youngs == nullptr: the global mu_0, lambda_0.
   const float* youngs;
   float kmu, kla, dkmu, dkla;
+  float dsand_coef;     // d sand_coef / d nu (sand_coef = 1 + d kla / (2 kmu) there)
   // use_g2p2g mode (mpm_solver.py:363-485, 773-787): 1 selects the fused
   // transfer; v_allowed = dx * g2p2g_allowed_cfl / dt clamps every grid
   // velocity component (mpm:589, 596-598; <= 0: no clamp).
@@ -307,7 +308,10 @@ __device__ __forceinline__ void cell_scatter(int sub, int ccnt, int cbeg, const 
 // with the approximate Givens angles of McAdams et al. 2011 (UW-Madison
 // TR1690), 5 cyclic sweeps, the rotations accumulated as a quaternion (so V
 // is exactly orthonormal after one final normalisation), a column sort and a
-// Givens QR of F V.  Reciprocal square roots use the hardware rsq.
+// Givens QR of F V.  On the device the Jacobi sweeps and the quaternion
+// normalisation use the hardware reciprocal-square-root estimate
+// (fast_rsqrt; the host build, used by mpm_host_svd, divides by sqrtf); the
+// Givens QR divides by sqrtf on both.
 // --------------------------------------------------------------------------
 __host__ __device__ __forceinline__ float fast_rsqrt(float x) {
 #ifdef __HIP_DEVICE_COMPILE__
@@ -383,10 +387,12 @@ __host__ __device__ __forceinline__ void givens_qr(float B[3][3], float U[3][3])
   float r2 = a * a + b * b;
   float c = 1.0f, s = 0.0f;
   if (r2 > 1e-30f) {
-    // a correctly rounded 1 / sqrt: the hardware estimate's last-bit error
-    // leaves c^2 + s^2 off 1 in a biased way, so U and R = U V^T shrink or
-    // grow by ~1e-7, which 2 mu (F - R) F^T turns into stress at F ~ I
-    // (2D snow falling freely drifted 4e-4 from the float64 oracle with it)
+    // a correctly rounded square root and a correctly rounded division
+    // (within about an ulp of 1 / sqrt and unbiased), not the hardware
+    // estimate: that estimate's last-bit error leaves c^2 + s^2 off 1 in a
+    // biased way, so U and R = U V^T shrink or grow by ~1e-7, which
+    // 2 mu (F - R) F^T turns into stress at F ~ I (2D snow falling freely
+    // drifted 4e-4 from the float64 oracle with it)
     float ir = 1.0f / sqrtf(r2);
     c = a * ir;
     s = b * ir;
@@ -598,8 +604,12 @@ __device__ __forceinline__ void constitutive(int mat, const Params& P, float mu_
     return;
   }
 
-  float h = 1.0f;  // mpm_solver.py:515-524 (sand ignores h: its stress uses mu_0, lambda_0)
-  if (mat == kSnow && plastic) h = __expf(10.0f * (1.0f - Jp));
+  // Hardening, mpm_solver.py:515-524 (p2g: every material but water) and
+  // :419-424 (g2p2g: every material): h = exp(10 (1 - Jp)) for snow and for
+  // stationary particles, whose Jp a caller may set; elastic is fixed at
+  // 0.3, and sand's stress uses mu_0, lambda_0, so its h is never formed.
+  float h = 1.0f;
+  if (plastic && (mat == kSnow || mat == kStationary)) h = __expf(10.0f * (1.0f - Jp));
   if (mat == kElastic) h = 0.3f;
   float mu = mu_0 * h, la = lambda_0 * h;
 
@@ -695,67 +705,168 @@ __device__ __forceinline__ void constitutive(int mat, const Params& P, float mu_
   }
 }
 
-// grid_postprocess operators on one node velocity (mpm_solver.py:600-687).
+// One grid_postprocess operator on one node velocity (mpm_solver.py:600-687).
 template <int D>
-__device__ __forceinline__ void apply_post(const Params& P, const int I[3], float v[3]) {
-  for (int k = 0; k < P.n_post; k++) {
-    const PostOp& op = P.post[k];
-    if (op.kind == kPostBBox) {  // mpm_solver.py:613-616
+__device__ __forceinline__ void apply_post_op(const PostOp& op, const Params& P, const int I[3],
+                                              float v[3]) {
+  if (op.kind == kPostBBox) {  // mpm_solver.py:613-616
 #pragma unroll
-      for (int a = 0; a < D; a++) {
-        if (I[a] < op.lo[a] && v[a] < 0.0f) v[a] = 0.0f;
-        if (I[a] >= op.hi[a] && v[a] > 0.0f) v[a] = 0.0f;
-      }
-    } else if (op.kind == kPostPlane) {  // mpm_solver.py:660-685
-      float dot = 0.0f;
+    for (int a = 0; a < D; a++) {
+      if (I[a] < op.lo[a] && v[a] < 0.0f) v[a] = 0.0f;
+      if (I[a] >= op.hi[a] && v[a] > 0.0f) v[a] = 0.0f;
+    }
+  } else if (op.kind == kPostPlane) {  // mpm_solver.py:660-685
+    float dot = 0.0f;
 #pragma unroll
-      for (int a = 0; a < D; a++) dot += ((float)I[a] * P.dx - op.point[a]) * op.normal[a];
-      if (dot < 0.0f) {
-        if (op.surface == kSticky) {
+    for (int a = 0; a < D; a++) dot += ((float)I[a] * P.dx - op.point[a]) * op.normal[a];
+    if (dot < 0.0f) {
+      if (op.surface == kSticky) {
 #pragma unroll
-          for (int a = 0; a < D; a++) v[a] = 0.0f;
-        } else {
-          float nc = 0.0f;
+        for (int a = 0; a < D; a++) v[a] = 0.0f;
+      } else {
+        float nc = 0.0f;
 #pragma unroll
-          for (int a = 0; a < D; a++) nc += op.normal[a] * v[a];
-          float proj = op.surface == kSlip ? nc : fminf(nc, 0.0f);
-          float vn2 = 0.0f;
+        for (int a = 0; a < D; a++) nc += op.normal[a] * v[a];
+        float proj = op.surface == kSlip ? nc : fminf(nc, 0.0f);
+        float vn2 = 0.0f;
 #pragma unroll
-          for (int a = 0; a < D; a++) {
-            v[a] = v[a] - op.normal[a] * proj;
-            vn2 += v[a] * v[a];
-          }
-          float vn = sqrtf(vn2);
-          if (nc < 0.0f && vn > 1e-30f) {
-            float mag = fmaxf(0.0f, vn + nc * op.friction);
-            float inv = 1.0f / vn;
-#pragma unroll
-            for (int a = 0; a < D; a++) v[a] = v[a] * inv * mag;
-          }
+        for (int a = 0; a < D; a++) {
+          v[a] = v[a] - op.normal[a] * proj;
+          vn2 += v[a] * v[a];
         }
-      }
-    } else if (op.kind == kPostSphere) {  // mpm_solver.py:621-640
-      float off[3], n2 = 0.0f;
+        float vn = sqrtf(vn2);
+        if (nc < 0.0f && vn > 1e-30f) {
+          float mag = fmaxf(0.0f, vn + nc * op.friction);
+          float inv = 1.0f / vn;
 #pragma unroll
-      for (int a = 0; a < D; a++) {
-        off[a] = (float)I[a] * P.dx - op.point[a];
-        n2 += off[a] * off[a];
-      }
-      if (n2 < op.radius * op.radius) {
-        if (op.surface == kSticky) {
-#pragma unroll
-          for (int a = 0; a < D; a++) v[a] = 0.0f;
-        } else {
-          float inv = 1.0f / (sqrtf(n2) + 1e-5f);
-          float nc = 0.0f;
-#pragma unroll
-          for (int a = 0; a < D; a++) nc += off[a] * inv * v[a];
-          float proj = op.surface == kSlip ? nc : fminf(nc, 0.0f);
-#pragma unroll
-          for (int a = 0; a < D; a++) v[a] = v[a] - off[a] * inv * proj;
+          for (int a = 0; a < D; a++) v[a] = v[a] * inv * mag;
         }
       }
     }
+  } else if (op.kind == kPostSphere) {  // mpm_solver.py:621-640
+    float off[3], n2 = 0.0f;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+      off[a] = (float)I[a] * P.dx - op.point[a];
+      n2 += off[a] * off[a];
+    }
+    if (n2 < op.radius * op.radius) {
+      if (op.surface == kSticky) {
+#pragma unroll
+        for (int a = 0; a < D; a++) v[a] = 0.0f;
+      } else {
+        float inv = 1.0f / (sqrtf(n2) + 1e-5f);
+        float nc = 0.0f;
+#pragma unroll
+        for (int a = 0; a < D; a++) nc += off[a] * inv * v[a];
+        float proj = op.surface == kSlip ? nc : fminf(nc, 0.0f);
+#pragma unroll
+        for (int a = 0; a < D; a++) v[a] = v[a] - off[a] * inv * proj;
+      }
+    }
+  }
+}
+
+// grid_postprocess operators on one node velocity, in the order added.
+template <int D>
+__device__ __forceinline__ void apply_post(const Params& P, const int I[3], float v[3]) {
+  for (int k = 0; k < P.n_post; k++) apply_post_op<D>(P.post[k], P, I, v);
+}
+
+// Adjoint of one operator: vbar (the adjoint of its output) becomes the
+// adjoint of its input v.
+//  - walls and sticky surfaces zero what they zero (a mask);
+//  - slip / separate planes and spheres project out the normal part:
+//    u = v - n p(n . v) with p = id (slip) or min(., 0) (separate), so
+//    vbar <- ubar - n p'(n . v) (n . ubar);
+//  - friction on a plane, out = u max(0, |u| + f (n . v)) / |u| when
+//    n . v < 0: ubar = (1 + f nc / |u|) vbar - f nc (u . vbar) u / |u|^3 and
+//    the normal component nc = n . v receives f (u . vbar) / |u| (zero where
+//    the max clamps the velocity to rest).
+template <int D>
+__device__ __forceinline__ void post_op_adjoint(const PostOp& op, const Params& P, const int I[3],
+                                                const float v[3], float vb[3]) {
+  if (op.kind == kPostBBox) {
+#pragma unroll
+    for (int a = 0; a < D; a++)
+      if ((I[a] < op.lo[a] && v[a] < 0.0f) || (I[a] >= op.hi[a] && v[a] > 0.0f)) vb[a] = 0.0f;
+    return;
+  }
+  float nrm[3] = {0.0f, 0.0f, 0.0f};
+  bool inside;
+  if (op.kind == kPostPlane) {
+    float dot = 0.0f;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+      dot += ((float)I[a] * P.dx - op.point[a]) * op.normal[a];
+      nrm[a] = op.normal[a];
+    }
+    inside = dot < 0.0f;
+  } else {
+    float off[3], n2 = 0.0f;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+      off[a] = (float)I[a] * P.dx - op.point[a];
+      n2 += off[a] * off[a];
+    }
+    inside = n2 < op.radius * op.radius;
+    const float inv = 1.0f / (sqrtf(n2) + 1e-5f);
+#pragma unroll
+    for (int a = 0; a < D; a++) nrm[a] = off[a] * inv;
+  }
+  if (!inside) return;
+  if (op.surface == kSticky) {
+#pragma unroll
+    for (int a = 0; a < D; a++) vb[a] = 0.0f;
+    return;
+  }
+  float nc = 0.0f;
+#pragma unroll
+  for (int a = 0; a < D; a++) nc += nrm[a] * v[a];
+  const float dproj = (op.surface == kSlip || nc < 0.0f) ? 1.0f : 0.0f;
+  const float proj = op.surface == kSlip ? nc : fminf(nc, 0.0f);
+  float ub[3] = {vb[0], vb[1], vb[2]}, ncb = 0.0f;
+  if (op.kind == kPostPlane) {
+    float u[3] = {0.0f, 0.0f, 0.0f}, un2 = 0.0f;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+      u[a] = v[a] - nrm[a] * proj;
+      un2 += u[a] * u[a];
+    }
+    const float un = sqrtf(un2);
+    if (nc < 0.0f && un > 1e-30f) {
+      const float mag = un + nc * op.friction;
+      if (mag <= 0.0f) {
+#pragma unroll
+        for (int a = 0; a < D; a++) vb[a] = 0.0f;
+        return;
+      }
+      float uv = 0.0f;
+#pragma unroll
+      for (int a = 0; a < D; a++) uv += u[a] * vb[a];
+      const float g = nc * op.friction / un;
+#pragma unroll
+      for (int a = 0; a < D; a++) ub[a] = (1.0f + g) * vb[a] - g * uv * u[a] / un2;
+      ncb = op.friction * uv / un;
+    }
+  }
+  float nu = 0.0f;
+#pragma unroll
+  for (int a = 0; a < D; a++) nu += nrm[a] * ub[a];
+#pragma unroll
+  for (int a = 0; a < D; a++) vb[a] = ub[a] - nrm[a] * dproj * nu + nrm[a] * ncb;
+}
+
+// Adjoint of the whole post-process list at a node whose velocity entering
+// it is v0: the operators in reverse order, each at its own input (the
+// forward re-run up to it).
+template <int D>
+__device__ __forceinline__ void apply_post_adjoint(const Params& P, const int I[3],
+                                                   const float v0[3], float vb[3]) {
+  for (int k = P.n_post - 1; k >= 0; k--) {
+    float v[3] = {v0[0], v0[1], v0[2]};
+    for (int j = 0; j < k; j++) apply_post_op<D>(P.post[j], P, I, v);
+    post_op_adjoint<D>(P.post[k], P, I, v, vb);
   }
 }
 

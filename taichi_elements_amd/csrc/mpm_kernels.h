@@ -142,6 +142,9 @@ void launch_scatter_f(const float* src, int64_t cap, int ncomp, const int* orig,
 void launch_scatter_i(const int* src, int64_t cap, int ncomp, const int* orig, int n, int* dst,
                       hipStream_t st);
 void launch_fill_u32(unsigned* p, unsigned v, int64_t n, hipStream_t st);
+// svd3 / svd2 on n row-major dim x dim matrices (tests of the device SVD).
+void launch_svd_batch(int dim, const float* A, float* U, float* sig, float* V, int64_t n,
+                      hipStream_t st);
 
 // Bounding box of the particle positions: groups x 6 partial (min, max) per axis.
 void launch_aabb(int D, bool quant, const float* pf, int64_t cap, int n, float* partial,

@@ -684,26 +684,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
 // Bounding-box walls zero the outward velocity components near the domain
 // faces (mpm_solver.py:600-616); bit a of the result is set when component a
 // of v (before the walls) passes unchanged.
-template <int D>
-__device__ __forceinline__ int wall_keep_mask(const Params& P, const int I[3], const float v[3]) {
-  int keep = (1 << D) - 1;
-  for (int k = 0; k < P.n_post; k++) {
-    const PostOp& op = P.post[k];
-    if (op.kind != kPostBBox) continue;
-#pragma unroll
-    for (int a = 0; a < D; a++) {
-      if ((I[a] < op.lo[a] && v[a] < 0.0f) || (I[a] >= op.hi[a] && v[a] > 0.0f))
-        keep &= ~(1 << a);
-    }
-  }
-  return keep;
-}
-
 // MODE 0: the grid operator (raw != 0: store the p2g sums unnormalised).
 // MODE 1: its adjoint -- the tiles hold the g2p-adjoint scatter vbar; with
 // the forward's raw sums (p, m) in grid, write (pbar, mbar) to gbar:
-// v = keep * (p / m + dt g) gives pbar = keep * vbar / m and
-// mbar = -sum_a pbar_a p_a / m (walls are the only post-process allowed).
+// v = post(p / m + dt g) gives vbar' = post^T(vbar) (apply_post_adjoint:
+// walls, planes and spheres), pbar = vbar' / m and
+// mbar = -sum_a pbar_a p_a / m.
 template <int D, int MODE>
 __global__ __launch_bounds__(kThreads) void k_grid_op(const Params* __restrict__ scenes,
                                                       const int* __restrict__ slab,
@@ -770,17 +756,18 @@ __global__ __launch_bounds__(kThreads) void k_grid_op(const Params* __restrict__
       if (node_has_mass(f.w)) {
         const float inv = 1.0f / f.w;
         const float pf4[3] = {f.x, f.y, f.z};
-        float v[3];
+        float v[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int a = 0; a < D; a++) v[a] = pf4[a] * inv + P.dt * P.gravity[a];
         int I[3], lbl[3];
         to_local<D>(lb, P, lbl);
 #pragma unroll
         for (int a = 0; a < D; a++) I[a] = (lbl[a] + P.blo[a]) * kBlock + l[a];
-        const int keep = wall_keep_mask<D>(P, I, v);
+        float vb[3] = {p[0], p[1], p[2]};
+        apply_post_adjoint<D>(P, I, v, vb);
 #pragma unroll
         for (int a = 0; a < D; a++) {
-          pb[a] = ((keep >> a) & 1) ? p[a] * inv : 0.0f;
+          pb[a] = vb[a] * inv;
           mb -= pb[a] * (pf4[a] * inv);
         }
       }
@@ -1593,6 +1580,40 @@ void launch_copy_segments(const CopySegments& S, hipStream_t st) {
   for (int s = 0; s < S.count; s++) most = std::max(most, S.n[s]);
   if (most > 0)
     k_copy_segments<<<clamp_groups(div_up(most, kThreads), 2048), kThreads, 0, st>>>(S);
+}
+
+// The device SVD (svd3 / svd2, the replacement of ti.svd) on n matrices,
+// for tests: A, U, V (n, dim, dim) row-major, sig (n, dim).
+__global__ __launch_bounds__(kThreads) void k_svd_batch(int dim, const float* __restrict__ A,
+                                                        float* __restrict__ U,
+                                                        float* __restrict__ sig,
+                                                        float* __restrict__ V, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kThreads) {
+    float a[3][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+    float u[3][3], v[3][3], s3[3] = {0.0f, 0.0f, 0.0f};
+    const int dd = dim * dim;
+    for (int r = 0; r < dim; r++)
+      for (int c = 0; c < dim; c++) a[r][c] = A[i * dd + r * dim + c];
+    if (dim == 3)
+      svd3(a, u, s3, v);
+    else
+      svd2(a, u, s3, v);
+    for (int r = 0; r < dim; r++) {
+      sig[i * dim + r] = s3[r];
+      for (int c = 0; c < dim; c++) {
+        U[i * dd + r * dim + c] = u[r][c];
+        V[i * dd + r * dim + c] = v[r][c];
+      }
+    }
+  }
+}
+
+void launch_svd_batch(int dim, const float* A, float* U, float* sig, float* V, int64_t n,
+                      hipStream_t st) {
+  if (n > 0)
+    k_svd_batch<<<clamp_groups(div_up(n, kThreads), 4096), kThreads, 0, st>>>(dim, A, U, sig, V,
+                                                                            n);
 }
 
 void launch_fill_u32(unsigned* p, unsigned v, int64_t n, hipStream_t st) {

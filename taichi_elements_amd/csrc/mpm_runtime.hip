@@ -200,6 +200,16 @@ struct mpm_solver {
   const float* diff_youngs = nullptr;
   double diff_nu = 0.0;
 
+  // mpm_snapshot: a device copy of the particle slots and, in use_g2p2g
+  // mode, of the grid the next substep reads (with n_final and, for an
+  // unbounded scene, the block window that grid belongs to)
+  float* snap_pf = nullptr;
+  int* snap_pi = nullptr;
+  float4* snap_grid = nullptr;
+  int64_t snap_n = -1, snap_cap = 0, snap_n_final = 0, snap_grid_nodes = 0;
+  int snap_nf = 0;
+  int snap_blo[3] = {0, 0, 0}, snap_bnl[3] = {1, 1, 1};
+
   // particle storage: quant = 1 packs x, v and F (PackedLayout); plastic = 0
   // is support_plasticity=False (no Jp in the packed layout)
   int quant = 0;
@@ -290,7 +300,7 @@ void fill_params(Params& P, const mpm_solver* s, const SceneHost& sc, float dt, 
   P.zoff = sc.zoff;
   P.skip = skip;
   P.youngs = nullptr;
-  P.kmu = P.kla = P.dkmu = P.dkla = 0.0f;
+  P.kmu = P.kla = P.dkmu = P.dkla = P.dsand_coef = 0.0f;
   P.g2p2g = s->g2p2g;
   P.plastic = s->plastic;
   P.v_allowed = s->g2p2g && s->g2p2g_clamp && s->g2p2g_cfl > 0
@@ -920,6 +930,9 @@ int mpm_destroy(mpm_solver* s) {
   dfree(s->maxv);
   dfree(s->d_params);
   dfree(s->d_slab);
+  dfree(s->snap_pf);
+  dfree(s->snap_pi);
+  dfree(s->snap_grid);
   dfree(s->diff_pi);
   dfree(s->diff_scratch);
   dfree(s->diff_gbar);
@@ -1351,6 +1364,27 @@ int mpm_host_svd(int32_t dim, const float* A, float* U, float* sig, float* V) {
   return MPM_OK;
 }
 
+int mpm_device_svd(int32_t device, int32_t dim, int64_t n, const float* A, float* U, float* sig,
+                   float* V) {
+  if ((dim != 2 && dim != 3) || n < 0) return fail(MPM_EINVAL, "bad argument");
+  if (n == 0) return MPM_OK;
+  if (!A || !U || !sig || !V) return fail(MPM_EINVAL, "null argument");
+  HIP_TRY(hipSetDevice(device));
+  const int64_t dd = (int64_t)dim * dim;
+  DeviceArray<float> a, u, sg, v;
+  TRY(a.alloc(n * dd));
+  TRY(u.alloc(n * dd));
+  TRY(v.alloc(n * dd));
+  TRY(sg.alloc(n * dim));
+  HIP_TRY(hipMemcpy(a.p, A, sizeof(float) * n * dd, hipMemcpyHostToDevice));
+  launch_svd_batch(dim, a.p, u.p, sg.p, v.p, n, nullptr);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpy(U, u.p, sizeof(float) * n * dd, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(V, v.p, sizeof(float) * n * dd, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(sig, sg.p, sizeof(float) * n * dim, hipMemcpyDeviceToHost));
+  return MPM_OK;
+}
+
 int mpm_last_counts(mpm_solver* s, int64_t* out4) {
   if (!s || !out4) return fail(MPM_EINVAL, "null argument");
   TRY(check_device_errors(s));
@@ -1373,11 +1407,9 @@ int mpm_diff_setup(mpm_solver* s, int64_t n, const int32_t* material) {
   if (s->scenes.size() != 1) return fail(MPM_EINVAL, "differentiable mode runs one scene");
   if (n <= 0 || n > (int64_t)INT32_MAX / 2) return fail(MPM_EINVAL, "bad particle count");
   for (int64_t i = 0; i < n; i++)
-    if (material[i] != MPM_WATER && material[i] != MPM_ELASTIC)
-      return fail(MPM_EINVAL, "differentiable mode supports water and elastic particles");
-  for (const PostOp& op : s->scenes[0].post)
-    if (op.kind != kPostBBox)
-      return fail(MPM_EINVAL, "differentiable mode supports the bounding box only");
+    if (material[i] < MPM_WATER || material[i] > MPM_SAND)
+      return fail(MPM_EINVAL,
+                  "differentiable mode supports water, elastic, snow and sand particles");
   if (s->g2p2g) return fail(MPM_EINVAL, "differentiable mode uses the p2g / g2p transfers");
   if (s->unbounded) return fail(MPM_EINVAL, "differentiable mode keeps its block window");
   if (s->quant || !s->plastic)
@@ -1427,6 +1459,14 @@ int diff_params(mpm_solver* s, float dt, const float* youngs, double nu, hipStre
   P.dkmu = (float)(-1.0 / (2.0 * (1.0 + nu) * (1.0 + nu)));
   const double g = (1.0 + nu) * (1.0 - 2.0 * nu);
   P.dkla = (float)((1.0 + 2.0 * nu * nu) / (g * g));
+  // sand's return-map coefficient (d lambda_0 + 2 mu_0) / (2 mu_0)
+  // (mpm_solver.py:336-337) with this nu: E cancels, nu does not
+  {
+    const double kmu = 1.0 / (2.0 * (1.0 + nu)), kla = nu / g;
+    const double dkmu = -1.0 / (2.0 * (1.0 + nu) * (1.0 + nu)), dkla = (1.0 + 2.0 * nu * nu) / (g * g);
+    P.sand_coef = (float)((s->D * kla + 2.0 * kmu) / (2.0 * kmu));
+    P.dsand_coef = (float)(0.5 * s->D * (dkla * kmu - kla * dkmu) / (kmu * kmu));
+  }
   HIP_TRY(hipMemcpyAsync(s->d_params, s->h_params, sizeof(Params), hipMemcpyHostToDevice, st));
   s->diff_params_valid = true;
   s->diff_dt = dt;
@@ -1585,6 +1625,86 @@ int mpm_set_tuning(mpm_solver* s, int32_t chunk, int32_t groups) {
   }
   // the next substep bins from scratch (chunk boundaries changed)
   s->have_perm = false;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return MPM_OK;
+}
+
+int mpm_snapshot(mpm_solver* s, int32_t op) {
+  if (!s) return fail(MPM_EINVAL, "null argument");
+  if (s->diff_n != 0) return fail(MPM_EINVAL, "a differentiable handle keeps its state in caller arrays");
+  HIP_TRY(hipSetDevice(s->device));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  const int ni = s->D == 3 ? Layout<3>::NI : Layout<2>::NI;
+  const int NPB = s->D == 3 ? Geo<3>::kNodesPerBlock : Geo<2>::kNodesPerBlock;
+  if (op == MPM_SNAPSHOT_RELEASE) {
+    dfree(s->snap_pf);
+    dfree(s->snap_pi);
+    dfree(s->snap_grid);
+    s->snap_n = -1;
+    return MPM_OK;
+  }
+  if (op == MPM_SNAPSHOT_SAVE) {
+    TRY(check_device_errors(s));
+    const int64_t nodes = s->g2p2g ? (int64_t)s->total_blocks * NPB : 0;
+    if (s->snap_cap != s->cap || s->snap_nf != s->nf() || !s->snap_pf) {
+      dfree(s->snap_pf);
+      dfree(s->snap_pi);
+      s->snap_cap = 0;
+      if (s->cap > 0) {
+        TRY(dalloc(&s->snap_pf, (int64_t)s->nf() * s->cap));
+        TRY(dalloc(&s->snap_pi, (int64_t)ni * s->cap));
+      }
+      s->snap_cap = s->cap;
+      s->snap_nf = s->nf();
+    }
+    if (s->snap_grid_nodes != nodes) {
+      dfree(s->snap_grid);
+      s->snap_grid_nodes = 0;
+      if (nodes > 0) TRY(dalloc(&s->snap_grid, nodes));
+      s->snap_grid_nodes = nodes;
+    }
+    if (s->cap > 0) {
+      HIP_TRY(hipMemcpyAsync(s->snap_pf, s->pf, sizeof(float) * s->nf() * s->cap,
+                             hipMemcpyDeviceToDevice, s->stream));
+      HIP_TRY(hipMemcpyAsync(s->snap_pi, s->pi, sizeof(int) * ni * s->cap,
+                             hipMemcpyDeviceToDevice, s->stream));
+    }
+    if (nodes > 0)
+      HIP_TRY(hipMemcpyAsync(s->snap_grid, s->grid, sizeof(float4) * nodes,
+                             hipMemcpyDeviceToDevice, s->stream));
+    s->snap_n = s->n;
+    s->snap_n_final = s->n_final;
+    for (int a = 0; a < 3; a++) {
+      s->snap_blo[a] = s->scenes[0].blo[a];
+      s->snap_bnl[a] = s->scenes[0].bnl[a];
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return MPM_OK;
+  }
+  if (op != MPM_SNAPSHOT_RESTORE) return fail(MPM_EINVAL, "unknown snapshot operation");
+  if (s->snap_n < 0) return fail(MPM_EINVAL, "no snapshot saved");
+  if (s->snap_n != s->n || s->snap_cap != s->cap || s->snap_nf != s->nf())
+    return fail(MPM_EINVAL, "particles were added or the storage changed since the snapshot");
+  if (s->unbounded) {  // the g2p2g grid belongs to the window it was saved with
+    bool moved = false;
+    for (int a = 0; a < s->D; a++)
+      moved |= s->scenes[0].blo[a] != s->snap_blo[a] || s->scenes[0].bnl[a] != s->snap_bnl[a];
+    if (moved) TRY(set_window(s, s->snap_blo, s->snap_bnl));
+  }
+  if (s->g2p2g && s->snap_grid_nodes != (int64_t)s->total_blocks * NPB)
+    return fail(MPM_EINVAL, "the grid window changed since the snapshot");
+  if (s->cap > 0) {
+    HIP_TRY(hipMemcpyAsync(s->pf, s->snap_pf, sizeof(float) * s->nf() * s->cap,
+                           hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->pi, s->snap_pi, sizeof(int) * ni * s->cap,
+                           hipMemcpyDeviceToDevice, s->stream));
+  }
+  if (s->g2p2g && s->snap_grid_nodes > 0)
+    HIP_TRY(hipMemcpyAsync(s->grid, s->snap_grid, sizeof(float4) * s->snap_grid_nodes,
+                           hipMemcpyDeviceToDevice, s->stream));
+  s->n_final = s->snap_n_final;
+  s->have_perm = false;  // slots moved: bin from scratch
+  s->reordered = false;
   HIP_TRY(hipStreamSynchronize(s->stream));
   return MPM_OK;
 }

@@ -6,14 +6,16 @@ is one global constant (mpm_solver.py:201).  Here one substep is a pure
 function of a particle state held in a torch tensor, with a per-particle
 Young's modulus E_p and a global Poisson ratio nu as differentiable inputs:
 
-    state' = substep(state, E, nu)        (water and fixed-corotated elastic)
+    state' = substep(state, E, nu)   (water, fixed-corotated elastic, snow, sand)
 
 The forward runs the engine's HIP kernels (binning, p2g, grid operator,
 g2p) on the caller's tensors; the backward runs hand-written adjoint kernels
-(scatter of the g2p adjoint into LDS node tiles, the grid-operator adjoint,
-and a per-particle kernel with the p2g/g2p gathers and the constitutive
-adjoint, including the derivative of the polar rotation R = U V^T), after
-recomputing the binning and the grid from the substep's input.  Each
+(scatter of the g2p adjoint into LDS node tiles, the grid-operator adjoint
+through the walls and the surface and sphere colliders, and a per-particle
+kernel with the p2g/g2p gathers and the constitutive adjoint, including the
+derivative of the polar rotation R = U V^T and, for snow and sand, of the
+singular-value clamp / Drucker-Prager return map and the hardening), on the
+forward's record of the substep (binning and raw grid sums).  Each
 substep is one ``torch.autograd.Function``; ``run`` checkpoints the state
 every ``checkpoint_every`` substeps (torch.utils.checkpoint) so only those
 states are stored and each segment is recomputed during the backward pass.
@@ -117,7 +119,8 @@ class DiffMPMSolver:
         return self._solver
 
     def setup(self, material, positions=None):
-        """Fix the particles' materials (water / elastic).  With the initial
+        """Fix the particles' materials (water, elastic, snow or sand; not
+        stationary).  With the initial
         ``positions`` given, the substeps work on the particles in a
         spatially sorted order (by 4^d grid block) so that the transfers
         read coherent memory; ``run`` maps to and from the caller's order
@@ -125,6 +128,7 @@ class DiffMPMSolver:
         material = np.ascontiguousarray(np.asarray(material, np.int32).reshape(-1))
         n = len(material)
         self._order = self._inverse = None
+        caller_material = material
         if positions is not None:
             x = np.asarray(positions, np.float32).reshape(n, self.dim)
             base = np.floor(x * np.float32(self._solver.inv_dx) - np.float32(0.5)).astype(np.int64)
@@ -142,11 +146,12 @@ class DiffMPMSolver:
         check(_lib.lib().mpm_diff_binning_bytes(self._handle, nbytes))
         self._binning_bytes = int(nbytes.value)
         self.n = n
-        self.material = material
+        self.material = caller_material
 
     def state(self, x, v=None):
         """Initial state tensor from positions (and velocities): F = I,
-        C = 0, Jp = 1 (seed_particle, mpm_solver.py:823-838)."""
+        C = 0, Jp = 0 for sand and 1 otherwise (seed_particle,
+        mpm_solver.py:823-838)."""
         d, n = self.dim, self.n
         x = torch.as_tensor(np.asarray(x, np.float32).reshape(n, d))
         s = torch.zeros(n_fields(d), n, dtype=torch.float32)
@@ -156,7 +161,7 @@ class DiffMPMSolver:
         f0 = 2 * d + d * d
         for a in range(d):
             s[f0 + a * d + a] = 1.0
-        s[-1] = 1.0
+        s[-1] = torch.as_tensor(np.where(self.material == 3, 0.0, 1.0).astype(np.float32))
         return s.to(self.device)
 
     def positions(self, state):

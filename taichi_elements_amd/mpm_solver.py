@@ -100,13 +100,19 @@ class _ParticleStorage:
         return self._solver.particle_bytes()
 
 
-def _autotune(obj, handle, dt, get_state, set_state, substeps, chunks, groups, verbose, dts):
+SNAPSHOT_SAVE, SNAPSHOT_RESTORE, SNAPSHOT_RELEASE = 0, 1, 2
+
+
+def _autotune(obj, handle, dt, substeps, chunks, groups, verbose, dts):
     """Shared by MPMSolver and MPMSolverBatch: time each launch shape on
-    the live particles (two untimed substeps re-bin after each change)."""
+    the live particles (two untimed substeps re-bin after each change).
+    The scene is put back from a device snapshot (mpm_snapshot) before
+    every candidate and at the end: the particle slots and, with use_g2p2g,
+    the grid the next substep reads and which particles are new."""
     L = _lib.lib()
     if obj.n_particles_total() == 0:
         return {}
-    snap = get_state()
+    check(L.mpm_snapshot(handle, SNAPSHOT_SAVE))
     steps = np.ascontiguousarray(np.asarray(dts(), np.float32))
     batched = len(steps) > 1 or hasattr(obj, "scenes")
 
@@ -121,7 +127,7 @@ def _autotune(obj, handle, dt, get_state, set_state, substeps, chunks, groups, v
     for ch in chunks:
         for gr in groups:
             check(L.mpm_set_tuning(handle, int(ch), int(gr)))
-            set_state(snap)
+            check(L.mpm_snapshot(handle, SNAPSHOT_RESTORE))
             run(2)
             t0 = time.perf_counter()
             run(substeps)
@@ -130,7 +136,8 @@ def _autotune(obj, handle, dt, get_state, set_state, substeps, chunks, groups, v
                 print(f"chunk {ch:4d} groups {gr:5d}: {results[(ch, gr)]:.4f} ms/substep")
     best = min(results, key=results.get)
     check(L.mpm_set_tuning(handle, int(best[0]), int(best[1])))
-    set_state(snap)
+    check(L.mpm_snapshot(handle, SNAPSHOT_RESTORE))
+    check(L.mpm_snapshot(handle, SNAPSHOT_RELEASE))
     return results
 
 
@@ -791,12 +798,13 @@ class MPMSolver:
     def autotune(self, dt=None, substeps=4, chunks=(64, 128, 256), groups=(1024, 2048, 4096),
                  verbose=False):
         """Times every (chunk, groups) candidate on the current particles
-        and keeps the fastest.  The particle state, t and the substep count
-        are restored afterwards; only float32 summation order depends on the
-        choice.  Returns {(chunk, groups): ms per substep}."""
+        and keeps the fastest.  The particle state (with use_g2p2g also the
+        grid the next substep reads), t and the substep count are restored
+        afterwards; only float32 summation order depends on the choice.
+        Returns {(chunk, groups): ms per substep}."""
         return _autotune(self, self._h(), dt if dt is not None else self.default_dt,
-                         self.get_state, lambda st: self.set_state(**st), substeps, chunks,
-                         groups, verbose, lambda: [dt if dt is not None else self.default_dt])
+                         substeps, chunks, groups, verbose,
+                         lambda: [dt if dt is not None else self.default_dt])
 
     def last_counts(self):
         out = np.zeros(4, np.int64)

@@ -23,9 +23,10 @@ operator).  Run from the repository root:
 
 which rewrites tests/golden/substep_{2,3}d.npz and g2p2g_{2,3}d.npz (three
 use_g2p2g substeps).  The states are
-pre-deformed (F = R diag(1 +- 0.05) Q^T, C ~ N(0, 5^2), snow Jp in
-[0.9, 1.1], sand Jp in [-0.05, 0], water Jp in [0.95, 1.05]) so that every
-constitutive branch does work, and the particles sit next to the low walls
+pre-deformed (F = R diag(1 +- 0.05) Q^T, C ~ N(0, 5^2), snow, elastic and
+stationary Jp in [0.9, 1.1], sand Jp in [-0.05, 0], water Jp in
+[0.95, 1.05]) so that every constitutive branch does work (the hardening of
+stationary particles included), and the particles sit next to the low walls
 so that the bounding box acts.
 """
 import math
@@ -280,6 +281,10 @@ def initial_state(dim, seed, n=64):
     Jp[mat == SAND] = rng.uniform(-0.05, 0.0, (mat == SAND).sum())
     Jp[mat == WATER] = rng.uniform(0.95, 1.05, (mat == WATER).sum())
     v = rng.uniform(-1, 1, (n, dim))
+    # elastic and stationary particles with Jp != 1: stationary ones are
+    # hardened by it, elastic ones are not (mpm_solver.py:515-521, 419-424)
+    for m in (ELASTIC, STATIONARY):
+        Jp[mat == m] = rng.uniform(0.9, 1.1, (mat == m).sum())
     st = {f: a.astype(np.float32).astype(np.float64) for f, a in
           dict(x=x, v=v, C=C, F=F, Jp=Jp).items()}
     st["material"] = mat.astype(np.int32)

@@ -22,8 +22,12 @@ def random_rotations(rng, n, dim):
 def deformed_state(rng, x, material, dim, stretch=0.05, c_scale=5.0):
     """Pre-deformed particle state so that one substep exercises every
     constitutive branch: F = R diag(1 +- stretch) Q^T, C ~ N(0, c_scale),
-    snow Jp in [0.9, 1.1], sand Jp in [-0.05, 0], water Jp in [0.95, 1.05]."""
+    snow Jp in [0.9, 1.1], sand Jp in [-0.05, 0], water Jp in [0.95, 1.05];
+    elastic and stationary Jp in [0.9, 1.1] as well (a caller may set them:
+    stationary particles are hardened by exp(10 (1 - Jp)), elastic ones are
+    not, mpm_solver.py:515-521)."""
     n = len(x)
+    material = np.asarray(material)
     R = random_rotations(rng, n, dim)
     Q = random_rotations(rng, n, dim)
     s = 1 + stretch * rng.uniform(-1, 1, size=(n, dim))
@@ -34,6 +38,8 @@ def deformed_state(rng, x, material, dim, stretch=0.05, c_scale=5.0):
     Jp[material == 3] = rng.uniform(-0.05, 0.0, (material == 3).sum())
     Jp[material == 0] = rng.uniform(0.95, 1.05, (material == 0).sum())
     v = rng.uniform(-1, 1, size=(n, dim)).astype(np.float32)
+    for m in (1, 4):
+        Jp[material == m] = rng.uniform(0.9, 1.1, (material == m).sum())
     return dict(x=np.asarray(x, np.float32), v=v, C=C, F=F, Jp=Jp,
                 material=np.asarray(material, np.int32))
 

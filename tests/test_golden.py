@@ -9,7 +9,12 @@ particles of all five materials beside the low walls, 2D and 3D.
   * float64 oracle: particle fields and the grid (mass, velocity after the
     grid operator) within 1e-9 relative (the two differ in summation order
     and SVD algorithm only);
-  * float32 oracle: within the parity bar of the HIP path (1e-5, C 3e-5).
+  * float32 oracle: within the parity bar of the HIP path: 1e-5 for x, v,
+    F, Jp, grid mass and grid momentum (m v); 3e-5 for C and the grid
+    velocity.  The velocity's rounding error sits on the light nodes at the
+    rim of the particle cloud (mass a few per cent of the heaviest, where
+    p / m divides rounding noise by a small number); weighted by mass, as
+    the momentum, it is within 1e-5.
 """
 import numpy as np
 import pytest
@@ -43,9 +48,10 @@ def test_oracle_matches_golden_substep(oracle, dim, dtype, tol, tol_c):
     errs = {f: rel_err(out[f], g[f"out_{f}"]) for f in FIELDS}
     errs["grid_m"] = rel_err(gm, g["grid_m"])
     errs["grid_v"] = rel_err(gv, g["grid_v"])
+    errs["grid_p"] = rel_err(gv * gm[:, None], g["grid_v"] * g["grid_m"][:, None])
     print(dim, np.dtype(dtype).name, {k: f"{e:.1e}" for k, e in errs.items()})
     for k, e in errs.items():
-        assert e <= (tol_c if k == "C" else tol), (k, e)
+        assert e <= (tol_c if k in ("C", "grid_v") else tol), (k, e)
     # the oracle's grid holds no mass outside the fixture's nodes
     assert abs(total - g["grid_m"].sum()) <= tol * g["grid_m"].sum()
 
@@ -54,6 +60,10 @@ def test_golden_fixtures_exercise_every_branch():
     for dim in (2, 3):
         g = load(dim)
         assert set(np.unique(g["in_material"])) == {0, 1, 2, 3, 4}
+        # stationary and elastic particles carry Jp != 1 (stationary ones
+        # are hardened by it, elastic ones are not)
+        for m in (1, 4):
+            assert np.abs(g["in_Jp"][g["in_material"] == m] - 1).max() > 0.03
         # snow clamped (Jp changed), sand projected (Jp moved), walls zeroed
         snow, sand = g["in_material"] == 2, g["in_material"] == 3
         assert (g["out_Jp"][snow] != g["in_Jp"][snow]).any()

@@ -1,0 +1,366 @@
+"""GPU tests at the benchmark workloads' own sizes (the inputs bench.py
+builds, same generators and seeds), against the oracle.
+
+3D four-material plain (1M particles, 128^3):
+  * one substep from a pre-deformed state (F = R diag(1 +- 0.05) Q^T,
+    C ~ N(0, 5), Jp as in the small parity tests, the workload's velocities):
+    x, F, Jp, grid mass and grid momentum within 1e-5 of the float32
+    oracle; v and C within 3e-5 of it and no further from the float64 oracle
+    than 1.5x the float32 oracle is (v and C are averages of the grid
+    velocity that cancel most of its magnitude, so their float32 rounding
+    is ~1e-5 relative at this size: the float32 oracle is 0.7-1.3e-5 from
+    float64 here);
+  * 100 substeps from the workload's initial state (at rest F = I, v ~
+    U[-1, 1]^3): x within 1e-5, F and Jp within 1e-4 of both the float32
+    and the float64 oracle (the float32 oracle: 1e-6, 1e-5, 2.4e-5 from
+    float64); per-particle v and C are chaotic under snow's clamp and sand's
+    return map (the float32 oracle drifts 6e-4 in v and 2e-3 in C from
+    float64 over these 100 substeps), so they are bounded in bulk: per
+    material, the centre of mass within 1e-8 (absolute, domain units), the
+    mean velocity within 1e-5 of its magnitude and the mean kinetic energy
+    within 1e-3 relative of the float64 oracle (float32 oracle: 5e-10, 5e-7,
+    6e-5); and per particle the HIP path is no further from float64 in v
+    than 2x the float32 oracle is.
+3D sparse large domain (16M particles in 64 ellipsoids, 512^3 window,
+plane collider): raw p2g mass and momentum at full size; 50 substeps with
+the particles thrown at the plane, none ending more than one cell below it;
+three of the ellipsoids alone against the oracle on a grid window around
+them (one substep at the parity bar, then 10 substeps within 1e-4).
+Batched (64 scenes, 10k-500k particles each, 64^3-128^3): the batch
+bitwise equal to 64 separate solvers after 6 substeps, and one batched
+substep of three of the scenes against the oracle.
+Differentiable (200k particles, 128^3, 64 substeps, checkpoints every 8):
+directional derivatives in v0, E and nu against float64 central finite
+differences of the oracle at 1e-3.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from tests.helpers import deformed_state, oracle_cfg_for, rel_err
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+THREADS = min(16, os.cpu_count() or 1)
+FIELDS = ("x", "v", "C", "F", "Jp")
+
+
+@pytest.fixture(autouse=True)
+def _quiet():
+    from taichi_elements_amd import mpm_solver
+    mpm_solver.PRINT_PROGRESS = False
+
+
+def _bench():
+    import bench
+    return bench
+
+
+def _to64(st):
+    return {k: (np.asarray(a, np.float64) if k != "material" else a) for k, a in st.items()}
+
+
+def _errs(a, b, fields=FIELDS, sel=None):
+    if sel is None:
+        return {k: rel_err(a[k], b[k]) for k in fields}
+    return {k: rel_err(a[k][sel], b[k][sel]) for k in fields}
+
+
+def _fmt(d):
+    return {k: f"{e:.2e}" for k, e in d.items()}
+
+
+# ----------------------------------------------------------------------------
+# 3D four-material plain substep, 1M particles, 128^3
+# ----------------------------------------------------------------------------
+def test_dense1m_one_substep_from_deformed_state(oracle):
+    from taichi_elements_amd import MPMSolver
+    x, v, m = _bench().make_scene()
+    n = len(x)
+    st = deformed_state(np.random.default_rng(5), x, m, 3)
+    st["v"] = v
+    s = MPMSolver(res=(128, 128, 128))
+    s.read_restart(n, st["x"], st["v"], st["material"], np.zeros(n, np.int32))
+    s.set_state(F=st["F"], C=st["C"], Jp=st["Jp"])
+    dt = s.default_dt
+    s.substep(dt)
+    got = s.get_state()
+    gm_hip, gv_hip = s.read_grid([-8] * 3, [144] * 3)
+    cfg = oracle_cfg_for(oracle, s)
+    print(f"dense1m one substep: {n} particles, oracle on {THREADS} threads", flush=True)
+    ref, gm, gv, _ = oracle.substep(cfg, st, dt, 0.0, np.float32, nthreads=THREADS)
+    ref64, gm64, gv64, _ = oracle.substep(cfg, _to64(st), dt, 0.0, np.float64, nthreads=THREADS)
+    e32, e64, o64 = _errs(got, ref), _errs(got, ref64), _errs(ref, ref64)
+    e32["grid_m"] = rel_err(gm_hip, gm)
+    e32["grid_p"] = rel_err(gv_hip * gm_hip[..., None], gv * gm[..., None])
+    print("  hip vs f32 oracle", _fmt(e32), flush=True)
+    print("  hip vs f64 oracle", _fmt(e64), flush=True)
+    print("  f32 oracle vs f64", _fmt(o64), flush=True)
+    for k in ("x", "F", "Jp", "grid_m", "grid_p"):
+        assert e32[k] <= 1e-5, (k, e32[k])
+    for k in ("v", "C"):
+        assert e32[k] <= 3e-5, (k, e32[k])
+        assert e64[k] <= 1.5 * o64[k] + 1e-7, (k, e64[k], o64[k])
+
+
+def _bulk(st, m):
+    out = {}
+    for mat in range(4):
+        sel = m == mat
+        xx = st["x"][sel].astype(np.float64)
+        vv = st["v"][sel].astype(np.float64)
+        out[mat] = (xx.mean(0), vv.mean(0), (vv * vv).sum(1).mean())
+    return out
+
+
+def test_dense1m_100_substeps(oracle):
+    from taichi_elements_amd import MPMSolver
+    x, v, m = _bench().make_scene()
+    n = len(x)
+    s = MPMSolver(res=(128, 128, 128))
+    s.read_restart(n, x, v, m, np.zeros(n, np.int32))
+    dt = s.default_dt
+    for _ in range(100):
+        s.substep(dt)
+    got = s.get_state()
+    cfg = oracle_cfg_for(oracle, s)
+    st = oracle.rest_state(x, v, m)
+    print(f"dense1m 100 substeps: oracle float32 and float64 on {THREADS} threads", flush=True)
+    r32 = oracle.Substepper(cfg, st, np.float32, nthreads=THREADS)
+    r64 = oracle.Substepper(cfg, _to64(st), np.float64, nthreads=THREADS)
+    for k in range(100):
+        r32.substep(dt)
+        r64.substep(dt)
+    e32, e64, o64 = _errs(got, r32.s), _errs(got, r64.s), _errs(r32.s, r64.s)
+    print("  hip vs f32 oracle", _fmt(e32), flush=True)
+    print("  hip vs f64 oracle", _fmt(e64), flush=True)
+    print("  f32 oracle vs f64", _fmt(o64), flush=True)
+    for k, bar in (("x", 1e-5), ("F", 1e-4), ("Jp", 1e-4)):
+        assert e32[k] <= bar and e64[k] <= bar, (k, e32[k], e64[k])
+    assert e64["v"] <= 2 * o64["v"], (e64["v"], o64["v"])
+    b, b64 = _bulk(got, m), _bulk(r64.s, m)
+    for mat in range(4):
+        com = np.abs(b[mat][0] - b64[mat][0]).max()
+        mom = np.abs(b[mat][1] - b64[mat][1]).max() / np.abs(b64[mat][1]).max()
+        ke = abs(b[mat][2] - b64[mat][2]) / b64[mat][2]
+        print(f"  material {mat}: centre of mass {com:.2e}, mean velocity {mom:.2e}, "
+              f"kinetic energy {ke:.2e} (vs f64)", flush=True)
+        assert com <= 1e-8 and mom <= 1e-5 and ke <= 1e-3, (mat, com, mom, ke)
+
+
+# ----------------------------------------------------------------------------
+# 3D sparse large domain, 16M particles, 512^3
+# ----------------------------------------------------------------------------
+PLANE_Y = 0.05
+
+
+def _sparse_solver(**kw):
+    from taichi_elements_amd import MPMSolver
+    s = MPMSolver(res=(512, 512, 512), **kw)
+    s.add_surface_collider((0.0, PLANE_Y, 0.0), (0.0, 1.0, 0.0), surface=s.surface_slip)
+    return s
+
+
+def test_sparse16m_conservation_and_plane_collider():
+    x, _, m, parts = _bench().make_sparse_scene(parts=True)
+    n = len(x)
+    assert n == 16_000_000
+    # every ellipsoid thrown at the plane with its own velocity
+    rng = np.random.default_rng(21)
+    vel = rng.uniform(-0.5, 0.5, (64, 3)).astype(np.float32)
+    vel[:, 1] = -2.0
+    v = np.repeat(vel, parts["counts"], axis=0)
+    s = _sparse_solver()
+    s.read_restart(n, x, v, m, np.zeros(n, np.int32))
+    s.set_debug(1)  # raw p2g sums: mass and momentum
+    s.substep(s.default_dt)
+    mass, mom = 0.0, np.zeros(3)
+    for z0 in range(-8, 520, 88):  # the window in slabs of 88 node layers
+        gm, gp = s.read_grid([-8, -8, z0], [528, 528, 88])
+        mass += gm.sum(dtype=np.float64)
+        mom += gp.reshape(-1, 3).sum(0, dtype=np.float64)
+    print(f"sparse16m raw p2g: mass {mass:.9e} (expected {n * s.p_mass:.9e})", flush=True)
+    np.testing.assert_allclose(mass, n * s.p_mass, rtol=1e-5)
+    want = s.p_mass * v.astype(np.float64).sum(0)
+    np.testing.assert_allclose(mom, want, rtol=1e-5, atol=1e-5 * np.abs(want).max())
+    # 50 substeps with the collider (a fresh solver, normal grid operator)
+    s = _sparse_solver()
+    s.read_restart(n, x, v, m, np.zeros(n, np.int32))
+    dt = s.default_dt
+    for _ in range(50):
+        s.substep(dt)
+    y = s.particle_info()["position"][:, 1]
+    assert np.all(np.isfinite(y))
+    above = x[:, 1] >= PLANE_Y
+    free_fall = x[:, 1] - 2.0 * 50 * dt  # where they would be without the plane
+    crossing = above & (free_fall < PLANE_Y - s.dx)
+    low = (PLANE_Y - y[above]).max() / s.dx
+    print(f"sparse16m: {crossing.sum()} particles would pass the plane by more than a cell; "
+          f"lowest of those starting above it ends {low:.3f} cells below", flush=True)
+    assert crossing.sum() > 1000
+    assert low <= 1.0
+
+
+def test_sparse16m_ellipsoids_against_the_oracle(oracle):
+    x, _, m, parts = _bench().make_sparse_scene(parts=True)
+    ends = np.cumsum(parts["counts"])
+    starts = ends - parts["counts"]
+    # three ellipsoids: the one reaching lowest (the plane cuts through it,
+    # so the collider acts) and its two nearest neighbours
+    c = parts["centres"]
+    low = int(np.argmin(c[:, 1] - parts["radii"][:, 1]))
+    pick = [int(k) for k in np.argsort(((c - c[low]) ** 2).sum(1))[:3]]
+    idx = np.concatenate([np.arange(starts[k], ends[k]) for k in pick])
+    xs, ms = x[idx], m[idx]
+    n = len(xs)
+    st = deformed_state(np.random.default_rng(22), xs, ms, 3, stretch=0.02, c_scale=1.0)
+    st["v"][:, 1] -= 1.0
+    s = _sparse_solver()
+    s.read_restart(n, st["x"], st["v"], st["material"], np.zeros(n, np.int32))
+    s.set_state(F=st["F"], C=st["C"], Jp=st["Jp"])
+    cfg = oracle_cfg_for(oracle, s)
+    lo = np.floor(xs.min(0) * 512).astype(int) - 12
+    hi = np.ceil(xs.max(0) * 512).astype(int) + 12
+    for a in range(3):
+        cfg.grid_lo[a] = int(lo[a])
+        cfg.grid_n[a] = int(hi[a] - lo[a])
+    print(f"sparse16m ellipsoids {pick}: {n} particles, oracle window {lo} + "
+          f"{(hi - lo).tolist()}", flush=True)
+    dt = s.default_dt
+    s.substep(dt)
+    ref, *_ = oracle.substep(cfg, st, dt, 0.0, np.float32, nthreads=THREADS)
+    e = _errs(s.get_state(), ref)
+    print("  one substep vs f32 oracle", _fmt(e), flush=True)
+    for k, err in e.items():
+        assert err <= (3e-5 if k == "C" else 1e-5), (k, err)
+    run = oracle.Substepper(cfg, ref, np.float32, nthreads=THREADS)
+    for _ in range(10):
+        s.substep(dt)
+        run.substep(dt)
+    e = _errs(s.get_state(), run.s, fields=("x", "v", "F", "Jp"))
+    print("  11 substeps vs f32 oracle", _fmt(e), flush=True)
+    for k, err in e.items():
+        assert err <= 1e-4, (k, err)
+
+
+# ----------------------------------------------------------------------------
+# batched: 64 scenes
+# ----------------------------------------------------------------------------
+def _batch(specs):
+    from taichi_elements_amd import MPMSolverBatch
+    b = MPMSolverBatch([dict(res=(sp["res"], ) * 3) for sp in specs])
+    for sc, sp in zip(b, specs):
+        sc.read_restart(len(sp["x"]), sp["x"], sp["v"], sp["m"], np.zeros(len(sp["x"]), np.int32))
+    return b
+
+
+def test_batched64_equals_separate_solvers():
+    from taichi_elements_amd import MPMSolver
+    specs = _bench().make_batched_scenes()
+    total = sum(len(sp["x"]) for sp in specs)
+    print(f"batched64: {len(specs)} scenes, {total} particles", flush=True)
+    b = _batch(specs)
+    b.substeps(6)
+    for k, (sc, sp) in enumerate(zip(b, specs)):
+        s = MPMSolver(res=(sp["res"], ) * 3)
+        s.read_restart(len(sp["x"]), sp["x"], sp["v"], sp["m"], np.zeros(len(sp["x"]), np.int32))
+        for _ in range(6):
+            s.substep(s.default_dt)
+        a, r = sc.get_state(), s.get_state()
+        for f in FIELDS:
+            assert np.array_equal(a[f], r[f]), (k, f)
+        del s
+    b.close()
+
+
+def test_batched64_three_scenes_against_the_oracle(oracle):
+    specs = _bench().make_batched_scenes()
+    order = np.argsort([len(sp["x"]) for sp in specs])
+    pick = [int(order[0]), int(order[len(order) // 2]), int(order[-1])]
+    b = _batch(specs)
+    states = {}
+    for k in pick:
+        sp = specs[k]
+        st = deformed_state(np.random.default_rng(30 + k), sp["x"], sp["m"], 3)
+        st["v"] = sp["v"]
+        b[k].set_state(F=st["F"], C=st["C"], Jp=st["Jp"])
+        states[k] = st
+    b.substep()
+    for k in pick:
+        sc = b[k]
+        print(f"batched64 scene {k}: {len(specs[k]['x'])} particles, res {specs[k]['res']}",
+              flush=True)
+        ref, *_ = oracle.substep(oracle_cfg_for(oracle, sc), states[k], sc.default_dt, 0.0,
+                                 np.float32, nthreads=THREADS)
+        e = _errs(sc.get_state(), ref)
+        print("  vs f32 oracle", _fmt(e), flush=True)
+        for f, err in e.items():
+            assert err <= (3e-5 if f == "C" else 1e-5), (k, f, err)
+    b.close()
+
+
+# ----------------------------------------------------------------------------
+# differentiable: 200k particles, 128^3, 64 substeps, checkpoints every 8
+# ----------------------------------------------------------------------------
+def _oracle_loss(O, x, v, mat, E, nu, target, n_el, nsub, dt):
+    cfg = O.with_youngs(O.make_cfg((128, ) * 3), np.asarray(E, np.float64), nu)
+    st = _to64(O.rest_state(x, v, mat))
+    st["x"] = np.asarray(x, np.float64)
+    st["v"] = np.asarray(v, np.float64)
+    run = O.Substepper(cfg, st, np.float64, nthreads=THREADS)
+    for _ in range(nsub):
+        run.substep(dt)
+    com = run.s["x"][:n_el].mean(0)
+    return float(((com - target) ** 2).sum())
+
+
+def test_fwdbwd200k_gradients_against_finite_differences(oracle):
+    from taichi_elements_amd.diff import DiffMPMSolver
+    x, v, mat, E, target, n_el = _bench().make_diff_scene()
+    nsub, nu0 = 64, 0.2
+    ds = DiffMPMSolver((128, 128, 128))
+    ds.setup(mat, x)
+    dt = ds.default_dt
+    dev = ds.device
+    v_t = torch.tensor(v, device=dev, requires_grad=True)
+    E_t = torch.tensor(E, device=dev, requires_grad=True)
+    nu_t = torch.tensor(nu0, dtype=torch.float32, device=dev, requires_grad=True)
+    s0 = ds.state(x, v)
+    s0 = torch.cat([s0[:3], v_t.T, s0[6:]], 0)
+    s = ds.run(s0, E_t, nu_t, nsub, dt=dt, checkpoint_every=8)
+    assert ds.last_checkpoint_every == 8
+    loss = ((ds.positions(s)[:n_el].mean(0) - torch.tensor(target, device=dev)) ** 2).sum()
+    loss.backward()
+    ds.check()
+    g = dict(v=v_t.grad.double().cpu().numpy(), E=E_t.grad.double().cpu().numpy(),
+             nu=float(nu_t.grad))
+    x64, v64, E64 = x.astype(np.float64), v.astype(np.float64), E.astype(np.float64)
+    L0 = _oracle_loss(oracle, x64, v64, mat, E64, nu0, target, n_el, nsub, dt)
+    print(f"fwdbwd200k: loss hip {float(loss.detach()):.6e} oracle f64 {L0:.6e}", flush=True)
+    assert abs(float(loss.detach()) - L0) <= 1e-4 * L0
+    rng = np.random.default_rng(40)
+    n = len(x)
+    # v0: the elastic body pushed along a random direction (plus noise);
+    # E: a random positive rescaling of the elastic particles' moduli
+    dv = np.zeros((n, 3))
+    dv[:n_el] = rng.normal(size=3) + 0.3 * rng.normal(size=(n_el, 3))
+    dE = np.zeros(n)
+    dE[:n_el] = rng.uniform(0.5, 1.5, n_el) * E64[:n_el]
+    for name, delta, eps in (("v", dv, 1e-4), ("E", dE, 1e-4), ("nu", 1.0, 1e-5)):
+        if name == "v":
+            f = lambda e: _oracle_loss(oracle, x64, v64 + e * delta, mat, E64, nu0, target,
+                                       n_el, nsub, dt)
+        elif name == "E":
+            f = lambda e: _oracle_loss(oracle, x64, v64, mat, E64 + e * delta, nu0, target,
+                                       n_el, nsub, dt)
+        else:
+            f = lambda e: _oracle_loss(oracle, x64, v64, mat, E64, nu0 + e, target, n_el,
+                                       nsub, dt)
+        fd = (f(eps) - f(-eps)) / (2 * eps)
+        an = float(np.sum(g[name] * delta))
+        print(f"  d/d{name}: adjoint {an:.6e}  fd {fd:.6e}  rel {abs(an - fd) / abs(fd):.2e}",
+              flush=True)
+        assert abs(an - fd) <= 1e-3 * abs(fd), name

@@ -152,8 +152,128 @@ def test_unsupported_setups_are_rejected():
     from taichi_elements_amd.diff import DiffMPMSolver
     ds = DiffMPMSolver((32, 32, 32))
     with pytest.raises(ValueError):
-        ds.setup(np.array([0, 1, 2], np.int32))  # snow
-    ds2 = DiffMPMSolver((32, 32, 32))
-    ds2.solver.add_sphere_collider((0.5, 0.5, 0.5), 0.1)
-    with pytest.raises(ValueError):
-        ds2.setup(np.array([0, 1], np.int32))
+        ds.setup(np.array([0, 1, 4], np.int32))  # stationary
+
+
+def _plastic_scene(dim, seed, res, n_each):
+    """A snow cube and a sand cube (pre-deformed: F = R diag(1 +- 0.04) Q^T,
+    snow Jp in [0.95, 1.05], sand Jp in [-0.04, -0.005], so that the sand
+    particles sit at the cone tip, inside the cone and on its surface) with
+    a water cube between them, all moving down onto a slip plane with
+    friction and past a separating sphere."""
+    from tests.helpers import random_rotations
+    rng = np.random.default_rng(seed)
+    lo = [[0.2, 0.16, 0.3], [0.42, 0.16, 0.3], [0.64, 0.16, 0.3]]
+    x = np.concatenate([cube_particles(rng, n_each, lo[k][:dim], [0.16] * dim, dim)
+                        for k in range(3)])
+    mat = np.repeat(np.array([2, 0, 3], np.int32), n_each)
+    n = len(x)
+    v = rng.uniform(-0.3, 0.3, (n, dim)).astype(np.float32)
+    v[:, 1] -= 0.6
+    R, Q = random_rotations(rng, n, dim), random_rotations(rng, n, dim)
+    sig = 1 + 0.04 * rng.uniform(-1, 1, (n, dim))
+    F = np.einsum("nij,nj,nkj->nik", R, sig, Q).astype(np.float32)
+    F[mat == 0] = np.eye(dim, dtype=np.float32)
+    Jp = np.ones(n, np.float32)
+    Jp[mat == 2] = rng.uniform(0.95, 1.05, (mat == 2).sum())
+    Jp[mat == 3] = rng.uniform(-0.04, -0.005, (mat == 3).sum())
+    E = rng.uniform(5e5, 2e6, n).astype(np.float32)
+    targets = [x[mat == m].mean(0) + rng.uniform(-0.05, 0.05, dim).astype(np.float32)
+               for m in (2, 3)]
+    return x, v, F, Jp, mat, E, targets
+
+
+def _oracle_plastic_loss(O, cfg, st, E, nu, targets, nsub, dt):
+    cfg = O.with_youngs(cfg, np.asarray(E, np.float64), nu)
+    run = O.Substepper(cfg, {k: (np.asarray(a, np.float64) if k != "material" else a)
+                             for k, a in st.items()}, np.float64)
+    for _ in range(nsub):
+        run.substep(dt)
+    mat = st["material"]
+    return float(sum(((run.s["x"][mat == m].mean(0) - t) ** 2).sum()
+                     for m, t in zip((2, 3), targets)))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_plastic_and_collider_gradients_against_finite_differences(oracle, dim):
+    """Snow (clamp, hardening), sand (return map, its coefficient's nu),
+    a slip plane with friction and a separating sphere: directional
+    derivatives of L = sum |COM_m - target_m|^2 (m = snow, sand) after 6
+    substeps in v0, F0, Jp0, E and nu against float64 central finite
+    differences of the oracle, at 1e-3."""
+    from taichi_elements_amd.diff import DiffMPMSolver, n_fields
+    from tests.helpers import oracle_cfg_for
+    res, nsub, nu0 = 32, 6, 0.2
+    n_each = 300 if dim == 3 else 200
+    x, v, F, Jp, mat, E, targets = _plastic_scene(dim, 70 + dim, res, n_each)
+    ds = DiffMPMSolver((res, ) * dim)
+    ds.solver.add_surface_collider([0.0, 0.15, 0.0][:dim], [0.1, 1.0, 0.05][:dim],
+                                   surface=ds.solver.surface_slip, friction=0.4)
+    ds.solver.add_sphere_collider([0.5, 0.2, 0.4][:dim], 0.06,
+                                  surface=ds.solver.surface_separate)
+    ds.setup(mat, x)
+    dt = ds.default_dt
+    dev = ds.device
+    d, n = dim, len(x)
+    s0 = ds.state(x, v).cpu()
+    f0 = 2 * d + d * d
+    s0[f0:f0 + d * d] = torch.as_tensor(F.reshape(n, d * d).T)
+    s0[-1] = torch.as_tensor(Jp)
+    s0 = s0.to(dev).requires_grad_(True)
+    E_t = torch.tensor(E, device=dev, requires_grad=True)
+    nu_t = torch.tensor(nu0, dtype=torch.float32, device=dev, requires_grad=True)
+    s = ds.run(s0, E_t, nu_t, nsub, dt=dt, checkpoint_every=3)
+    pos = ds.positions(s)
+    tg = [torch.tensor(t, device=dev) for t in targets]
+    loss = sum(((pos[torch.as_tensor(mat == m, device=dev)].mean(0) - t) ** 2).sum()
+               for m, t in zip((2, 3), tg))
+    loss.backward()
+    ds.check()
+    gs = s0.grad.double().cpu().numpy()
+    g = dict(v=gs[d:2 * d].T, F=gs[f0:f0 + d * d].T.reshape(n, d, d), Jp=gs[-1],
+             E=E_t.grad.double().cpu().numpy(), nu=float(nu_t.grad))
+    cfg = oracle_cfg_for(oracle, ds.solver)
+    st = dict(x=x.astype(np.float64), v=v.astype(np.float64), C=np.zeros((n, d, d)),
+              F=F.astype(np.float64), Jp=Jp.astype(np.float64), material=mat)
+    E64 = E.astype(np.float64)
+    L0 = _oracle_plastic_loss(oracle, cfg, st, E64, nu0, targets, nsub, dt)
+    print(f"dim={dim} loss gpu {float(loss.detach()):.6e} oracle f64 {L0:.6e}", flush=True)
+    assert abs(float(loss.detach()) - L0) <= 1e-4 * L0
+    # directions that move whole bodies (so that the derivative does not
+    # cancel to a small remainder of its terms): the snow and sand bodies
+    # pushed along their own random directions, one symmetric F
+    # perturbation of every snow and sand particle, their Jp raised, every
+    # E scaled by U[0.5, 1.5]; each with 30% per-particle noise where it
+    # applies.  eps = 1e-5: the finite differences agree with those at 1e-6
+    # to 2e-4 or better (nearer the kinks of the clamp, the cone and the
+    # colliders a larger step would straddle them)
+    rng = np.random.default_rng(98)
+    solid = (mat == 2) | (mat == 3)
+    for name in ("v", "F", "Jp", "E", "nu"):
+        eps = 1e-5
+        if name == "nu":
+            delta = 1.0
+            f = lambda e: _oracle_plastic_loss(oracle, cfg, st, E64, nu0 + e, targets, nsub, dt)
+        elif name == "E":
+            delta = rng.uniform(0.5, 1.5, n) * E64
+            f = lambda e: _oracle_plastic_loss(oracle, cfg, st, E64 + e * delta, nu0, targets,
+                                               nsub, dt)
+        else:
+            if name == "v":
+                delta = 0.3 * rng.normal(size=(n, d))
+                for m in (2, 3):
+                    delta[mat == m] += rng.normal(size=d)
+            elif name == "F":
+                S = rng.normal(size=(d, d))
+                delta = np.zeros((n, d, d))
+                delta[solid] = S + S.T
+            else:
+                delta = np.where(solid, 1.0 + 0.3 * rng.normal(size=n), 0.0)
+            f = lambda e: _oracle_plastic_loss(oracle, cfg, {**st, name: st[name] + e * delta},
+                                               E64, nu0, targets, nsub, dt)
+        fd = (f(eps) - f(-eps)) / (2 * eps)
+        an = float(np.sum(g[name] * delta))
+        print(f"  d/d{name}: adjoint {an:.6e}  fd {fd:.6e}  rel {abs(an - fd) / abs(fd):.2e}",
+              flush=True)
+        assert abs(an - fd) <= 1e-3 * abs(fd), name
+    assert s.shape == (n_fields(dim), n)

@@ -119,6 +119,27 @@ def test_uniform_dilation_closed_form(oracle, dim):
 
 
 @pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("g2p2g", [0, 1])
+def test_hardening_of_stationary_and_elastic(oracle, dim, g2p2g):
+    """h = exp(10 (1 - Jp)) for every material but water in p2g and for
+    every material in g2p2g (mpm:515-518, 419-421), then h = 0.3 for elastic
+    (mpm:519-521).  A stationary particle at F = s I with Jp = j carries
+    tau = h (2 mu_0 (s - 1) s + lambda_0 s^d (s^d - 1)) I and keeps Jp; an
+    elastic one carries the h = 0.3 stress whatever its Jp."""
+    cfg = _cfg(oracle, dim)
+    for j in (0.9, 0.97, 1.0, 1.08):
+        for s in (0.95, 1.03):
+            J = s ** dim
+            base = (2 * cfg.mu_0 * (s - 1) * s + cfg.lambda_0 * J * (J - 1)) * np.eye(dim)
+            for mat, h in ((oracle.STATIONARY, math.exp(10 * (1 - j))), (oracle.ELASTIC, 0.3)):
+                Fn, Jp, st = oracle.constitutive(cfg, mat, 0.0, s * np.eye(dim),
+                                                 np.zeros((dim, dim)), j, F64, g2p2g=g2p2g)
+                np.testing.assert_allclose(st, h * base, rtol=1e-12, atol=1e-6)
+                np.testing.assert_allclose(Fn, s * np.eye(dim), rtol=1e-13)
+                assert Jp == pytest.approx(j, rel=1e-15)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
 def test_water_volume_state(oracle, dim):
     """Water keeps its volume in F[0,0] and Jp (mpm:508-511, 537-542): with
     J = Jp det(I + dt C), tau = lambda_0 J (J - 1) I (mu = 0, h = 1)."""
