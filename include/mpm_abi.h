@@ -277,6 +277,21 @@ int mpm_diff_substep_backward(mpm_solver* s, float dt, const float* state_in,
                               const float* grad_out, float* grad_in, float* grad_youngs,
                               float* grad_nu, void* stream);
 int mpm_diff_sync(mpm_solver* s, void* stream);
+/* Allocation-free form of mpm_diff_substep (no counterpart in the reference:
+ * it is for callers that cannot allocate device memory -- code under HIP
+ * graph capture, tight serving loops).  Same state_out as mpm_diff_substep
+ * on the same inputs (bitwise: the same kernels in the same order), without
+ * the binning record.  Its workspace (binning, chunk list, scan and sort
+ * state, node tiles, grid) lives in a static 64 MiB device array that each
+ * GPU the library runs on holds until the process ends; the call performs
+ * no hipMalloc / hipFree.  A scene whose workspace exceeds 64 MiB is
+ * MPM_EINVAL (its size is in the message).  Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (other streams, other
+ * threads, other handles) race.  The scene table is uploaded (and the stream
+ * synchronised) only when dt, youngs or nu change from the handle's last
+ * call. */
+int mpm_diff_substep_noalloc(mpm_solver* s, float dt, const float* state_in, float* state_out,
+                             const float* youngs, double nu, void* stream);
 
 /* The quantised particle file of ParticleIO.write_particles
  * (engine/particle_io.py:12-76), computed on the GPU: ranges (2, dim, 2)

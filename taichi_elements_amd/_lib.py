@@ -75,6 +75,8 @@ SIGNATURES = [
     ("mpm_diff_substep_backward", ctypes.c_int, [_P, ctypes.c_float, _P, _P, ctypes.c_double, _P,
                                                  _P, _P, _P, _P, _P]),
     ("mpm_diff_sync", ctypes.c_int, [_P, _P]),
+    ("mpm_diff_substep_noalloc", ctypes.c_int, [_P, ctypes.c_float, _P, _P, _P,
+                                                ctypes.c_double, _P]),
     ("mpm_quantize_particles", ctypes.c_int, [_P, _FP, ctypes.POINTER(ctypes.c_uint32),
                                               ctypes.POINTER(ctypes.c_uint8)]),
     ("mpm_set_storage", ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),

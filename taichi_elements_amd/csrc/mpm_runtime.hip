@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -641,6 +642,82 @@ int voxel_events(const VoxGrid& g, int64_t ntri, const double* triangles, hipStr
   return MPM_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Static workspace of mpm_diff_substep_noalloc: a file-scope device array that
+// every GPU the library runs on holds until the process ends.  Host code uses
+// the device address hipGetSymbolAddress returns for it (resolved once per
+// device when a solver is created on that device), never the array's name.
+// ---------------------------------------------------------------------------
+constexpr int64_t kScratchBytes = 64ll << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+void* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// Resolves this device's copy of the workspace; the device must be current.
+int resolve_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices)
+    return fail(MPM_EINVAL, "device index beyond the workspace table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return MPM_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(MPM_EHIP, std::string("workspace address: ") + hipGetErrorString(e));
+  g_scratch_ptr[device] = p;
+  return MPM_OK;
+}
+
+// Carves 256-byte aligned arrays out of one region (base == nullptr: only
+// counts the bytes).
+struct Carve {
+  char* base;
+  int64_t off = 0;
+  template <typename T>
+  T* take(int64_t count) {
+    off = (off + 255) & ~(int64_t)255;
+    T* p = base ? (T*)(base + off) : nullptr;
+    off += (int64_t)sizeof(T) * std::max<int64_t>(count, 1);
+    return p;
+  }
+};
+
+// The workspace of one differentiable forward substep of handle s, carved
+// from c (binning, chunk list, scan and sort state, node tiles, grid,
+// per-workgroup maxima); the particle state stays in the caller's arrays.
+void carve_diff_workspace(const mpm_solver* s, Carve& c, SubstepBuffers& B) {
+  const int64_t n = s->diff_n, tb = s->total_blocks;
+  const int NPB = s->D == 3 ? Geo<3>::kNodesPerBlock : Geo<2>::kNodesPerBlock;
+  const int TN = s->D == 3 ? Geo<3>::kTileNodes : Geo<2>::kTileNodes;
+  const int64_t chunk_cap = std::min<int64_t>(tb, n) + (n + s->chunk - 1) / s->chunk;
+  const int64_t nparts = (tb + kScanTile - 1) / kScanTile;
+  const int64_t rtiles = (n + kRadixTile - 1) / kRadixTile;
+  B.keys = c.take<int>(n);
+  B.keys_alt = c.take<int>(2 * n);
+  B.perm = c.take<int>(n);
+  B.perm_alt = c.take<int>(n);
+  B.count = c.take<int>(tb + 1);
+  B.count_next = c.take<int>(tb + 1);
+  B.pstart = c.take<int>(tb);
+  B.pchunk_begin = B.pchunk_begin_prev = c.take<int>(tb);
+  B.pchunk_cnt = B.pchunk_cnt_prev = c.take<int>(tb);
+  B.chunks = B.chunks_prev = c.take<int4>(chunk_cap);
+  B.active = c.take<int>(tb);
+  B.tile_prefix = c.take<int4>(nparts);
+  B.totals = B.totals_prev = c.take<int>(16);
+  B.ghist = c.take<int>(2 * kMaxRadixPasses * kRadixBins);
+  B.lb_status_scan = c.take<unsigned long long>(4 * nparts);
+  B.lb_status_radix = c.take<unsigned long long>(kRadixBins * rtiles);
+  B.lbs = c.take<LBState>(1);
+  B.code = c.take<int>(n);
+  B.nkey = c.take<int>(n);
+  B.dcount = c.take<int>(27 * chunk_cap);
+  B.doff = nullptr;  // steady-state binning only
+  B.maxv = c.take<unsigned>(s->persistent_groups);
+  B.tiles = c.take<float4>(chunk_cap * TN);
+  B.grid = c.take<float4>(tb * NPB);
+}
+
 }  // namespace
 
 extern "C" {
@@ -861,6 +938,10 @@ static int create_impl(const mpm_config* cfgs, int nscenes, mpm_solver** out) {
   if (hipSetDevice(s->device) != hipSuccess || hipStreamCreate(&s->stream) != hipSuccess) {
     delete s;
     return fail(MPM_EHIP, "cannot create a HIP stream");
+  }
+  if ((r = resolve_scratch(s->device)) != MPM_OK) {
+    mpm_destroy(s);
+    return r;
   }
   if ((r = dalloc(&s->lbs, 1)) || (r = dalloc(&s->ghist, 2 * kMaxRadixPasses * kRadixBins)) ||
       (r = dalloc(&s->totals, 32)) || (r = dalloc(&s->err, 1)) ||
@@ -1757,6 +1838,49 @@ int mpm_diff_substep(mpm_solver* s, float dt, const float* state_in, float* stat
   }
   take_buffers(s, B);
   s->substep_counter++;
+  HIP_TRY(hipGetLastError());
+  return MPM_OK;
+}
+
+int mpm_diff_substep_noalloc(mpm_solver* s, float dt, const float* state_in, float* state_out,
+                             const float* youngs, double nu, void* stream) {
+  if (!s || !state_in || !state_out) return fail(MPM_EINVAL, "null argument");
+  if (s->diff_n == 0) return fail(MPM_EINVAL, "mpm_diff_setup first");
+  Carve need{nullptr};
+  SubstepBuffers B;
+  carve_diff_workspace(s, need, B);
+  if (need.off > kScratchBytes)
+    return fail(MPM_EINVAL, "mpm_diff_substep_noalloc: this scene needs " +
+                                std::to_string(need.off) + " bytes of workspace, beyond the " +
+                                std::to_string(kScratchBytes) +
+                                " bytes of the static workspace (use mpm_diff_substep)");
+  char* base = (char*)g_scratch_ptr[s->device];
+  if (!base) return fail(MPM_EINVAL, "the workspace of this device is not resolved");
+  hipStream_t st = (hipStream_t)stream;  // NULL: the null stream
+  HIP_TRY(hipSetDevice(s->device));
+  TRY(diff_params(s, dt, youngs, nu, st));
+  diff_buffers(s, B, const_cast<float*>(state_in));
+  B.pf_out = state_out;  // p2g / g2p write the new state out of place
+  Carve c{base};
+  carve_diff_workspace(s, c, B);
+  B.raw_out = nullptr;
+  B.step = nullptr;
+  B.parity = 0;
+  // what the kernels expect to start at zero (the workspace keeps the last
+  // call's contents): the digit histograms and the look-back scan and sort
+  // state (the block counts are cleared by the binning itself); fill kernels,
+  // not memsets, so that a captured graph replays them correctly
+  const int64_t tb = s->total_blocks;
+  const int64_t nparts = (tb + kScanTile - 1) / kScanTile;
+  const int64_t rtiles = (s->diff_n + kRadixTile - 1) / kRadixTile;
+  launch_fill_u32((unsigned*)B.ghist, 0u, 2 * kMaxRadixPasses * kRadixBins, st);
+  launch_fill_u32((unsigned*)B.lb_status_scan, 0u, 2 * 4 * nparts, st);
+  launch_fill_u32((unsigned*)B.lb_status_radix, 0u, 2 * kRadixBins * rtiles, st);
+  launch_fill_u32((unsigned*)B.lbs, 0u, sizeof(LBState) / 4, st);
+  if (s->D == 3)
+    launch_substep<3>(s->d_params, s->d_slab, B, 0, 0, st, nullptr);
+  else
+    launch_substep<2>(s->d_params, s->d_slab, B, 0, 0, st, nullptr);
   HIP_TRY(hipGetLastError());
   return MPM_OK;
 }
