@@ -245,8 +245,9 @@ int mpm_last_counts(mpm_solver* s, int64_t* out4);
 /* Differentiable mode (no reference counterpart: the reference's p2g is not
  * differentiable -- it mutates F and Jp in place -- and it has no per-particle
  * stiffness).  One substep is a function of a caller-owned device state:
- * state arrays are float32 SoA of NF = 2d + 2d^2 + 1 rows of n entries
- * (x[d], v[d], C[d*d] row-major, F[d*d], Jp), in particle order; youngs is
+ * state arrays are float32 SoA of NF = 2d + 2d^2 + 1 dense rows of n entries
+ * (x[d], v[d], C[d*d] row-major, F[d*d], Jp; row r starts at element r n),
+ * in particle order; youngs is
  * the per-particle Young's modulus E_p (device, n floats) and nu the global
  * Poisson ratio: mu_0 = E_p / (2 (1 + nu)), lambda_0 = E_p nu / ((1 + nu)(1 -
  * 2 nu)), scaled by h = 0.3 for elastic particles as the reference does.

@@ -1786,6 +1786,10 @@ int mpm_snapshot(mpm_solver* s, int32_t op) {
   s->n_final = s->snap_n_final;
   s->have_perm = false;  // slots moved: bin from scratch
   s->reordered = false;
+  // the saved slots need not be in block order (a snapshot taken right after
+  // particles were added holds insertion order, which makes every transfer's
+  // particle loads scattered): reorder at the first opportunity
+  s->since_reorder = 1 << 30;
   HIP_TRY(hipStreamSynchronize(s->stream));
   return MPM_OK;
 }

@@ -51,6 +51,9 @@ class _Substep(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, state, youngs, nu, solver, nu_value, dt):
+        # the kernels read the state as dense (NF, n) rows and E as n floats
+        if not (state.is_contiguous() and youngs.is_contiguous()):
+            raise ValueError("state and youngs must be contiguous tensors")
         out = torch.empty_like(state)
         # the substep's binning, kept for the backward pass (which then only
         # recomputes the grid)
@@ -197,6 +200,10 @@ class DiffMPMSolver:
         if self._order is not None:
             state = state[:, self._order]
             youngs = youngs[self._order]
+        # a transposed or sliced tensor is copied into the (NF, n) row layout
+        # the kernels read (the copy is part of the autograd graph)
+        state = state.contiguous()
+        youngs = youngs.contiguous()
         if checkpoint_every is None:
             per = state.numel() * state.element_size() + self._binning_bytes
             checkpoint_every = 1 if nsub * per <= self.keep_bytes else 8

@@ -108,11 +108,14 @@ def test_positions_on_cell_and_block_boundaries(oracle):
     ref64, *_ = oracle.substep(cfg, st64, dt, 0.0, np.float64)
     for f, tol in TOL.items():
         e = rel_err(got[f], ref[f])
-        if f == "C":  # as in the one-substep parity test: at least as close
-            # to the float64 oracle as the float32 oracle is (within 2x)
+        if f == "C":  # particles on cell faces and on the (k + 1/2) dx planes
+            # where the stencil switches put the weights' rounding at its
+            # worst: the float32 oracle is 2.5e-5 from float64 in C here, so
+            # C is held to 5e-5 of the float32 oracle and 3e-5 of float64
             e64, o64 = rel_err(got[f], ref64[f]), rel_err(ref[f], ref64[f])
             print(f"boundary C: vs f32 {e:.2e}, vs f64 {e64:.2e}, f32 vs f64 {o64:.2e}", flush=True)
-            assert e <= tol or e64 <= 2 * o64 + 1e-7, (f, e, e64, o64)
+            assert e <= 5e-5, (f, "vs f32", e)
+            assert e64 <= 3e-5, (f, "vs f64", e64)
         else:
             assert e <= tol, (f, e)
 

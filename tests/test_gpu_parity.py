@@ -11,8 +11,10 @@ Tolerances (float32 arithmetic, different summation order on the grid):
     the float64 oracle in C, so the test also requires the HIP path to be at
     least as close to float64 as the float32 oracle is (within 2x);
   * binning keys and permutation: bit-exact;
-  * 100 substeps: relative drift <= 1e-4 from the float32 or the float64
-    oracle (stated bound; measured values are printed).
+  * 100 substeps: x and v within 1e-4 of the float64 oracle and within
+    2e-4 of the float32 oracle (the float32 oracle is itself 1.3e-4 from
+    float64 in v after 100 substeps of the four-material scene, whose snow
+    and sand branches amplify rounding; measured values are printed).
 """
 import numpy as np
 import pytest
@@ -23,7 +25,8 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-5
 TOL_C = 3e-5
-DRIFT_TOL = 1e-4
+DRIFT_TOL = 1e-4      # 100 substeps, against the float64 oracle
+DRIFT_TOL_F32 = 2e-4  # 100 substeps, against the float32 oracle
 
 
 def _tol(k):
@@ -208,10 +211,8 @@ def _drift(oracle, s, st, nsteps, dt, label):
     print(label, "vs f32 oracle", {k: f"{e:.2e}" for k, e in e32.items()}, flush=True)
     print(label, "vs f64 oracle", {k: f"{e:.2e}" for k, e in e64.items()}, flush=True)
     for k in ("x", "v"):
-        # within the drift bound of the float32 oracle, or of the float64 one
-        # (a path more accurate than the float32 oracle may drift from its
-        # particular rounding while staying closer to float64)
-        assert e32[k] <= DRIFT_TOL or e64[k] <= DRIFT_TOL, (k, e32[k], e64[k])
+        assert e64[k] <= DRIFT_TOL, (k, "vs f64", e64[k])
+        assert e32[k] <= DRIFT_TOL_F32, (k, "vs f32", e32[k])
     return e32, e64
 
 
