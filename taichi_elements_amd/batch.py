@@ -300,7 +300,7 @@ class MPMSolverBatch:
     def tuning(self):
         return MPMSolver.tuning(self)
 
-    def autotune(self, substeps=4, chunks=(64, 128, 256), groups=(1024, 2048, 4096),
+    def autotune(self, substeps=4, chunks=None, groups=(1024, 2048, 4096),
                  verbose=False):
         """MPMSolver.autotune for the whole batch (each scene at its
         default_dt); the state of every scene is restored afterwards."""

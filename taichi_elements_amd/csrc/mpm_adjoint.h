@@ -474,8 +474,7 @@ __global__ __launch_bounds__(kThreads) void k_p2g_adj(
       gt[i] = gbar[(int64_t)gb * NPB + node];
     }
     __syncthreads();
-    const int j = ck.y + threadIdx.x;
-    if (j < ck.z) {
+    for (int j = ck.y + threadIdx.x; j < ck.z; j += kThreads) {  // the chunk, kThreads at a time
       const int p = perm[j];
       const int mat = pi[(int64_t)L::MAT * cap + p];
       float mass = P.p_mass;

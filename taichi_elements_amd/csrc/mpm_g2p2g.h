@@ -44,12 +44,12 @@ __global__ __launch_bounds__(kThreads) void k_g2p2g(
   constexpr int WAVES = kThreads / 64;
   constexpr int SUBS = WAVES * GROUPS;
   __shared__ float tin_x[TN], tin_y[TN], tin_z[TN];  // input grid velocity (as k_g2p)
-  __shared__ float4 stage[4][kChunkMax];  // [record part][particle]: lane-contiguous
+  __shared__ float4 stage[4][kBatch];  // [record part][particle]: lane-contiguous
   __shared__ float4 ptile[SUBS][TN];  // inner particles: nodes 1..6 of the output tile
   __shared__ float4 stile[TN8];       // particles that crossed a cell face (LDS atomics)
   __shared__ int wcnt[WAVES][CELLS];
   __shared__ int cstart[CELLS], ccount[CELLS];
-  __shared__ unsigned char order[kChunkMax];
+  __shared__ unsigned char order[kBatch];
   __shared__ int dwcnt[WAVES][32];
   __shared__ float red[WAVES];
 

@@ -10,7 +10,8 @@
 namespace mpm {
 
 constexpr int kThreads = 256;
-constexpr int kChunkMax = 256;                      // particles per transfer work item (max)
+constexpr int kBatch = kThreads;  // particles a transfer workgroup holds at a time
+constexpr int kChunkMax = 1024;  // particles per transfer work item (max), kBatch at a time
 constexpr int kScanItems = 2;                       // per thread in the block scan
 constexpr int kScanTile = kThreads * kScanItems;    // 512 blocks per workgroup
 constexpr int kRadixItems = 16;

@@ -462,7 +462,8 @@ __global__ __launch_bounds__(kThreads) void k_onesweep(
 // ---------------------------------------------------------------------------
 // P2G (mpm_solver.py:487-584)
 //
-// One workgroup per chunk (<= kChunkMax particles of one grid block).
+// One workgroup per chunk (<= kChunkMax particles of one grid block), taken
+// kBatch particles at a time through phases 1-3; one node tile per chunk.
 //  phase 1  one particle per thread: constitutive update (F, Jp written back)
 //           and the particle's affine momentum terms staged in LDS;
 //  phase 2  a stable counting sort of the chunk's particles by cell;
@@ -495,18 +496,17 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
   constexpr int GROUPS = 64 / CELLS;             // lane groups per wave
   constexpr int WAVES = kThreads / 64;
   constexpr int SUBS = WAVES * GROUPS;           // private tiles
-  __shared__ float4 stage[4][kChunkMax];  // [record part][particle]: lane-contiguous
+  __shared__ float4 stage[4][kBatch];  // [record part][particle]: lane-contiguous
   __shared__ float4 ptile[SUBS][TN];
   __shared__ int wcnt[WAVES][CELLS];
   __shared__ int cstart[CELLS], ccount[CELLS];
-  __shared__ unsigned char order[kChunkMax];
+  __shared__ unsigned char order[kBatch];
 
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   const int nchunks = totals[kTotChunks];
   for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
     const int4 ck = chunks[c];
-    const int nc = ck.z - ck.y;  // <= kChunkMax
     int lbc[3], b[3];
     decode_local<D>(ck.x, scenes[0], lbc);
     const Params& P = scenes[scene_of<D>(lbc, slab)];
@@ -516,149 +516,153 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     for (int a = 0; a < D; a++) b[a] = (b[a] + P.blo[a]) * kBlock;  // first node of the block
     for (int i = threadIdx.x; i < SUBS * TN; i += kThreads)
       (&ptile[0][0])[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    for (int i = threadIdx.x; i < WAVES * CELLS; i += kThreads) (&wcnt[0][0])[i] = 0;
+    // the chunk's particles, kBatch at a time, all into the same private tiles
+    for (int b0 = ck.y; b0 < ck.z; b0 += kBatch) {
+      const int nc = min(kBatch, ck.z - b0);
+      for (int i = threadIdx.x; i < WAVES * CELLS; i += kThreads) (&wcnt[0][0])[i] = 0;
 
-    // ---- phase 1: constitutive update and staging ----
-    const int t = threadIdx.x;
-    const bool valid = t < nc;
-    int cell = 0;
-    if (MODE == 1 && valid) {
-      const int p = perm[ck.y + t];
-      const int mat = pi[(int64_t)L::MAT * cap + p];
-      float fx[3] = {0.0f, 0.0f, 0.0f};
-      int l[3] = {0, 0, 0};
-#pragma unroll
-      for (int a = 0; a < D; a++) {
-        int base;
-        base_fx(pf[(int64_t)(L::X + a) * cap + p], P.inv_dx, base, fx[a]);
-        l[a] = min(max(base - b[a], 0), kBlock - 1);
-      }
-      cell = D == 3 ? l[0] + 4 * (l[1] + 4 * l[2]) : l[0] + 4 * l[1];
-      // a stationary particle keeps x, v, C in g2p: no grid adjoint
-      const float on = mat == kStationary ? 0.0f : 1.0f;
-      float Ad[3][3] = {{0.0f}}, q[3] = {0.0f, 0.0f, 0.0f};
-#pragma unroll
-      for (int a = 0; a < D; a++) {
-        float s = on * (gf[(int64_t)(L::V + a) * cap + p] + P.dt * gf[(int64_t)(L::X + a) * cap + p]);
-#pragma unroll
-        for (int e = 0; e < D; e++) {
-          Ad[a][e] = on * P.c4 * gf[(int64_t)(L::C + a * D + e) * cap + p];
-          s -= Ad[a][e] * fx[e];
+      // ---- phase 1: constitutive update and staging ----
+      const int t = threadIdx.x;
+      const bool valid = t < nc;
+      int cell = 0;
+      if (MODE == 1 && valid) {
+        const int p = perm[b0 + t];
+        const int mat = pi[(int64_t)L::MAT * cap + p];
+        float fx[3] = {0.0f, 0.0f, 0.0f};
+        int l[3] = {0, 0, 0};
+  #pragma unroll
+        for (int a = 0; a < D; a++) {
+          int base;
+          base_fx(pf[(int64_t)(L::X + a) * cap + p], P.inv_dx, base, fx[a]);
+          l[a] = min(max(base - b[a], 0), kBlock - 1);
         }
-        q[a] = s;
-      }
-      if (D == 3) {
-        stage[0][t] = make_float4(Ad[0][0], Ad[0][1], Ad[0][2], Ad[1][0]);
-        stage[1][t] = make_float4(Ad[1][1], Ad[1][2], Ad[2][0], Ad[2][1]);
-        stage[2][t] = make_float4(Ad[2][2], q[0], q[1], q[2]);
-        stage[3][t] = make_float4(fx[0], fx[1], fx[2], 0.0f);
-      } else {
-        stage[0][t] = make_float4(Ad[0][0], Ad[0][1], Ad[1][0], Ad[1][1]);
-        stage[1][t] = make_float4(q[0], q[1], fx[0], fx[1]);
-        stage[2][t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      }
-    } else if (MODE == 0 && valid) {
-      using PS = Particles<D, Q>;
-      const int p = perm[ck.y + t];
-      const int mat = pi[(int64_t)L::MAT * cap + p];
-      float xp[3], vp[3], C[3][3], F[3][3], stress[3][3];
-      PS::load_x(pf, cap, p, xp);
-      PS::load_v(pf, cap, p, vp);
-      PS::load_C(pf, cap, p, C);
-      PS::load_F(pf, cap, p, F);
-      float Jp = P.plastic ? PS::load_Jp(pf, cap, p) : 1.0f;
-      float fx[3] = {0.0f, 0.0f, 0.0f};
-      int l[3] = {0, 0, 0};
-#pragma unroll
-      for (int a = 0; a < D; a++) {
-        int base;
-        base_fx(xp[a], P.inv_dx, base, fx[a]);
-        // the binning pass put this particle in this block from the same x
-        l[a] = min(max(base - b[a], 0), kBlock - 1);
-      }
-      cell = D == 3 ? l[0] + 4 * (l[1] + 4 * l[2]) : l[0] + 4 * l[1];
-      float mu_0 = P.mu_0, lambda_0 = P.lambda_0;
-      if (P.youngs) {
-        const float E = P.youngs[p];
-        mu_0 = E * P.kmu;
-        lambda_0 = E * P.kla;
-      }
-      constitutive<D>(mat, P, mu_0, lambda_0, F, C, Jp, stress);
-      PS::store_F(pf_out, cap, p, F);  // the stress already used F before rounding (mpm:567)
-      if (P.plastic) PS::store_Jp(pf_out, cap, p, Jp);
-      float mass = P.p_mass;  // mpm_solver.py:571-573
-      if (mat == kWater) mass *= P.water_density;
-      // contribution to node o: w_o * (mass v + A (o - fx) dx) = w_o * (q + Adx o)
-      // with A = stress_scale tau + mass C (mpm_solver.py:569-583)
-      float Ad[3][3] = {{0.0f}}, q[3] = {0.0f, 0.0f, 0.0f};
-#pragma unroll
-      for (int a = 0; a < D; a++) {
-#pragma unroll
-        for (int e = 0; e < D; e++) Ad[a][e] = (P.stress_scale * stress[a][e] + mass * C[a][e]) * P.dx;
-        float s = mass * vp[a];
-#pragma unroll
-        for (int e = 0; e < D; e++) s -= Ad[a][e] * fx[e];
-        q[a] = s;
-      }
-      if (D == 3) {
-        stage[0][t] = make_float4(Ad[0][0], Ad[0][1], Ad[0][2], Ad[1][0]);
-        stage[1][t] = make_float4(Ad[1][1], Ad[1][2], Ad[2][0], Ad[2][1]);
-        stage[2][t] = make_float4(Ad[2][2], q[0], q[1], q[2]);
-        stage[3][t] = make_float4(fx[0], fx[1], fx[2], mass);
-      } else {
-        stage[0][t] = make_float4(Ad[0][0], Ad[0][1], Ad[1][0], Ad[1][1]);
-        stage[1][t] = make_float4(q[0], q[1], fx[0], fx[1]);
-        stage[2][t] = make_float4(mass, 0.0f, 0.0f, 0.0f);
-      }
-    }
-    lds_barrier();
-
-    // ---- phase 2: stable counting sort of the chunk by cell ----
-    unsigned long long peers = __ballot(valid);
-#pragma unroll
-    for (int bit = 0; bit < (D == 3 ? 6 : 4); bit++) {
-      const bool on = (cell >> bit) & 1;
-      const unsigned long long bal = __ballot(on);
-      peers &= on ? bal : ~bal;
-    }
-    const int rank = (int)__popcll(peers & lt);
-    if (valid && rank == 0) wcnt[wave][cell] = (int)__popcll(peers);
-    lds_barrier();
-    if (threadIdx.x < 64) {  // one wave: per-cell totals and exclusive scan
-      int run = 0;
-      if (lane < CELLS) {
-#pragma unroll
-        for (int w = 0; w < WAVES; w++) {
-          const int cnt = wcnt[w][lane];
-          wcnt[w][lane] = run;
-          run += cnt;
+        cell = D == 3 ? l[0] + 4 * (l[1] + 4 * l[2]) : l[0] + 4 * l[1];
+        // a stationary particle keeps x, v, C in g2p: no grid adjoint
+        const float on = mat == kStationary ? 0.0f : 1.0f;
+        float Ad[3][3] = {{0.0f}}, q[3] = {0.0f, 0.0f, 0.0f};
+  #pragma unroll
+        for (int a = 0; a < D; a++) {
+          float s = on * (gf[(int64_t)(L::V + a) * cap + p] + P.dt * gf[(int64_t)(L::X + a) * cap + p]);
+  #pragma unroll
+          for (int e = 0; e < D; e++) {
+            Ad[a][e] = on * P.c4 * gf[(int64_t)(L::C + a * D + e) * cap + p];
+            s -= Ad[a][e] * fx[e];
+          }
+          q[a] = s;
+        }
+        if (D == 3) {
+          stage[0][t] = make_float4(Ad[0][0], Ad[0][1], Ad[0][2], Ad[1][0]);
+          stage[1][t] = make_float4(Ad[1][1], Ad[1][2], Ad[2][0], Ad[2][1]);
+          stage[2][t] = make_float4(Ad[2][2], q[0], q[1], q[2]);
+          stage[3][t] = make_float4(fx[0], fx[1], fx[2], 0.0f);
+        } else {
+          stage[0][t] = make_float4(Ad[0][0], Ad[0][1], Ad[1][0], Ad[1][1]);
+          stage[1][t] = make_float4(q[0], q[1], fx[0], fx[1]);
+          stage[2][t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+      } else if (MODE == 0 && valid) {
+        using PS = Particles<D, Q>;
+        const int p = perm[b0 + t];
+        const int mat = pi[(int64_t)L::MAT * cap + p];
+        float xp[3], vp[3], C[3][3], F[3][3], stress[3][3];
+        PS::load_x(pf, cap, p, xp);
+        PS::load_v(pf, cap, p, vp);
+        PS::load_C(pf, cap, p, C);
+        PS::load_F(pf, cap, p, F);
+        float Jp = P.plastic ? PS::load_Jp(pf, cap, p) : 1.0f;
+        float fx[3] = {0.0f, 0.0f, 0.0f};
+        int l[3] = {0, 0, 0};
+  #pragma unroll
+        for (int a = 0; a < D; a++) {
+          int base;
+          base_fx(xp[a], P.inv_dx, base, fx[a]);
+          // the binning pass put this particle in this block from the same x
+          l[a] = min(max(base - b[a], 0), kBlock - 1);
+        }
+        cell = D == 3 ? l[0] + 4 * (l[1] + 4 * l[2]) : l[0] + 4 * l[1];
+        float mu_0 = P.mu_0, lambda_0 = P.lambda_0;
+        if (P.youngs) {
+          const float E = P.youngs[p];
+          mu_0 = E * P.kmu;
+          lambda_0 = E * P.kla;
+        }
+        constitutive<D>(mat, P, mu_0, lambda_0, F, C, Jp, stress);
+        PS::store_F(pf_out, cap, p, F);  // the stress already used F before rounding (mpm:567)
+        if (P.plastic) PS::store_Jp(pf_out, cap, p, Jp);
+        float mass = P.p_mass;  // mpm_solver.py:571-573
+        if (mat == kWater) mass *= P.water_density;
+        // contribution to node o: w_o * (mass v + A (o - fx) dx) = w_o * (q + Adx o)
+        // with A = stress_scale tau + mass C (mpm_solver.py:569-583)
+        float Ad[3][3] = {{0.0f}}, q[3] = {0.0f, 0.0f, 0.0f};
+  #pragma unroll
+        for (int a = 0; a < D; a++) {
+  #pragma unroll
+          for (int e = 0; e < D; e++) Ad[a][e] = (P.stress_scale * stress[a][e] + mass * C[a][e]) * P.dx;
+          float s = mass * vp[a];
+  #pragma unroll
+          for (int e = 0; e < D; e++) s -= Ad[a][e] * fx[e];
+          q[a] = s;
+        }
+        if (D == 3) {
+          stage[0][t] = make_float4(Ad[0][0], Ad[0][1], Ad[0][2], Ad[1][0]);
+          stage[1][t] = make_float4(Ad[1][1], Ad[1][2], Ad[2][0], Ad[2][1]);
+          stage[2][t] = make_float4(Ad[2][2], q[0], q[1], q[2]);
+          stage[3][t] = make_float4(fx[0], fx[1], fx[2], mass);
+        } else {
+          stage[0][t] = make_float4(Ad[0][0], Ad[0][1], Ad[1][0], Ad[1][1]);
+          stage[1][t] = make_float4(q[0], q[1], fx[0], fx[1]);
+          stage[2][t] = make_float4(mass, 0.0f, 0.0f, 0.0f);
         }
       }
-      int inc = run;
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const int y = __shfl_up(inc, off);
-        if (lane >= off) inc += y;
-      }
-      if (lane < CELLS) {
-        ccount[lane] = run;
-        cstart[lane] = inc - run;
-      }
-    }
-    lds_barrier();
-    if (valid) order[cstart[cell] + wcnt[wave][cell] + rank] = (unsigned char)t;
-    lds_barrier();
+      lds_barrier();
 
-    // ---- phase 3: cell-major scatter into private tiles ----
-    {
-      const int my_cell = scatter_cell_of_lane<D>(lane);
-      const int sub = wave * GROUPS + lane / CELLS;
-      const int lc[3] = {my_cell & 3, (my_cell >> 2) & 3, my_cell >> 4};
-      const int cbeg = cstart[my_cell], ccnt = ccount[my_cell];
-      float4* tp = ptile[sub];
-      cell_scatter<D, SUBS>(sub, ccnt, cbeg, order, stage, lc, tp);
-    }
-    lds_barrier();
+      // ---- phase 2: stable counting sort of the chunk by cell ----
+      unsigned long long peers = __ballot(valid);
+  #pragma unroll
+      for (int bit = 0; bit < (D == 3 ? 6 : 4); bit++) {
+        const bool on = (cell >> bit) & 1;
+        const unsigned long long bal = __ballot(on);
+        peers &= on ? bal : ~bal;
+      }
+      const int rank = (int)__popcll(peers & lt);
+      if (valid && rank == 0) wcnt[wave][cell] = (int)__popcll(peers);
+      lds_barrier();
+      if (threadIdx.x < 64) {  // one wave: per-cell totals and exclusive scan
+        int run = 0;
+        if (lane < CELLS) {
+  #pragma unroll
+          for (int w = 0; w < WAVES; w++) {
+            const int cnt = wcnt[w][lane];
+            wcnt[w][lane] = run;
+            run += cnt;
+          }
+        }
+        int inc = run;
+  #pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+          const int y = __shfl_up(inc, off);
+          if (lane >= off) inc += y;
+        }
+        if (lane < CELLS) {
+          ccount[lane] = run;
+          cstart[lane] = inc - run;
+        }
+      }
+      lds_barrier();
+      if (valid) order[cstart[cell] + wcnt[wave][cell] + rank] = (unsigned char)t;
+      lds_barrier();
+
+      // ---- phase 3: cell-major scatter into private tiles ----
+      {
+        const int my_cell = scatter_cell_of_lane<D>(lane);
+        const int sub = wave * GROUPS + lane / CELLS;
+        const int lc[3] = {my_cell & 3, (my_cell >> 2) & 3, my_cell >> 4};
+        const int cbeg = cstart[my_cell], ccnt = ccount[my_cell];
+        float4* tp = ptile[sub];
+        cell_scatter<D, SUBS>(sub, ccnt, cbeg, order, stage, lc, tp);
+      }
+      lds_barrier();
+    }  // batches of the chunk
 
     // ---- phase 4: sum the private tiles, write the node tile ----
     float4* out = tiles + (int64_t)c * TN;
@@ -819,6 +823,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p(const Params* __restrict__ sce
   __shared__ float tile_x[TN], tile_y[TN], tile_z[TN];
   __shared__ float red[kThreads / 64];
   __shared__ int dwcnt[kThreads / 64][32];
+  __shared__ int dbase[32];  // particles per direction in the chunk's earlier batches
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   float vmax = 0.0f;
@@ -832,7 +837,7 @@ __global__ __launch_bounds__(kThreads) void k_g2p(const Params* __restrict__ sce
     // a skipped scene keeps its particles (and so their blocks) unchanged; its
     // grid was not rebuilt and is not read
     const bool skip = P.skip != 0;
-    for (int i = threadIdx.x; i < (kThreads / 64) * 32; i += kThreads) (&dwcnt[0][0])[i] = 0;
+    if (threadIdx.x < 32) dbase[threadIdx.x] = 0;
     for (int i = skip ? TN : threadIdx.x; i < TN; i += kThreads) {
       const int tl[3] = {i % kTile, (i / kTile) % kTile, i / (kTile * kTile)};
       int q[3], ln[3];
@@ -853,165 +858,176 @@ __global__ __launch_bounds__(kThreads) void k_g2p(const Params* __restrict__ sce
     int b[3];
 #pragma unroll
     for (int a = 0; a < D; a++) b[a] = (lbl[a] + P.blo[a]) * kBlock;
-    // one particle per thread (chunks hold at most kThreads particles)
-    const int j = ck.y + threadIdx.x;
-    const bool live = j < ck.z;
-    int dir = 13;  // block offset of the particle after advection, (dz+1)*9 + (dy+1)*3 + (dx+1)
-    int key_new = ck.x;
-    if (live) {
-      using PS = Particles<D, Q>;
-      const int p = perm[j];
-      const int mat = pi[(int64_t)L::MAT * cap + p];
-      float xp[3] = {0.0f, 0.0f, 0.0f}, fx[3], w[3][3];
-      int l[3];
-      bool inside = true;
-      PS::load_x(pf, cap, p, xp);
-#pragma unroll
-      for (int a = 0; a < D; a++) {
-        int base;
-        base_fx(xp[a], P.inv_dx, base, fx[a]);
-        bspline(fx[a], w[a]);
-        l[a] = base - b[a];
-        inside &= (l[a] >= 0) & (l[a] < kBlock);
-      }
-      // v = sum_o w_o g_o and C = 4 inv_dx sum_o w_o g_o (o - fx)^T
-      // (mpm_solver.py:714-721), evaluated through the tensor-product
-      // weights w_o = wx[ox] wy[oy] wz[oz]: P_e = sum_o w_o g_o (o_e - fx_e)
-      // with the factor (o_e - fx_e) folded into axis e's weights (wd), so
-      // C = 4 inv_dx P needs no subtraction of two nearly equal sums.  One
-      // pass over the 27 nodes, per-axis partial sums instead of a 3x3 outer
-      // product per node.
-      float nv[3] = {0.0f, 0.0f, 0.0f}, pe[3][3] = {{0.0f}}, nC[3][3];
-      float wd[3][3];
-#pragma unroll
-      for (int a = 0; a < 3; a++)
-#pragma unroll
-        for (int k = 0; k < 3; k++) wd[a][k] = a < D ? ((float)k - fx[a]) * w[a][k] : 0.0f;
-      if (inside && !skip) {
-#pragma unroll
-        for (int oz = 0; oz < (D == 3 ? 3 : 1); oz++) {
-          float b0[3] = {0.0f, 0.0f, 0.0f}, bx[3] = {0.0f, 0.0f, 0.0f}, by[3] = {0.0f, 0.0f, 0.0f};
-#pragma unroll
-          for (int oy = 0; oy < 3; oy++) {
-            float a0[3] = {0.0f, 0.0f, 0.0f}, ax[3] = {0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int ox = 0; ox < 3; ox++) {
-              const int node = D == 3 ? (l[0] + ox) + kTile * ((l[1] + oy) + kTile * (l[2] + oz))
-                                      : (l[0] + ox) + kTile * (l[1] + oy);
-              const float gg[3] = {tile_x[node], tile_y[node], D == 3 ? tile_z[node] : 0.0f};
-#pragma unroll
+    // one particle per thread, kBatch particles of the chunk at a time
+    for (int b0 = ck.y; b0 < ck.z; b0 += kBatch) {
+      for (int i = threadIdx.x; i < (kThreads / 64) * 32; i += kThreads) (&dwcnt[0][0])[i] = 0;
+      lds_barrier();
+      const int j = b0 + threadIdx.x;
+      const bool live = j < ck.z;
+      int dir = 13;  // block offset of the particle after advection, (dz+1)*9 + (dy+1)*3 + (dx+1)
+      int key_new = ck.x;
+      if (live) {
+        using PS = Particles<D, Q>;
+        const int p = perm[j];
+        const int mat = pi[(int64_t)L::MAT * cap + p];
+        float xp[3] = {0.0f, 0.0f, 0.0f}, fx[3], w[3][3];
+        int l[3];
+        bool inside = true;
+        PS::load_x(pf, cap, p, xp);
+  #pragma unroll
+        for (int a = 0; a < D; a++) {
+          int base;
+          base_fx(xp[a], P.inv_dx, base, fx[a]);
+          bspline(fx[a], w[a]);
+          l[a] = base - b[a];
+          inside &= (l[a] >= 0) & (l[a] < kBlock);
+        }
+        // v = sum_o w_o g_o and C = 4 inv_dx sum_o w_o g_o (o - fx)^T
+        // (mpm_solver.py:714-721), evaluated through the tensor-product
+        // weights w_o = wx[ox] wy[oy] wz[oz]: P_e = sum_o w_o g_o (o_e - fx_e)
+        // with the factor (o_e - fx_e) folded into axis e's weights (wd), so
+        // C = 4 inv_dx P needs no subtraction of two nearly equal sums.  One
+        // pass over the 27 nodes, per-axis partial sums instead of a 3x3 outer
+        // product per node.
+        float nv[3] = {0.0f, 0.0f, 0.0f}, pe[3][3] = {{0.0f}}, nC[3][3];
+        float wd[3][3];
+  #pragma unroll
+        for (int a = 0; a < 3; a++)
+  #pragma unroll
+          for (int k = 0; k < 3; k++) wd[a][k] = a < D ? ((float)k - fx[a]) * w[a][k] : 0.0f;
+        if (inside && !skip) {
+  #pragma unroll
+          for (int oz = 0; oz < (D == 3 ? 3 : 1); oz++) {
+            float b0[3] = {0.0f, 0.0f, 0.0f}, bx[3] = {0.0f, 0.0f, 0.0f}, by[3] = {0.0f, 0.0f, 0.0f};
+  #pragma unroll
+            for (int oy = 0; oy < 3; oy++) {
+              float a0[3] = {0.0f, 0.0f, 0.0f}, ax[3] = {0.0f, 0.0f, 0.0f};
+  #pragma unroll
+              for (int ox = 0; ox < 3; ox++) {
+                const int node = D == 3 ? (l[0] + ox) + kTile * ((l[1] + oy) + kTile * (l[2] + oz))
+                                        : (l[0] + ox) + kTile * (l[1] + oy);
+                const float gg[3] = {tile_x[node], tile_y[node], D == 3 ? tile_z[node] : 0.0f};
+  #pragma unroll
+                for (int c = 0; c < D; c++) {
+                  const float g = gg[c];
+                  a0[c] += w[0][ox] * g;
+                  ax[c] += wd[0][ox] * g;
+                }
+              }
+  #pragma unroll
               for (int c = 0; c < D; c++) {
-                const float g = gg[c];
-                a0[c] += w[0][ox] * g;
-                ax[c] += wd[0][ox] * g;
+                b0[c] += w[1][oy] * a0[c];
+                bx[c] += w[1][oy] * ax[c];
+                by[c] += wd[1][oy] * a0[c];
               }
             }
-#pragma unroll
+            const float wz = D == 3 ? w[2][oz] : 1.0f;
+  #pragma unroll
             for (int c = 0; c < D; c++) {
-              b0[c] += w[1][oy] * a0[c];
-              bx[c] += w[1][oy] * ax[c];
-              by[c] += wd[1][oy] * a0[c];
+              nv[c] += wz * b0[c];
+              pe[c][0] += wz * bx[c];
+              pe[c][1] += wz * by[c];
+              if (D == 3) pe[c][2] += wd[2][oz] * b0[c];
             }
           }
-          const float wz = D == 3 ? w[2][oz] : 1.0f;
-#pragma unroll
-          for (int c = 0; c < D; c++) {
-            nv[c] += wz * b0[c];
-            pe[c][0] += wz * bx[c];
-            pe[c][1] += wz * by[c];
-            if (D == 3) pe[c][2] += wd[2][oz] * b0[c];
+        }
+  #pragma unroll
+        for (int c = 0; c < D; c++)
+  #pragma unroll
+          for (int e = 0; e < D; e++) nC[c][e] = P.c4 * pe[c][e];
+        float xn[3] = {xp[0], xp[1], xp[2]};
+        if (skip) {
+        } else if (mat != kStationary) {  // mpm_solver.py:722-724
+          PS::store_v(pf_out, cap, p, nv);  // x advances by the stored v
+  #pragma unroll
+          for (int a = 0; a < D; a++) {
+            xn[a] = xp[a] + P.dt * nv[a];
+            vmax = fmaxf(vmax, fabsf(nv[a]));
+          }
+          PS::store_x(pf_out, cap, p, xn);  // xn: the stored x (binned next)
+          PS::store_C(pf_out, cap, p, nC);
+        } else {
+          float vs[3];
+          PS::load_v(pf, cap, p, vs);
+  #pragma unroll
+          for (int a = 0; a < D; a++) {
+            const float va = vs[a];
+            vmax = fmaxf(vmax, fabsf(va));
+            if (!Q && pf_out != pf) {  // out of place: a stationary particle keeps x, v, C
+              pf_out[(int64_t)(L::V + a) * cap + p] = va;
+              pf_out[(int64_t)(L::X + a) * cap + p] = xp[a];
+  #pragma unroll
+              for (int e = 0; e < D; e++)
+                pf_out[(int64_t)(L::C + a * D + e) * cap + p] = pf[(int64_t)(L::C + a * D + e) * cap + p];
+            }
           }
         }
-      }
-#pragma unroll
-      for (int c = 0; c < D; c++)
-#pragma unroll
-        for (int e = 0; e < D; e++) nC[c][e] = P.c4 * pe[c][e];
-      float xn[3] = {xp[0], xp[1], xp[2]};
-      if (skip) {
-      } else if (mat != kStationary) {  // mpm_solver.py:722-724
-        PS::store_v(pf_out, cap, p, nv);  // x advances by the stored v
-#pragma unroll
+        // block of the new position for the next substep's binning
+        // (build_pid, mpm_solver.py:357-361); particles move less than one block
+        // per substep under any sane CFL number
+        int dd[3] = {0, 0, 0}, q[3] = {0, 0, 0};
+        bool near = true;
+  #pragma unroll
         for (int a = 0; a < D; a++) {
-          xn[a] = xp[a] + P.dt * nv[a];
-          vmax = fmaxf(vmax, fabsf(nv[a]));
+          int base;
+          float f;
+          base_fx(xn[a], P.inv_dx, base, f);
+          q[a] = (base >> 2) - P.blo[a];  // scene-local block
+          dd[a] = q[a] - lbl[a];
+          near &= (dd[a] >= -1) & (dd[a] <= 1) & (q[a] >= 0) & (q[a] < P.bnl[a] - 1);
         }
-        PS::store_x(pf_out, cap, p, xn);  // xn: the stored x (binned next)
-        PS::store_C(pf_out, cap, p, nC);
-      } else {
-        float vs[3];
-        PS::load_v(pf, cap, p, vs);
-#pragma unroll
-        for (int a = 0; a < D; a++) {
-          const float va = vs[a];
-          vmax = fmaxf(vmax, fabsf(va));
-          if (!Q && pf_out != pf) {  // out of place: a stationary particle keeps x, v, C
-            pf_out[(int64_t)(L::V + a) * cap + p] = va;
-            pf_out[(int64_t)(L::X + a) * cap + p] = xp[a];
-#pragma unroll
-            for (int e = 0; e < D; e++)
-              pf_out[(int64_t)(L::C + a * D + e) * cap + p] = pf[(int64_t)(L::C + a * D + e) * cap + p];
-          }
+        if (near) {
+          dir = (dd[2] + 1) * 9 + (dd[1] + 1) * 3 + (dd[0] + 1);
+          q[D - 1] += P.zoff;
+          key_new = encode_local<D>(q, P);
+        } else {
+          // left the window (bit 0) or moved more than one block (bit 3): the
+          // particle stays binned in its old block and the caller is told
+          atomicOr(err, (q[0] < 0 || q[0] >= P.bnl[0] - 1 || q[1] < 0 || q[1] >= P.bnl[1] - 1 ||
+                         (D == 3 && (q[2] < 0 || q[2] >= P.bnl[2] - 1)))
+                            ? 1
+                            : 8);
         }
       }
-      // block of the new position for the next substep's binning
-      // (build_pid, mpm_solver.py:357-361); particles move less than one block
-      // per substep under any sane CFL number
-      int dd[3] = {0, 0, 0}, q[3] = {0, 0, 0};
-      bool near = true;
-#pragma unroll
-      for (int a = 0; a < D; a++) {
-        int base;
-        float f;
-        base_fx(xn[a], P.inv_dx, base, f);
-        q[a] = (base >> 2) - P.blo[a];  // scene-local block
-        dd[a] = q[a] - lbl[a];
-        near &= (dd[a] >= -1) & (dd[a] <= 1) & (q[a] >= 0) & (q[a] < P.bnl[a] - 1);
+      // stable rank of the particle among the chunk's particles with the same
+      // direction; per-chunk direction counts; block counts of the next substep
+      unsigned long long peers = __ballot(live);
+  #pragma unroll
+      for (int bit = 0; bit < 5; bit++) {
+        const bool on = (dir >> bit) & 1;
+        const unsigned long long bal = __ballot(on);
+        peers &= on ? bal : ~bal;
       }
-      if (near) {
-        dir = (dd[2] + 1) * 9 + (dd[1] + 1) * 3 + (dd[0] + 1);
-        q[D - 1] += P.zoff;
-        key_new = encode_local<D>(q, P);
-      } else {
-        // left the window (bit 0) or moved more than one block (bit 3): the
-        // particle stays binned in its old block and the caller is told
-        atomicOr(err, (q[0] < 0 || q[0] >= P.bnl[0] - 1 || q[1] < 0 || q[1] >= P.bnl[1] - 1 ||
-                       (D == 3 && (q[2] < 0 || q[2] >= P.bnl[2] - 1)))
-                          ? 1
-                          : 8);
+      const int rank = (int)__popcll(peers & lt);
+      if (live && rank == 0) dwcnt[wave][dir] = (int)__popcll(peers);
+      lds_barrier();
+      if (threadIdx.x < 27) {
+        const int d = threadIdx.x;
+        int run = dbase[d];
+  #pragma unroll
+        for (int w = 0; w < kThreads / 64; w++) {
+          const int cnt = dwcnt[w][d];
+          dwcnt[w][d] = run;
+          run += cnt;
+        }
+        dbase[d] = run;
       }
-    }
-    // stable rank of the particle among the chunk's particles with the same
-    // direction; per-chunk direction counts; block counts of the next substep
-    unsigned long long peers = __ballot(live);
-#pragma unroll
-    for (int bit = 0; bit < 5; bit++) {
-      const bool on = (dir >> bit) & 1;
-      const unsigned long long bal = __ballot(on);
-      peers &= on ? bal : ~bal;
-    }
-    const int rank = (int)__popcll(peers & lt);
-    if (live && rank == 0) dwcnt[wave][dir] = (int)__popcll(peers);
-    lds_barrier();
+      lds_barrier();
+      if (live) {
+        code[j] = dir | ((dwcnt[wave][dir] + rank) << 8);
+        nkey[j] = key_new;
+      }
+      lds_barrier();
+    }  // batches of the chunk
+    // per-chunk direction counts; block counts of the next substep
     if (threadIdx.x < 27) {
       const int d = threadIdx.x;
-      int run = 0;
-#pragma unroll
-      for (int w = 0; w < kThreads / 64; w++) {
-        const int cnt = dwcnt[w][d];
-        dwcnt[w][d] = run;
-        run += cnt;
-      }
+      const int run = dbase[d];
       dcount[(int64_t)c * 27 + d] = run;
       if (run > 0) {
         int q[3] = {lb[0] + d % 3 - 1, lb[1] + (d / 3) % 3 - 1, lb[2] + d / 9 - 1};
         atomicAdd(&count_next[encode_local<D>(q, P)], run);
       }
-    }
-    lds_barrier();
-    if (live) {
-      code[j] = dir | ((dwcnt[wave][dir] + rank) << 8);
-      nkey[j] = key_new;
     }
     lds_barrier();
   }

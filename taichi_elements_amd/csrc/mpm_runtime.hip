@@ -173,7 +173,7 @@ struct mpm_solver {
   };
   std::vector<GraphEntry> graphs;
   int reorder_interval = 16;
-  int chunk = kChunkMax;
+  int chunk = kBatch;  // particles per transfer work item (autotuned up to kChunkMax)
   int persistent_groups = 2048;
   int debug = 0;
   bool timing = false;
@@ -219,6 +219,10 @@ struct mpm_solver {
   // +-grid_half cells and the block window follows the particles
   int unbounded = 0;
   int grid_half = 2048;
+
+  // chunk size the binning uses: the fused g2p2g transfer holds a whole
+  // chunk in one batch
+  int eff_chunk() const { return g2p2g ? std::min(chunk, kBatch) : chunk; }
 
   // 32-bit words per particle slot in the float buffers
   int nf() const {
@@ -404,7 +408,8 @@ int alloc_chunk_buffers(mpm_solver* s) {
   dfree(s->chunks);
   dfree(s->dcount);
   dfree(s->doff);
-  s->chunk_cap = std::min<int64_t>(s->total_blocks, s->cap) + (s->cap + s->chunk - 1) / s->chunk;
+  const int chunk = s->eff_chunk();
+  s->chunk_cap = std::min<int64_t>(s->total_blocks, s->cap) + (s->cap + chunk - 1) / chunk;
   int TN = s->D == 3 ? Geo<3>::kTileNodes : Geo<2>::kTileNodes;
   if (s->g2p2g) TN = s->D == 3 ? (kBlock + 4) * (kBlock + 4) * (kBlock + 4) : (kBlock + 4) * (kBlock + 4);
   TRY(dalloc(&s->tiles, s->chunk_cap * TN));
@@ -463,7 +468,7 @@ void fill_buffers(mpm_solver* s, SubstepBuffers& B) {
   B.step = s->d_step;
   B.total_blocks = s->total_blocks;
   B.key_bits = s->key_bits;
-  B.chunk = s->chunk;
+  B.chunk = s->eff_chunk();
   B.persistent_groups = s->persistent_groups;
   B.parity = (int)(s->substep_counter & 1);
 }
@@ -689,7 +694,7 @@ void carve_diff_workspace(const mpm_solver* s, Carve& c, SubstepBuffers& B) {
   const int64_t n = s->diff_n, tb = s->total_blocks;
   const int NPB = s->D == 3 ? Geo<3>::kNodesPerBlock : Geo<2>::kNodesPerBlock;
   const int TN = s->D == 3 ? Geo<3>::kTileNodes : Geo<2>::kTileNodes;
-  const int64_t chunk_cap = std::min<int64_t>(tb, n) + (n + s->chunk - 1) / s->chunk;
+  const int64_t chunk_cap = std::min<int64_t>(tb, n) + (n + s->eff_chunk() - 1) / s->eff_chunk();
   const int64_t nparts = (tb + kScanTile - 1) / kScanTile;
   const int64_t rtiles = (n + kRadixTile - 1) / kRadixTile;
   B.keys = c.take<int>(n);

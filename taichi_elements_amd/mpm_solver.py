@@ -123,6 +123,9 @@ def _autotune(obj, handle, dt, substeps, chunks, groups, verbose, dts):
             check(L.mpm_substeps(handle, k, float(dt), 0.0))
         check(L.mpm_sync(handle))
 
+    if chunks is None:
+        # the fused g2p2g transfer holds a chunk in one batch of 256
+        chunks = (64, 128, 256) if getattr(obj, "use_g2p2g", False) else (128, 256, 512, 1024)
     results = {}
     for ch in chunks:
         for gr in groups:
@@ -795,10 +798,11 @@ class MPMSolver:
         """Replay steady-state substep pairs as HIP graphs (default on)."""
         check(_lib.lib().mpm_set_graphs(self._h(), int(bool(enabled))))
 
-    def autotune(self, dt=None, substeps=4, chunks=(64, 128, 256), groups=(1024, 2048, 4096),
+    def autotune(self, dt=None, substeps=4, chunks=None, groups=(1024, 2048, 4096),
                  verbose=False):
         """Times every (chunk, groups) candidate on the current particles
-        and keeps the fastest.  The particle state (with use_g2p2g also the
+        and keeps the fastest (chunks: particles per transfer work item,
+        default 128-1024, or 64-256 with use_g2p2g).  The particle state (with use_g2p2g also the
         grid the next substep reads), t and the substep count are restored
         afterwards; only float32 summation order depends on the choice.
         Returns {(chunk, groups): ms per substep}."""

@@ -70,7 +70,7 @@ def test_set_tuning_checks_its_arguments():
     with pytest.raises(ValueError):
         s.set_tuning(chunk=8)
     with pytest.raises(ValueError):
-        s.set_tuning(chunk=1024)
+        s.set_tuning(chunk=2048)
     with pytest.raises(ValueError):
         s.set_tuning(groups=10)
     s.set_tuning(chunk=128, groups=1024)
