@@ -271,7 +271,7 @@ def fwd_bwd_bench(nsub=64, checkpoint_every=None, repeats=3, seed=4):
     checkpoint_every = 8
     t_fb8, (_, gv8, _, _) = timed(forward_backward)
     ds.check()
-    n = n_el + n_wa
+    n = x.shape[0]
     return dict(particles=n, substeps=nsub, checkpoint_every=chosen,
                 fwd_ms=1000 * t_f, fwd_bwd_ms=1000 * t_fb, ratio=t_fb / t_f,
                 fwd_value=n * nsub / t_f, fwd_bwd_value=n * nsub / t_fb,
