@@ -147,6 +147,37 @@ int mpm_read_max_velocity(mpm_solver* s, int32_t n, float* out);
  * substep and without particles.  One-scene solvers.  Synchronises. */
 int mpm_max_grid_velocity(mpm_solver* s, float* out);
 
+/* One substep in two halves, for hosts that run their own grid
+ * post-processes: the callables of grid_postprocess (mpm_solver.py:300,
+ * 794-795; the reference calls each as p(t, dt, grid_v) between
+ * grid_normalization_and_gravity and g2p).  One-scene solvers.
+ *
+ * mpm_substep_begin runs build_pid, p2g, grid_normalization_and_gravity and
+ * the built-in post-processes (bounding box, colliders) of one substep with
+ * time step dt (use_g2p2g: the whole fused substep, whose output grid is
+ * the next substep's input).  Until mpm_substep_end, the nodes of the
+ * active grid blocks (4^dim nodes each, as the reference's leaf blocks) can
+ * be read and written; every other call on the handle fails with
+ * MPM_EINVAL.  mpm_substep_end runs g2p and compute_max_velocity (read with
+ * mpm_read_max_velocity(s, 1, ...)).  Asynchronous. */
+int mpm_substep_begin(mpm_solver* s, float dt);
+/* Number of grid nodes of the active blocks of the substep in flight.
+ * Synchronises. */
+int mpm_grid_node_count(mpm_solver* s, int64_t* count);
+/* Copies the count nodes (count from mpm_grid_node_count) into DEVICE
+ * arrays of the handle's device: coords (count, dim) int32 global node
+ * indices I (node position I * dx, the reference's grid_v index), v
+ * (count, dim) float32 velocities, m (count,) float32 masses.  Waits for
+ * the copy before returning. */
+int mpm_grid_gather(mpm_solver* s, int32_t* coords, float* v, float* m, int64_t count);
+/* Writes the count velocities v (count, dim) float32 (a DEVICE array, same
+ * node order as mpm_grid_gather) back into the grid; masses are unchanged.
+ * The caller's own work on v must be complete.  Waits for the copy before
+ * returning. */
+int mpm_grid_scatter(mpm_solver* s, const float* v, int64_t count);
+/* Second half of the substep begun by mpm_substep_begin. */
+int mpm_substep_end(mpm_solver* s);
+
 /* particle_info (mpm_solver.py:1172-1191): position (n,dim), velocity
  * (n,dim), material (n,), color (n,) in insertion order.  Any pointer may
  * be NULL to skip that field.  Synchronises. */

@@ -49,6 +49,11 @@ SIGNATURES = [
     ("mpm_substeps", ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_float, ctypes.c_double]),
     ("mpm_read_max_velocity", ctypes.c_int, [_P, ctypes.c_int32, _FP]),
     ("mpm_max_grid_velocity", ctypes.c_int, [_P, _FP]),
+    ("mpm_substep_begin", ctypes.c_int, [_P, ctypes.c_float]),
+    ("mpm_grid_node_count", ctypes.c_int, [_P, _I64P]),
+    ("mpm_grid_gather", ctypes.c_int, [_P, _P, _P, _P, ctypes.c_int64]),
+    ("mpm_grid_scatter", ctypes.c_int, [_P, _P, ctypes.c_int64]),
+    ("mpm_substep_end", ctypes.c_int, [_P]),
     ("mpm_particle_info", ctypes.c_int, [_P, _FP, _FP, _I32P, _I32P]),
     ("mpm_get_state", ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_void_p]),
     ("mpm_set_state", ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_void_p]),

@@ -97,6 +97,14 @@ template <int D>
 void launch_substep(const Params* P, const int* slab, SubstepBuffers& B, int substep_index,
                     int raw_grid, hipStream_t st, Timer* timer);
 
+// One half of launch_substep for hosts that run their own grid
+// post-processes between the halves: half 0 = binning, p2g and the grid
+// operator (use_g2p2g: the whole fused substep), half 1 = g2p (use_g2p2g:
+// nothing).  B carries over from half 0 to half 1.
+template <int D>
+void launch_substep_half(const Params* P, const int* slab, SubstepBuffers& B, int substep_index,
+                         int half, hipStream_t st, Timer* timer);
+
 // p2g + grid operator on the binning already in B
 template <int D>
 void launch_transfer(const Params* P, const int* slab, SubstepBuffers& B, int raw_grid,
@@ -160,6 +168,20 @@ void launch_remap_grid(const float4* old_grid, float4* new_grid, const WindowMap
 // of max_a |v_a|, as float bits; out is cleared first.
 void launch_grid_vmax(int D, const float4* grid, const int* active, const int* totals,
                       int64_t total_blocks, unsigned* out, hipStream_t st);
+
+// Nodes of the active blocks (count at totals[kTotActive], NPB nodes each,
+// node i = block active[i / NPB], local node i % NPB) <-> dense arrays:
+// gather writes the global node coordinates coords[count * D] (block window
+// origin blo, extent bn in blocks, the scene's offset zoff on the last
+// axis), velocities v[count * D] and masses m[count]; scatter writes v back
+// into the grid's velocity components (masses untouched).
+struct NodeWindow {
+  int blo[3], bn[3], zoff;
+};
+void launch_grid_gather(int D, const float4* grid, const int* active, int64_t count,
+                        const NodeWindow& w, int* coords, float* v, float* m, hipStream_t st);
+void launch_grid_scatter(int D, float4* grid, const int* active, int64_t count, const float* v,
+                         hipStream_t st);
 
 // Up to 8 int array copies in one launch.
 struct CopySegments {

@@ -1475,6 +1475,19 @@ void launch_substep(const Params* P, const int* slab, SubstepBuffers& B, int s_i
 }
 
 template <int D>
+void launch_substep_half(const Params* P, const int* slab, SubstepBuffers& B, int s_index,
+                         int half, hipStream_t st, Timer* tm) {
+  if (B.g2p2g) {
+    if (half == 0) launch_g2p2g<D>(P, slab, B, s_index, st, tm);
+    return;
+  }
+  if (half == 0)
+    launch_front<D>(P, slab, B, 0, st, tm);
+  else
+    launch_g2p<D>(P, slab, B, s_index, st, tm);
+}
+
+template <int D>
 void launch_adjoint(const Params* P, const int* slab, SubstepBuffers& B, const float* pf_in,
                     const float* gout, float* gin, float* gE, float4* gbar, float* abuf,
                     float* nu_part, float* gnu, hipStream_t st, Timer* tm) {
@@ -1750,6 +1763,68 @@ void launch_grid_vmax(int D, const float4* grid, const int* active, const int* t
     k_grid_vmax<2><<<groups, kThreads, 0, st>>>(grid, active, totals, out);
 }
 
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_grid_gather(const float4* __restrict__ grid,
+                                                         const int* __restrict__ active,
+                                                         int64_t count, NodeWindow w,
+                                                         int* __restrict__ coords,
+                                                         float* __restrict__ v,
+                                                         float* __restrict__ m) {
+  constexpr int NPB = Geo<D>::kNodesPerBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * kThreads) {
+    const int bid = active[i / NPB], l = (int)(i % NPB);
+    const float4 g = grid[(int64_t)bid * NPB + l];
+    const int lb0 = bid % w.bn[0];
+    const int lb1 = D == 3 ? (bid / w.bn[0]) % w.bn[1] : bid / w.bn[0] - w.zoff;
+    coords[i * D + 0] = 4 * (w.blo[0] + lb0) + (l & 3);
+    coords[i * D + 1] = 4 * (w.blo[1] + lb1) + ((l >> 2) & 3);
+    v[i * D + 0] = g.x;
+    v[i * D + 1] = g.y;
+    if (D == 3) {
+      const int lb2 = bid / (w.bn[0] * w.bn[1]) - w.zoff;
+      coords[i * D + 2] = 4 * (w.blo[2] + lb2) + (l >> 4);
+      v[i * D + 2] = g.z;
+    }
+    m[i] = g.w;
+  }
+}
+
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_grid_scatter(float4* __restrict__ grid,
+                                                          const int* __restrict__ active,
+                                                          int64_t count,
+                                                          const float* __restrict__ v) {
+  constexpr int NPB = Geo<D>::kNodesPerBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * kThreads) {
+    float4* g = &grid[(int64_t)active[i / NPB] * NPB + i % NPB];
+    g->x = v[i * D + 0];
+    g->y = v[i * D + 1];
+    if (D == 3) g->z = v[i * D + 2];
+  }
+}
+
+void launch_grid_gather(int D, const float4* grid, const int* active, int64_t count,
+                        const NodeWindow& w, int* coords, float* v, float* m, hipStream_t st) {
+  if (count <= 0) return;
+  const int groups = clamp_groups(div_up(count, kThreads), 4096);
+  if (D == 3)
+    k_grid_gather<3><<<groups, kThreads, 0, st>>>(grid, active, count, w, coords, v, m);
+  else
+    k_grid_gather<2><<<groups, kThreads, 0, st>>>(grid, active, count, w, coords, v, m);
+}
+
+void launch_grid_scatter(int D, float4* grid, const int* active, int64_t count, const float* v,
+                         hipStream_t st) {
+  if (count <= 0) return;
+  const int groups = clamp_groups(div_up(count, kThreads), 4096);
+  if (D == 3)
+    k_grid_scatter<3><<<groups, kThreads, 0, st>>>(grid, active, count, v);
+  else
+    k_grid_scatter<2><<<groups, kThreads, 0, st>>>(grid, active, count, v);
+}
+
 template void launch_transfer<2>(const Params*, const int*, SubstepBuffers&, int, hipStream_t,
                                  Timer*);
 template void launch_transfer<3>(const Params*, const int*, SubstepBuffers&, int, hipStream_t,
@@ -1768,6 +1843,10 @@ template void launch_substep<2>(const Params*, const int*, SubstepBuffers&, int,
                                 Timer*);
 template void launch_substep<3>(const Params*, const int*, SubstepBuffers&, int, int, hipStream_t,
                                 Timer*);
+template void launch_substep_half<2>(const Params*, const int*, SubstepBuffers&, int, int,
+                                     hipStream_t, Timer*);
+template void launch_substep_half<3>(const Params*, const int*, SubstepBuffers&, int, int,
+                                     hipStream_t, Timer*);
 template void launch_reorder<2>(SubstepBuffers&, float*, int*, hipStream_t, Timer*);
 template void launch_reorder<3>(SubstepBuffers&, float*, int*, hipStream_t, Timer*);
 template void launch_append<2>(SubstepBuffers&, const float*, const float*, const int*, int, int,

@@ -160,6 +160,10 @@ struct mpm_solver {
   bool have_perm = false;  // perm of the previous substep matches the slots
   int since_reorder = 1 << 30;  // reorder at the first opportunity
   int64_t substep_counter = 0;
+  // mpm_substep_begin ran and mpm_substep_end has not: the buffers of the
+  // substep in flight (binning, grid) between the two halves
+  bool half_pending = false;
+  SubstepBuffers half_B{};
   int* d_step = nullptr;  // device substep counter (max-velocity slot)
   unsigned* d_gvmax = nullptr;  // compute_max_grid_velocity result
   float* d_aabb = nullptr;      // unbounded: per-workgroup particle boxes (kAabbGroups x 6)
@@ -256,6 +260,13 @@ void dfree(T*& p) {
     int r_ = (expr);        \
     if (r_ != MPM_OK) return r_; \
   } while (0)
+
+int no_half_pending(const mpm_solver* s) {
+  if (s->half_pending)
+    return fail(MPM_EINVAL, "a substep begun by mpm_substep_begin has not been ended "
+                            "(mpm_substep_end)");
+  return MPM_OK;
+}
 
 int check_device_errors(mpm_solver* s) {
   HIP_TRY(hipSetDevice(s->device));
@@ -486,32 +497,41 @@ void take_buffers(mpm_solver* s, const SubstepBuffers& B) {
   s->count_next = B.count_next;
 }
 
+// The physical reorder of the particle slots when one is due, then B for
+// the next substep.
+void begin_buffers(mpm_solver* s, SubstepBuffers& B, Timer* tm) {
+  fill_buffers(s, B);
+  if (s->have_perm && s->reorder_interval > 0 && s->since_reorder >= s->reorder_interval) {
+    if (s->D == 3)
+      launch_reorder<3>(B, s->pf_alt, s->pi_alt, s->stream, tm);
+    else
+      launch_reorder<2>(B, s->pf_alt, s->pi_alt, s->stream, tm);
+    std::swap(s->pf, s->pf_alt);
+    std::swap(s->pi, s->pi_alt);
+    s->since_reorder = 0;
+    s->reordered = true;
+    fill_buffers(s, B);
+  }
+}
+
+// Host bookkeeping after a substep's kernels were launched.
+void end_buffers(mpm_solver* s, const SubstepBuffers& B) {
+  take_buffers(s, B);
+  s->substep_counter++;
+  s->have_perm = true;
+  s->reordered = false;
+  s->since_reorder++;
+}
+
 // One substep launched kernel by kernel (also what a graph captures).
 int eager_substep(mpm_solver* s, int k, Timer* tm) {
-  {
-    SubstepBuffers B;
-    fill_buffers(s, B);
-    if (s->have_perm && s->reorder_interval > 0 && s->since_reorder >= s->reorder_interval) {
-      if (s->D == 3)
-        launch_reorder<3>(B, s->pf_alt, s->pi_alt, s->stream, tm);
-      else
-        launch_reorder<2>(B, s->pf_alt, s->pi_alt, s->stream, tm);
-      std::swap(s->pf, s->pf_alt);
-      std::swap(s->pi, s->pi_alt);
-      s->since_reorder = 0;
-      s->reordered = true;
-      fill_buffers(s, B);
-    }
-    if (s->D == 3)
-      launch_substep<3>(s->d_params, s->d_slab, B, k, s->debug & 1, s->stream, tm);
-    else
-      launch_substep<2>(s->d_params, s->d_slab, B, k, s->debug & 1, s->stream, tm);
-    take_buffers(s, B);
-    s->substep_counter++;
-    s->have_perm = true;
-    s->reordered = false;
-    s->since_reorder++;
-  }
+  SubstepBuffers B;
+  begin_buffers(s, B, tm);
+  if (s->D == 3)
+    launch_substep<3>(s->d_params, s->d_slab, B, k, s->debug & 1, s->stream, tm);
+  else
+    launch_substep<2>(s->d_params, s->d_slab, B, k, s->debug & 1, s->stream, tm);
+  end_buffers(s, B);
   return MPM_OK;
 }
 
@@ -984,6 +1004,7 @@ int mpm_create_batch(const mpm_config* cfgs, int32_t nscenes, mpm_solver** out) 
 
 int mpm_set_active_scene(mpm_solver* s, int32_t scene) {
   if (!s) return fail(MPM_EINVAL, "null argument");
+  TRY(no_half_pending(s));
   if (scene < 0 || scene >= (int)s->scenes.size()) return fail(MPM_EINVAL, "no such scene");
   s->active_scene = scene;
   return MPM_OK;
@@ -1032,12 +1053,14 @@ int mpm_destroy(mpm_solver* s) {
 
 int mpm_set_gravity(mpm_solver* s, const double* g) {
   if (!s || !g) return fail(MPM_EINVAL, "null argument");
+  TRY(no_half_pending(s));
   for (int a = 0; a < s->D; a++) s->scenes[s->active_scene].gravity[a] = g[a];
   return MPM_OK;
 }
 
 int mpm_add_bounding_box(mpm_solver* s, int32_t unbounded) {
   if (!s) return fail(MPM_EINVAL, "null argument");
+  TRY(no_half_pending(s));
   SceneHost& sc = s->scenes[s->active_scene];
   std::vector<PostOp>& post = sc.post;
   if (post.size() >= (size_t)kMaxPost) return fail(MPM_EINVAL, "too many grid post-processes");
@@ -1055,6 +1078,7 @@ int mpm_add_bounding_box(mpm_solver* s, int32_t unbounded) {
 int mpm_add_surface_collider(mpm_solver* s, const double* point, const double* normal,
                              int32_t surface, double friction) {
   if (!s || !point || !normal) return fail(MPM_EINVAL, "null argument");
+  TRY(no_half_pending(s));
   if (surface < 0 || surface > 2) return fail(MPM_EINVAL, "unknown surface type");
   if (surface == MPM_STICKY && friction != 0)
     return fail(MPM_EINVAL, "friction must be 0 on sticky surfaces.");
@@ -1078,6 +1102,7 @@ int mpm_add_surface_collider(mpm_solver* s, const double* point, const double* n
 
 int mpm_add_sphere_collider(mpm_solver* s, const double* center, double radius, int32_t surface) {
   if (!s || !center) return fail(MPM_EINVAL, "null argument");
+  TRY(no_half_pending(s));
   if (surface < 0 || surface > 2) return fail(MPM_EINVAL, "unknown surface type");
   std::vector<PostOp>& post = s->scenes[s->active_scene].post;
   if (post.size() >= (size_t)kMaxPost) return fail(MPM_EINVAL, "too many grid post-processes");
@@ -1092,6 +1117,7 @@ int mpm_add_sphere_collider(mpm_solver* s, const double* center, double radius, 
 
 int mpm_clear_grid_postprocess(mpm_solver* s) {
   if (!s) return fail(MPM_EINVAL, "null argument");
+  TRY(no_half_pending(s));
   s->scenes[s->active_scene].post.clear();
   return MPM_OK;
 }
@@ -1099,6 +1125,7 @@ int mpm_clear_grid_postprocess(mpm_solver* s) {
 int mpm_add_particles(mpm_solver* s, int64_t k, const float* x, const float* v,
                       const int32_t* material, const int32_t* color) {
   if (!s) return fail(MPM_EINVAL, "null argument");
+  TRY(no_half_pending(s));
   if (k < 0) return fail(MPM_EINVAL, "negative particle count");
   if (k == 0) return MPM_OK;
   if (!x || !material) return fail(MPM_EINVAL, "x and material are required");
@@ -1141,10 +1168,9 @@ int mpm_num_particles(mpm_solver* s, int64_t* n) {
   return MPM_OK;
 }
 
-// dts[k] is scene k's time step; with batch set, a zero time step means the
-// scene sits the substeps out.
-static int substeps_impl(mpm_solver* s, int32_t nsub, const float* dts, bool batch) {
-  if (nsub < 0) return fail(MPM_EINVAL, "negative substep count");
+// Everything before the first substep of a run of nsub: the max-velocity
+// slots, the scene table upload and the device substep counter.
+static int prepare_substeps(mpm_solver* s, int32_t nsub, const float* dts, bool batch) {
   for (size_t k = 0; k < s->scenes.size(); k++)
     if (batch && !(std::isfinite(dts[k]) && dts[k] >= 0.0f))
       return fail(MPM_EINVAL, "time steps of a batch must be finite and >= 0");
@@ -1165,8 +1191,18 @@ static int substeps_impl(mpm_solver* s, int32_t nsub, const float* dts, bool bat
     fill_params(s->h_params[k], s, s->scenes[k], dts[k], batch && dts[k] == 0.0f);
   HIP_TRY(hipMemcpyAsync(s->d_params, s->h_params, sizeof(Params) * s->scenes.size(),
                          hipMemcpyHostToDevice, s->stream));
-  Timer* tm = s->timing ? &s->timer : nullptr;
   launch_fill_u32((unsigned*)s->d_step, 0xFFFFFFFFu, 1, s->stream);  // slot k for substep k
+  return MPM_OK;
+}
+
+// dts[k] is scene k's time step; with batch set, a zero time step means the
+// scene sits the substeps out.
+static int substeps_impl(mpm_solver* s, int32_t nsub, const float* dts, bool batch) {
+  if (nsub < 0) return fail(MPM_EINVAL, "negative substep count");
+  TRY(no_half_pending(s));
+  TRY(prepare_substeps(s, nsub, dts, batch));
+  if (s->n == 0) return MPM_OK;
+  Timer* tm = s->timing ? &s->timer : nullptr;
   // unbounded: the window is checked against the particles every kSegment substeps
   constexpr int kSegment = 8;
   int next_check = 0;
@@ -1199,6 +1235,95 @@ int mpm_substeps(mpm_solver* s, int32_t nsub, float dt, double t0) {
 int mpm_substeps_batch(mpm_solver* s, int32_t nsub, const float* dts) {
   if (!s || !dts) return fail(MPM_EINVAL, "null argument");
   return substeps_impl(s, nsub, dts, true);
+}
+
+int mpm_substep_begin(mpm_solver* s, float dt) {
+  if (!s) return fail(MPM_EINVAL, "null argument");
+  if (s->scenes.size() != 1) return fail(MPM_EINVAL, "mpm_substep_begin: single-scene solvers only");
+  TRY(no_half_pending(s));
+  TRY(prepare_substeps(s, 1, &dt, false));
+  if (s->n > 0) {
+    if (s->unbounded) TRY(follow_particles(s, 1));
+    Timer* tm = s->timing ? &s->timer : nullptr;
+    SubstepBuffers& B = s->half_B;
+    begin_buffers(s, B, tm);
+    if (s->D == 3)
+      launch_substep_half<3>(s->d_params, s->d_slab, B, 0, 0, s->stream, tm);
+    else
+      launch_substep_half<2>(s->d_params, s->d_slab, B, 0, 0, s->stream, tm);
+    HIP_TRY(hipGetLastError());
+  }
+  s->half_pending = true;
+  return MPM_OK;
+}
+
+int mpm_grid_node_count(mpm_solver* s, int64_t* count) {
+  if (!s || !count) return fail(MPM_EINVAL, "null argument");
+  if (!s->half_pending) return fail(MPM_EINVAL, "no substep begun (mpm_substep_begin)");
+  *count = 0;
+  if (s->n == 0) return check_device_errors(s);
+  int nact = 0;
+  HIP_TRY(hipMemcpyAsync(&nact, s->half_B.totals + kTotActive, sizeof(int),
+                         hipMemcpyDeviceToHost, s->stream));
+  TRY(check_device_errors(s));
+  *count = (int64_t)nact * (s->D == 3 ? Geo<3>::kNodesPerBlock : Geo<2>::kNodesPerBlock);
+  return MPM_OK;
+}
+
+static int half_nodes(mpm_solver* s, int64_t count) {
+  if (!s->half_pending) return fail(MPM_EINVAL, "no substep begun (mpm_substep_begin)");
+  int64_t have = 0;
+  TRY(mpm_grid_node_count(s, &have));
+  if (count != have)
+    return fail(MPM_EINVAL, "node count " + std::to_string(count) + " differs from the " +
+                                std::to_string(have) + " nodes of the active blocks");
+  return MPM_OK;
+}
+
+int mpm_grid_gather(mpm_solver* s, int32_t* coords, float* v, float* m, int64_t count) {
+  if (!s) return fail(MPM_EINVAL, "null argument");
+  TRY(half_nodes(s, count));
+  if (count == 0) return MPM_OK;
+  if (!coords || !v || !m) return fail(MPM_EINVAL, "null argument");
+  const SceneHost& sc = s->scenes[0];
+  NodeWindow w{};
+  for (int a = 0; a < 3; a++) {
+    w.blo[a] = sc.blo[a];
+    w.bn[a] = s->bn[a];
+  }
+  w.zoff = sc.zoff;
+  launch_grid_gather(s->D, s->half_B.grid, s->half_B.active, count, w, coords, v, m, s->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return MPM_OK;
+}
+
+int mpm_grid_scatter(mpm_solver* s, const float* v, int64_t count) {
+  if (!s) return fail(MPM_EINVAL, "null argument");
+  TRY(half_nodes(s, count));
+  if (count == 0) return MPM_OK;
+  if (!v) return fail(MPM_EINVAL, "null argument");
+  launch_grid_scatter(s->D, s->half_B.grid, s->half_B.active, count, v, s->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return MPM_OK;
+}
+
+int mpm_substep_end(mpm_solver* s) {
+  if (!s) return fail(MPM_EINVAL, "null argument");
+  if (!s->half_pending) return fail(MPM_EINVAL, "no substep begun (mpm_substep_begin)");
+  s->half_pending = false;
+  if (s->n == 0) return MPM_OK;
+  HIP_TRY(hipSetDevice(s->device));
+  Timer* tm = s->timing ? &s->timer : nullptr;
+  SubstepBuffers& B = s->half_B;
+  if (s->D == 3)
+    launch_substep_half<3>(s->d_params, s->d_slab, B, 0, 1, s->stream, tm);
+  else
+    launch_substep_half<2>(s->d_params, s->d_slab, B, 0, 1, s->stream, tm);
+  end_buffers(s, B);
+  HIP_TRY(hipGetLastError());
+  return MPM_OK;
 }
 
 int mpm_read_max_velocity(mpm_solver* s, int32_t nsub, float* out) {
@@ -1240,6 +1365,7 @@ static int field_info(mpm_solver* s, int32_t field, int* comp0, int* ncomp, bool
 
 int mpm_get_state(mpm_solver* s, int32_t field, void* host) {
   if (!s || !host) return fail(MPM_EINVAL, "null argument");
+  TRY(no_half_pending(s));
   if (field == MPM_FIELD_COLOR) {
     std::memcpy(host, s->color.data(), sizeof(int32_t) * s->n);
     return MPM_OK;
@@ -1273,6 +1399,7 @@ int mpm_get_state(mpm_solver* s, int32_t field, void* host) {
 
 int mpm_set_state(mpm_solver* s, int32_t field, const void* host) {
   if (!s || !host) return fail(MPM_EINVAL, "null argument");
+  TRY(no_half_pending(s));
   if (field == MPM_FIELD_COLOR) {
     std::memcpy(s->color.data(), host, sizeof(int32_t) * s->n);
     return MPM_OK;
@@ -1320,6 +1447,7 @@ int mpm_set_state(mpm_solver* s, int32_t field, const void* host) {
 
 int mpm_particle_info(mpm_solver* s, float* x, float* v, int32_t* material, int32_t* color) {
   if (!s) return fail(MPM_EINVAL, "null argument");
+  TRY(no_half_pending(s));
   if (x) TRY(mpm_get_state(s, MPM_FIELD_X, x));
   if (v) TRY(mpm_get_state(s, MPM_FIELD_V, v));
   if (material) TRY(mpm_get_state(s, MPM_FIELD_MATERIAL, material));
@@ -1329,6 +1457,7 @@ int mpm_particle_info(mpm_solver* s, float* x, float* v, int32_t* material, int3
 
 int mpm_read_grid(mpm_solver* s, const int32_t* lo, const int32_t* nn, float* m, float* v) {
   if (!s || !lo || !nn) return fail(MPM_EINVAL, "null argument");
+  TRY(no_half_pending(s));
   TRY(check_device_errors(s));
   const int D = s->D;
   const int NPB = D == 3 ? 64 : 16;
@@ -1382,6 +1511,7 @@ int mpm_read_grid(mpm_solver* s, const int32_t* lo, const int32_t* nn, float* m,
 
 int mpm_read_binning(mpm_solver* s, int32_t* keys, int32_t* perm, int32_t* slot_to_insertion) {
   if (!s) return fail(MPM_EINVAL, "null argument");
+  TRY(no_half_pending(s));
   TRY(check_device_errors(s));
   if (!s->have_perm) return fail(MPM_EINVAL, "no substep has run since the last change");
   if (s->n == 0) return MPM_OK;
@@ -1489,6 +1619,7 @@ int mpm_last_counts(mpm_solver* s, int64_t* out4) {
 // ---------------------------------------------------------------------------
 int mpm_diff_setup(mpm_solver* s, int64_t n, const int32_t* material) {
   if (!s || (n > 0 && !material)) return fail(MPM_EINVAL, "null argument");
+  TRY(no_half_pending(s));
   if (s->n != 0) return fail(MPM_EINVAL, "a differentiable handle holds no particles of its own");
   if (s->scenes.size() != 1) return fail(MPM_EINVAL, "differentiable mode runs one scene");
   if (n <= 0 || n > (int64_t)INT32_MAX / 2) return fail(MPM_EINVAL, "bad particle count");
@@ -1602,6 +1733,7 @@ void diff_buffers(mpm_solver* s, SubstepBuffers& B, float* pf) {
 
 int mpm_set_g2p2g(mpm_solver* s, int32_t enabled, double allowed_cfl, int32_t clamp) {
   if (!s) return fail(MPM_EINVAL, "null argument");
+  TRY(no_half_pending(s));
   if (s->substep_counter != 0 || s->diff_n != 0)
     return fail(MPM_EINVAL, "choose the transfer mode before the first substep");
   if (!(allowed_cfl >= 0.0)) return fail(MPM_EINVAL, "g2p2g_allowed_cfl must be >= 0");
@@ -1629,6 +1761,7 @@ int mpm_set_g2p2g(mpm_solver* s, int32_t enabled, double allowed_cfl, int32_t cl
 
 int mpm_set_storage(mpm_solver* s, int32_t quant, int32_t support_plasticity) {
   if (!s) return fail(MPM_EINVAL, "null argument");
+  TRY(no_half_pending(s));
   if (s->diff_n != 0) return fail(MPM_EINVAL, "the differentiable mode keeps the float layout");
   const int q = quant ? 1 : 0, pl = support_plasticity ? 1 : 0;
   if (q == s->quant && pl == s->plastic) return MPM_OK;
@@ -1648,6 +1781,7 @@ int mpm_particle_bytes(mpm_solver* s, int32_t* bytes) {
 
 int mpm_max_grid_velocity(mpm_solver* s, float* out) {
   if (!s || !out) return fail(MPM_EINVAL, "null argument");
+  TRY(no_half_pending(s));
   if (s->scenes.size() != 1) return fail(MPM_EINVAL, "the grid of a batch holds many scenes");
   HIP_TRY(hipSetDevice(s->device));
   *out = 0.0f;
@@ -1663,6 +1797,7 @@ int mpm_max_grid_velocity(mpm_solver* s, float* out) {
 
 int mpm_set_unbounded(mpm_solver* s, int32_t enabled) {
   if (!s) return fail(MPM_EINVAL, "null argument");
+  TRY(no_half_pending(s));
   if (enabled && s->scenes.size() != 1)
     return fail(MPM_EINVAL, "unbounded scenes are supported for one-scene solvers");
   if (enabled && s->diff_n != 0)
@@ -1689,6 +1824,7 @@ int mpm_set_graphs(mpm_solver* s, int32_t enabled) {
 
 int mpm_set_tuning(mpm_solver* s, int32_t chunk, int32_t groups) {
   if (!s) return fail(MPM_EINVAL, "null argument");
+  TRY(no_half_pending(s));
   if (chunk != 0 && (chunk < 16 || chunk > kChunkMax))
     return fail(MPM_EINVAL, "chunk must lie in [16, " + std::to_string(kChunkMax) + "]");
   if (groups != 0 && (groups < 64 || groups > 16384))
@@ -1717,6 +1853,7 @@ int mpm_set_tuning(mpm_solver* s, int32_t chunk, int32_t groups) {
 
 int mpm_snapshot(mpm_solver* s, int32_t op) {
   if (!s) return fail(MPM_EINVAL, "null argument");
+  TRY(no_half_pending(s));
   if (s->diff_n != 0) return fail(MPM_EINVAL, "a differentiable handle keeps its state in caller arrays");
   HIP_TRY(hipSetDevice(s->device));
   HIP_TRY(hipStreamSynchronize(s->stream));
@@ -1816,6 +1953,7 @@ int mpm_diff_binning_bytes(mpm_solver* s, int64_t* bytes) {
 int mpm_diff_substep(mpm_solver* s, float dt, const float* state_in, float* state_out,
                      const float* youngs, double nu, void* binning, void* stream) {
   if (!s || !state_in || !state_out) return fail(MPM_EINVAL, "null argument");
+  TRY(no_half_pending(s));
   if (s->diff_n == 0) return fail(MPM_EINVAL, "mpm_diff_setup first");
   hipStream_t st = (hipStream_t)stream;  // NULL: the null stream
   HIP_TRY(hipSetDevice(s->device));
@@ -1854,6 +1992,7 @@ int mpm_diff_substep(mpm_solver* s, float dt, const float* state_in, float* stat
 int mpm_diff_substep_noalloc(mpm_solver* s, float dt, const float* state_in, float* state_out,
                              const float* youngs, double nu, void* stream) {
   if (!s || !state_in || !state_out) return fail(MPM_EINVAL, "null argument");
+  TRY(no_half_pending(s));
   if (s->diff_n == 0) return fail(MPM_EINVAL, "mpm_diff_setup first");
   Carve need{nullptr};
   SubstepBuffers B;
@@ -1900,6 +2039,7 @@ int mpm_diff_substep_backward(mpm_solver* s, float dt, const float* state_in,
                               float* grad_nu, void* stream) {
   if (!s || !state_in || !grad_out || !grad_in || !grad_nu)
     return fail(MPM_EINVAL, "null argument");
+  TRY(no_half_pending(s));
   if (s->diff_n == 0) return fail(MPM_EINVAL, "mpm_diff_setup first");
   hipStream_t st = (hipStream_t)stream;  // NULL: the null stream
   HIP_TRY(hipSetDevice(s->device));
@@ -1948,6 +2088,7 @@ int mpm_diff_sync(mpm_solver* s, void* stream) {
 
 int mpm_quantize_particles(mpm_solver* s, float* ranges, uint32_t* x_and_v, uint8_t* color) {
   if (!s || !ranges || !x_and_v || !color) return fail(MPM_EINVAL, "null argument");
+  TRY(no_half_pending(s));
   // np.min over no particles raises ValueError (particle_io.py:42)
   if (s->n == 0) return fail(MPM_EINVAL, "no particles to write");
   TRY(check_device_errors(s));
@@ -2052,6 +2193,7 @@ int mpm_add_mesh(mpm_solver* s, int64_t ntri, const double* triangles, int32_t m
                  int32_t color, int32_t sample_density, const float* velocity,
                  const float* translation, int32_t super_sample, uint32_t seed, int64_t* added) {
   if (!s) return fail(MPM_EINVAL, "null argument");
+  TRY(no_half_pending(s));
   if (added) *added = 0;
   if (s->D != 3) return fail(MPM_EINVAL, "add_mesh needs a 3D solver");
   if (material < 0 || material > MPM_STATIONARY)

@@ -25,7 +25,14 @@ Differences from the reference, all explicit:
     every substep, as the reference does (one host synchronisation per
     substep);
   * ``grid_postprocess`` lists descriptions of the built-in post-processes
-    (bounding box and colliders); arbitrary Taichi callables cannot be added.
+    (bounding box and colliders), which run inside the grid operator kernel,
+    and any callables the caller appends.  A callable is called as the
+    reference calls its entries, ``p(t, dt, grid_v)`` after the grid
+    operator and before g2p (mpm_solver.py:794-795), but ``grid_v`` is a
+    ``GridNodes`` of torch tensors on the device (node coordinates, writable
+    velocities, masses of the active blocks' nodes) instead of a Taichi
+    field; the built-ins run before every callable, wherever they sit in
+    the list.
 """
 import math
 import numbers
@@ -38,6 +45,42 @@ from ._lib import check
 
 # Print one '.' per substep as the reference's step() does (mpm_solver.py:760).
 PRINT_PROGRESS = True
+
+
+class GridNodes:
+    """The grid a callable of ``grid_postprocess`` receives as ``grid_v``.
+
+    The reference hands its post-processes the sparse grid velocity field and
+    they loop over its active nodes (``for I in ti.grouped(grid_v)``,
+    mpm_solver.py:618-685).  Here the same nodes -- every node of the grid
+    blocks the substep's particles touch, 4^dim nodes per block as the
+    reference's leaf blocks -- are torch tensors on the solver's device:
+
+      ``I``  (n, dim) int32 node indices (node position ``I * dx``),
+      ``v``  (n, dim) float32 velocities after the grid operator; the
+             callables change them in place (or assign a new (n, dim)
+             float32 tensor to ``v``), and g2p reads what they leave,
+      ``m``  (n,) float32 masses (read only: writes are not kept).
+
+    Nodes without mass have v = 0, as in the reference.
+    """
+
+    def __init__(self, I, v, m, dx):
+        self.I = I
+        self.v = v
+        self.m = m
+        self.dx = dx
+
+    def __len__(self):
+        return int(self.m.shape[0])
+
+    def positions(self):
+        """Node positions ``I * dx`` (float32 (n, dim))."""
+        return self.I.float() * self.dx
+
+
+def _tptr(t):
+    return _lib.ctypes.c_void_p(t.data_ptr())
 
 
 class _Field0D:
@@ -105,10 +148,12 @@ SNAPSHOT_SAVE, SNAPSHOT_RESTORE, SNAPSHOT_RELEASE = 0, 1, 2
 
 def _autotune(obj, handle, dt, substeps, chunks, groups, verbose, dts):
     """Shared by MPMSolver and MPMSolverBatch: time each launch shape on
-    the live particles (two untimed substeps re-bin after each change).
-    The scene is put back from a device snapshot (mpm_snapshot) before
-    every candidate and at the end: the particle slots and, with use_g2p2g,
-    the grid the next substep reads and which particles are new."""
+    the live particles.  Four untimed substeps after each change re-bin,
+    reorder the slots and capture the shape's substep-pair graph; the
+    shape's time is the best of three timed runs of ``substeps``.  The
+    scene is put back from a device snapshot (mpm_snapshot) before every
+    candidate and at the end: the particle slots and, with use_g2p2g, the
+    grid the next substep reads and which particles are new."""
     L = _lib.lib()
     if obj.n_particles_total() == 0:
         return {}
@@ -131,10 +176,14 @@ def _autotune(obj, handle, dt, substeps, chunks, groups, verbose, dts):
         for gr in groups:
             check(L.mpm_set_tuning(handle, int(ch), int(gr)))
             check(L.mpm_snapshot(handle, SNAPSHOT_RESTORE))
-            run(2)
-            t0 = time.perf_counter()
-            run(substeps)
-            results[(ch, gr)] = 1000 * (time.perf_counter() - t0) / substeps
+            run(4)
+            best = None
+            for _ in range(3):
+                t0 = time.perf_counter()
+                run(substeps)
+                el = time.perf_counter() - t0
+                best = el if best is None else min(best, el)
+            results[(ch, gr)] = 1000 * best / substeps
             if verbose:
                 print(f"chunk {ch:4d} groups {gr:5d}: {results[(ch, gr)]:.4f} ms/substep")
     best = min(results, key=results.get)
@@ -463,7 +512,11 @@ class MPMSolver:
         self._create_handle()
         check(_lib.lib().mpm_set_gravity(self._h(), _lib.dvec(self._gravity)))
         self.grid_postprocess = []
-        for kind, kw in post:
+        for entry in post:
+            if callable(entry):
+                self.grid_postprocess.append(entry)
+                continue
+            kind, kw = entry
             if kind == "bounding_box":
                 self.add_bounding_box(kw.get("unbounded", False))
             elif kind == "sphere":
@@ -650,10 +703,61 @@ class MPMSolver:
         return len(maxv), dt, maxv
 
     def _run_substeps(self, count, dt):
+        hooks = [p for p in self.grid_postprocess if callable(p)]
+        if hooks:
+            return self._run_substeps_with_callables(count, dt, hooks)
         L = _lib.lib()
         check(L.mpm_substeps(self._h(), int(count), float(dt), float(self.t)))
         out = np.zeros(max(count, 1), np.float32)
         check(L.mpm_read_max_velocity(self._h(), int(count), _lib.fptr(out)))
+        return out[:count]
+
+    def _run_substeps_with_callables(self, count, dt, hooks):
+        """count substeps, each split around the callables of
+        grid_postprocess: the first half (binning, p2g, grid operator with
+        the built-in post-processes) and the second (g2p) run in the HIP
+        kernels, the callables on the active nodes in between, with the
+        time t of the reference's step() loop (self.t before the substep,
+        advanced by dt per substep in float64)."""
+        import torch
+        L = _lib.lib()
+        h = self._h()
+        dev = torch.device("cuda", int(self._device))
+        stream = torch.cuda.current_stream(dev)
+        out = np.zeros(max(count, 1), np.float32)
+        t = self.t
+        for k in range(count):
+            check(L.mpm_substep_begin(h, float(dt)))
+            failed = True
+            try:
+                n = _lib.ctypes.c_int64()
+                check(L.mpm_grid_node_count(h, _lib.ctypes.byref(n)))
+                n = n.value
+                I = torch.empty((n, self.dim), dtype=torch.int32, device=dev)
+                v = torch.empty((n, self.dim), dtype=torch.float32, device=dev)
+                m = torch.empty((n,), dtype=torch.float32, device=dev)
+                stream.synchronize()  # the gather runs on the solver's own stream
+                check(L.mpm_grid_gather(h, _tptr(I), _tptr(v), _tptr(m), n))
+                nodes = GridNodes(I, v, m, self.dx)
+                for fn in hooks:
+                    fn(t, dt, nodes)
+                nv = nodes.v
+                if (not torch.is_tensor(nv) or nv.dtype != torch.float32
+                        or tuple(nv.shape) != (n, self.dim) or nv.device != dev):
+                    raise ValueError(f"grid_postprocess: grid_v.v must stay a float32 ({n}, "
+                                     f"{self.dim}) tensor on {dev}")
+                nv = nv.contiguous()
+                stream.synchronize()  # the callables' writes, before the scatter reads them
+                check(L.mpm_grid_scatter(h, _tptr(nv), n))
+                failed = False
+            finally:
+                rc = L.mpm_substep_end(h)
+            if not failed:
+                check(rc)
+            mv = np.zeros(1, np.float32)
+            check(L.mpm_read_max_velocity(h, 1, _lib.fptr(mv)))
+            out[k] = mv[0]
+            t += dt
         return out[:count]
 
     def substep(self, dt):
