@@ -18,7 +18,8 @@ singular-value clamp / Drucker-Prager return map and the hardening), on the
 forward's record of the substep (binning and raw grid sums).  Each
 substep is one ``torch.autograd.Function``; ``run`` checkpoints the state
 every ``checkpoint_every`` substeps (torch.utils.checkpoint) so only those
-states are stored and each segment is recomputed during the backward pass.
+states are stored and each segment but the last is recomputed during the
+backward pass.
 
 State layout: float32 tensor of shape (NF, n), NF = 2d + 2d^2 + 1, rows
 x[d], v[d], C[d*d] (row-major), F[d*d], Jp.
@@ -55,8 +56,8 @@ class _Substep(torch.autograd.Function):
         if not (state.is_contiguous() and youngs.is_contiguous()):
             raise ValueError("state and youngs must be contiguous tensors")
         out = torch.empty_like(state)
-        # the substep's binning, kept for the backward pass (which then only
-        # recomputes the grid)
+        # the substep's record (binning and grid sums), kept for the backward
+        # pass, which then runs only the adjoint kernels
         binning = torch.empty(solver._binning_bytes, dtype=torch.uint8, device=state.device)
         check(_lib.lib().mpm_diff_substep(solver._handle, float(dt), _ptr(state), _ptr(out),
                                           _ptr(youngs), float(nu_value), _ptr(binning),
@@ -188,8 +189,10 @@ class DiffMPMSolver:
 
     def run(self, state, youngs, nu, nsub, dt=None, checkpoint_every=None, nu_value=None):
         """``nsub`` substeps; with gradients enabled and checkpoint_every > 1
-        only every checkpoint_every-th state is kept and each segment is
-        recomputed in the backward pass.  checkpoint_every=None keeps every
+        only every checkpoint_every-th state is kept and each segment but the
+        last is recomputed in the backward pass (the last segment keeps its
+        substeps' inputs: the backward pass would recompute them first, so
+        the peak memory is the same).  checkpoint_every=None keeps every
         substep's input (no recomputation) when they fit in ``keep_bytes``,
         else checkpoints every 8 substeps.  The gradients are the same
         either way."""
@@ -211,7 +214,7 @@ class DiffMPMSolver:
         k = max(1, int(checkpoint_every))
         for t0 in range(0, nsub, k):
             m = min(k, nsub - t0)
-            if k > 1 and torch.is_grad_enabled():
+            if k > 1 and torch.is_grad_enabled() and t0 + m < nsub:
                 state = torch.utils.checkpoint.checkpoint(self._segment, state, youngs, nu,
                                                           nu_value, dt, m, use_reentrant=False)
             else:
