@@ -192,6 +192,43 @@ __device__ __forceinline__ unsigned lb_lookback(const unsigned long long* status
   return excl;
 }
 
+// The same prefix, looked back by a whole wave: lane j reads the status of
+// tile - 1 - j (64 predecessors per round trip instead of one); the nearest
+// inclusive prefix in the window ends the sum.  Every lane returns the sum.
+__device__ __forceinline__ unsigned lb_lookback_wave(const unsigned long long* status, int stride,
+                                                     int tile, int chan, unsigned epoch,
+                                                     int* err) {
+  const unsigned long long ep = lb_epoch_bits(epoch);
+  const int lane = threadIdx.x & 63;
+  unsigned excl = 0;
+  for (int top = tile - 1; top >= 0; top -= 64) {
+    const int t = top - lane;
+    unsigned long long w = 0;
+    if (t >= 0) {
+      int spins = 0;
+      for (;;) {
+        w = __hip_atomic_load(&status[(int64_t)t * stride + chan], __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_AGENT);
+        if ((w & ~0x3ffffffffull) == ep && (w & (kFlagAgg | kFlagPre))) break;
+        if (++spins > kSpinLimit) {
+          atomicOr(err, 2);
+          w = ep | kFlagPre;  // give up: count this tile as a zero prefix
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    const unsigned long long pre = __ballot(t >= 0 && (w & kFlagPre) != 0);
+    const int stop = pre ? __ffsll((unsigned long long)pre) - 1 : 63;
+    unsigned v = (t >= 0 && lane <= stop) ? (unsigned)w : 0u;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    excl += v;
+    if (pre) break;
+  }
+  return excl;
+}
+
 struct LBTicket {
   int tile;
   unsigned epoch;
@@ -277,22 +314,26 @@ __global__ __launch_bounds__(kThreads) void k_bscan(
         atomicAdd(&lh[ps][(total_blocks >> (8 * ps)) & 255], inval);
   }
 
-  if (threadIdx.x < 4 && tile_prefix) {
-    const int4 tp = tile_prefix[tile];
-    const int c = threadIdx.x;
-    excl_sh[c] = (unsigned)(c == 0 ? tp.x : c == 1 ? tp.y : c == 2 ? tp.z : tp.w);
-  } else if (threadIdx.x < 4) {
-    const int c = threadIdx.x;
+  static_assert(kThreads == 256, "one wave per scanned channel");
+  if (tile_prefix) {
+    if (threadIdx.x < 4) {
+      const int4 tp = tile_prefix[tile];
+      const int c = threadIdx.x;
+      excl_sh[c] = (unsigned)(c == 0 ? tp.x : c == 1 ? tp.y : c == 2 ? tp.z : tp.w);
+    }
+  } else {
+    // wave c looks back channel c
+    const int c = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const unsigned agg = (unsigned)(c == 0 ? tot.x : c == 1 ? tot.y : c == 2 ? tot.z : tot.w);
     unsigned excl = 0;
     if (tile == 0) {
-      lb_publish(&status[c], kFlagPre, tk.epoch, agg);
+      if (lane == 0) lb_publish(&status[c], kFlagPre, tk.epoch, agg);
     } else {
-      lb_publish(&status[(int64_t)tile * 4 + c], kFlagAgg, tk.epoch, agg);
-      excl = lb_lookback(status, 4, tile, c, tk.epoch, err);
-      lb_publish(&status[(int64_t)tile * 4 + c], kFlagPre, tk.epoch, excl + agg);
+      if (lane == 0) lb_publish(&status[(int64_t)tile * 4 + c], kFlagAgg, tk.epoch, agg);
+      excl = lb_lookback_wave(status, 4, tile, c, tk.epoch, err);
+      if (lane == 0) lb_publish(&status[(int64_t)tile * 4 + c], kFlagPre, tk.epoch, excl + agg);
     }
-    excl_sh[c] = excl;
+    if (lane == 0) excl_sh[c] = excl;
   }
   __syncthreads();
   pre = add4(pre, make_int4((int)excl_sh[0], (int)excl_sh[1], (int)excl_sh[2], (int)excl_sh[3]));
@@ -350,16 +391,23 @@ __global__ __launch_bounds__(kThreads) void k_bscan_reduce(const Params* __restr
   if (threadIdx.x == 0) partials[blockIdx.x] = tot;
 }
 
+// Each thread owns a contiguous run of the tile sums: one pass of
+// independent loads, one workgroup scan of the run totals, then the run's
+// exclusive prefixes (one dependent round instead of one per 256 tiles).
 __global__ __launch_bounds__(kThreads) void k_bscan_top(int4* __restrict__ partials, int nparts) {
   __shared__ int4 wsum[kThreads / 64];
-  int4 carry = make_int4(0, 0, 0, 0);
-  for (int base = 0; base < nparts; base += kThreads) {
-    const int i = base + threadIdx.x;
-    const int4 v = i < nparts ? partials[i] : make_int4(0, 0, 0, 0);
-    int4 tot;
-    const int4 ex = block_exscan4(v, &tot, wsum);
-    if (i < nparts) partials[i] = add4(ex, carry);
-    carry = add4(carry, tot);
+  const int per = (nparts + kThreads - 1) / kThreads;
+  const int i0 = min(nparts, (int)threadIdx.x * per), i1 = min(nparts, i0 + per);
+  int4 acc = make_int4(0, 0, 0, 0);
+#pragma unroll 8
+  for (int i = i0; i < i1; i++) acc = add4(acc, partials[i]);
+  int4 tot;
+  int4 run = block_exscan4(acc, &tot, wsum);
+#pragma unroll 8
+  for (int i = i0; i < i1; i++) {
+    const int4 v = partials[i];
+    partials[i] = run;
+    run = add4(run, v);
   }
 }
 

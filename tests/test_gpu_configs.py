@@ -4,12 +4,13 @@ builds, same generators and seeds), against the oracle.
 3D four-material plain (1M particles, 128^3):
   * one substep from a pre-deformed state (F = R diag(1 +- 0.05) Q^T,
     C ~ N(0, 5), Jp as in the small parity tests, the workload's velocities):
-    x, F, Jp, grid mass and grid momentum within 1e-5 of the float32
-    oracle; v and C within 3e-5 of it and no further from the float64 oracle
-    than 1.5x the float32 oracle is (v and C are averages of the grid
-    velocity that cancel most of its magnitude, so their float32 rounding
-    is ~1e-5 relative at this size: the float32 oracle is 0.7-1.3e-5 from
-    float64 here);
+    x, v, C, F and Jp within 1e-5 of the float64 oracle; x, F, Jp, grid
+    mass and grid momentum within 1e-5 of the float32 oracle, v and C within
+    3e-5 of it and no further from the float64 oracle than 1.5x the float32
+    oracle is (v and C are averages of the grid velocity that cancel most of
+    its magnitude, so their float32 rounding is ~1e-5 relative at this size:
+    the float32 oracle is 0.7-1.3e-5 from float64 here, the HIP path
+    2-3e-6);
   * 100 substeps from the workload's initial state (at rest F = I, v ~
     U[-1, 1]^3): x within 1e-5, F and Jp within 1e-4 of both the float32
     and the float64 oracle (the float32 oracle: 1e-6, 1e-5, 2.4e-5 from
@@ -25,10 +26,12 @@ builds, same generators and seeds), against the oracle.
 plane collider): raw p2g mass and momentum at full size; 50 substeps with
 the particles thrown at the plane, none ending more than one cell below it;
 three of the ellipsoids alone against the oracle on a grid window around
-them (one substep at the parity bar, then 10 substeps within 1e-4).
+them (one substep at the parity bar -- every field within 1e-5 of the
+float64 oracle; within 1e-5, C 3e-5, of the float32 oracle -- then 10
+substeps within 1e-4).
 Batched (64 scenes, 10k-500k particles each, 64^3-128^3): the batch
 bitwise equal to 64 separate solvers after 6 substeps, and one batched
-substep of three of the scenes against the oracle.
+substep of three of the scenes against the oracle at the same bar.
 Differentiable (200k particles, 128^3, 64 substeps, checkpoints every 8):
 directional derivatives in v0 (1e-3), E and nu (1e-2) against float64
 central finite differences of the oracle.
@@ -104,6 +107,8 @@ def test_dense1m_one_substep_from_deformed_state(oracle):
     for k in ("v", "C"):
         assert e32[k] <= 3e-5, (k, e32[k])
         assert e64[k] <= 1.5 * o64[k] + 1e-7, (k, e64[k], o64[k])
+    for k in ("x", "v", "C", "F", "Jp"):
+        assert e64[k] <= 1e-5, (k, e64[k])
 
 
 def _bulk(st, m):
@@ -232,10 +237,15 @@ def test_sparse16m_ellipsoids_against_the_oracle(oracle):
     dt = s.default_dt
     s.substep(dt)
     ref, *_ = oracle.substep(cfg, st, dt, 0.0, np.float32, nthreads=THREADS)
-    e = _errs(s.get_state(), ref)
+    ref64, *_ = oracle.substep(cfg, _to64(st), dt, 0.0, np.float64, nthreads=THREADS)
+    got = s.get_state()
+    e, e64 = _errs(got, ref), _errs(got, ref64)
     print("  one substep vs f32 oracle", _fmt(e), flush=True)
+    print("  one substep vs f64 oracle", _fmt(e64), flush=True)
     for k, err in e.items():
         assert err <= (3e-5 if k == "C" else 1e-5), (k, err)
+    for k, err in e64.items():
+        assert err <= 1e-5, (k, "vs f64", err)
     run = oracle.Substepper(cfg, ref, np.float32, nthreads=THREADS)
     for _ in range(10):
         s.substep(dt)
@@ -293,12 +303,19 @@ def test_batched64_three_scenes_against_the_oracle(oracle):
         sc = b[k]
         print(f"batched64 scene {k}: {len(specs[k]['x'])} particles, res {specs[k]['res']}",
               flush=True)
-        ref, *_ = oracle.substep(oracle_cfg_for(oracle, sc), states[k], sc.default_dt, 0.0,
-                                 np.float32, nthreads=THREADS)
-        e = _errs(sc.get_state(), ref)
+        cfg = oracle_cfg_for(oracle, sc)
+        ref, *_ = oracle.substep(cfg, states[k], sc.default_dt, 0.0, np.float32,
+                                 nthreads=THREADS)
+        ref64, *_ = oracle.substep(cfg, _to64(states[k]), sc.default_dt, 0.0, np.float64,
+                                   nthreads=THREADS)
+        got = sc.get_state()
+        e, e64 = _errs(got, ref), _errs(got, ref64)
         print("  vs f32 oracle", _fmt(e), flush=True)
+        print("  vs f64 oracle", _fmt(e64), flush=True)
         for f, err in e.items():
             assert err <= (3e-5 if f == "C" else 1e-5), (k, f, err)
+        for f, err in e64.items():
+            assert err <= 1e-5, (k, f, "vs f64", err)
     b.close()
 
 

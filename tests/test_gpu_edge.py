@@ -4,7 +4,8 @@
     work on an empty solver (write_particles() raises ValueError, as the
     reference's range computation does);
   * one particle: one substep from a pre-deformed state of every material
-    within the parity bar (1e-5; C 3e-5);
+    within the parity bar (1e-5 of the float64 oracle; of the float32
+    oracle 1e-5, C 3e-5);
   * positions on cell and block faces and where floor(x inv_dx - 0.5)
     switches (k dx and (k + 0.5) dx exactly, and one float step either
     side): block keys and permutation bit-exact against the oracle and one
@@ -64,10 +65,14 @@ def test_single_particle_each_material(oracle, dim):
         dt = s.default_dt
         s.substep(dt)
         got = s.get_state()
-        ref, *_ = oracle.substep(oracle_cfg_for(oracle, s), st, dt, 0.0, np.float32)
+        cfg = oracle_cfg_for(oracle, s)
+        ref, *_ = oracle.substep(cfg, st, dt, 0.0, np.float32)
+        st64 = {k: (np.asarray(a, np.float64) if k != "material" else a) for k, a in st.items()}
+        ref64, *_ = oracle.substep(cfg, st64, dt, 0.0, np.float64)
         for k, tol in TOL.items():
-            e = rel_err(got[k], ref[k])
+            e, e64 = rel_err(got[k], ref[k]), rel_err(got[k], ref64[k])
             assert e <= tol, (mat, k, e)
+            assert e64 <= 1e-5, (mat, k, "vs f64", e64)
 
 
 def test_positions_on_cell_and_block_boundaries(oracle):

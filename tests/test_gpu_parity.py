@@ -4,12 +4,17 @@ Every test drives the HIP path through the C ABI (ctypes, via the drop-in
 MPMSolver) on seeded inputs and compares with the oracle on the same inputs.
 Tolerances (float32 arithmetic, different summation order on the grid):
   * one substep from a pre-deformed state: norm-wise relative error
-    <= 1e-5 per field for x, v, F, Jp, grid mass and grid velocity, and
-    <= 3e-5 for the affine velocity C.  C is a grid-velocity gradient
-    (4 inv_dx sum w v_i (x_i - x_p)^T) and amplifies grid-velocity rounding;
-    on these stressed states the float32 oracle itself is ~1e-5 away from
-    the float64 oracle in C, so the test also requires the HIP path to be at
-    least as close to float64 as the float32 oracle is (within 2x);
+    <= 1e-5 per field (x, v, C, F, Jp) against the float64 oracle, and
+    against the float32 oracle <= 1e-5 for x, v, F, Jp, grid mass and grid
+    velocity and <= 3e-5 for the affine velocity C.  C is a grid-velocity
+    gradient (4 inv_dx sum w v_i (x_i - x_p)^T) and amplifies the rounding
+    of the grid sums: two float32 runs that sum in different orders (the
+    oracle's particle order against the HIP path's node tiles) differ in C
+    by up to ~1.3e-5 on these stressed states, while each is within ~1e-5
+    of float64 (the float32 oracle itself is 2-8e-6 from float64 in C, the
+    HIP path 2-3e-6), so the 1e-5 bar on C is held against float64; the
+    test also requires the HIP path to be at least as close to float64 as
+    the float32 oracle is (within 2x);
   * binning keys and permutation: bit-exact;
   * 100 substeps: x and v within 1e-4 of the float64 oracle and within
     2e-4 of the float32 oracle (the float32 oracle is itself 1.3e-4 from
@@ -80,6 +85,7 @@ def test_one_substep_from_deformed_state(oracle, dim):
         e_gpu, e_32 = rel_err(got[k], ref64[k]), rel_err(ref[k], ref64[k])
         print(f"  {k}: hip vs f64 {e_gpu:.2e}, f32 oracle vs f64 {e_32:.2e}", flush=True)
         assert e_gpu <= 2 * e_32 + 1e-7, k
+        assert e_gpu <= TOL, (k, e_gpu)
     # grid after the grid operator over the oracle's window
     m, v = s.read_grid([-8] * dim, [res + 16] * dim)
     assert rel_err(m, gm) <= TOL
