@@ -3,8 +3,9 @@
 between grid_normalization_and_gravity and g2p).
 
   * a callable that changes nothing leaves every particle field bitwise as
-    the fused substep does (2D/3D, plain and use_g2p2g), so splitting the
-    substep around the callables changes nothing by itself;
+    the fused substep does (2D/3D, plain and use_g2p2g; unbounded, packed
+    storage), so splitting the substep around the callables changes nothing
+    by itself;
   * a callable zeroing the velocity of the nodes below a plane gives bitwise
     the state of the built-in sticky surface collider on that plane (whose
     oracle parity is tested elsewhere): node coordinates and the write-back
@@ -35,9 +36,9 @@ def _quiet():
     mpm_solver.PRINT_PROGRESS = False
 
 
-def _scene(dim, seed=3, g2p2g=False, vy=0.0):
+def _scene(dim, seed=3, g2p2g=False, vy=0.0, **kw):
     from taichi_elements_amd import MPMSolver
-    s = MPMSolver(res=(32, ) * dim, use_g2p2g=g2p2g)
+    s = MPMSolver(res=(32, ) * dim, use_g2p2g=g2p2g, **kw)
     rng = np.random.default_rng(seed)
     n = 3000 if dim == 2 else 8000
     x = cube_particles(rng, n, [0.3] * dim, [0.3] * dim, dim)
@@ -67,6 +68,20 @@ def test_noop_callable_is_the_fused_substep(dim, g2p2g):
     assert len(calls) == b.total_substeps and min(calls) > 0
     _same_state(a, b)
     assert a.all_time_max_velocity == b.all_time_max_velocity
+
+
+@pytest.mark.parametrize("opts", [dict(unbounded=True), dict(quant=True),
+                                  dict(quant=True, g2p2g=True)])
+def test_noop_callable_with_storage_and_window_options(opts):
+    a = _scene(3, **opts)
+    b = _scene(3, **opts)
+    calls = []
+    b.grid_postprocess.append(lambda t, dt, g: calls.append(len(g)))
+    for _ in range(3):
+        a.step(2e-3)
+        b.step(2e-3)
+    assert len(calls) == b.total_substeps and min(calls) > 0
+    _same_state(a, b)
 
 
 @pytest.mark.parametrize("dim", [2, 3])
