@@ -304,14 +304,18 @@ __device__ __forceinline__ void cell_scatter(int sub, int ccnt, int cbeg, const 
 
 // --------------------------------------------------------------------------
 // SVD F = U diag(s) V^T, U, V in SO(d), s decreasing, s[d-1] signed.
-// Replaces ti.svd (mpm_solver.py:525).  3x3: Jacobi eigen-analysis of F^T F
-// with the approximate Givens angles of McAdams et al. 2011 (UW-Madison
-// TR1690), 5 cyclic sweeps, the rotations accumulated as a quaternion (so V
-// is exactly orthonormal after one final normalisation), a column sort and a
-// Givens QR of F V.  On the device the Jacobi sweeps and the quaternion
-// normalisation use the hardware reciprocal-square-root estimate
-// (fast_rsqrt; the host build, used by mpm_host_svd, divides by sqrtf); the
-// Givens QR divides by sqrtf on both.
+// Replaces ti.svd (mpm_solver.py:525).  3x3: cyclic Jacobi eigen-analysis
+// of F^T F with exact rotation angles in the form of Rutishauser (Numer.
+// Math. 9, 1966: t = tan theta from the 2x2 subproblem, the off-diagonal
+// entry set to zero); three sweeps leave the off-diagonal part of
+// V^T F^T F V at ~1e-7 of its norm, as five sweeps of the approximate
+// quaternion rotations of McAdams et al. 2011 (UW-Madison TR1690) did, for
+// about half the arithmetic.  The rotations are accumulated in V, then one
+// Newton-Schulz step V (3 I - V^T V) / 2 makes V orthonormal to rounding;
+// a column sort and a Givens QR of F V give U and the singular values.  On the device the rotation angles use the
+// hardware square-root, reciprocal and reciprocal-square-root estimates
+// (fast_sqrt, fast_rcp, fast_rsqrt; the host build, used by mpm_host_svd,
+// uses sqrtf and divisions); the Givens QR divides by sqrtf on both.
 // --------------------------------------------------------------------------
 __host__ __device__ __forceinline__ float fast_rsqrt(float x) {
 #ifdef __HIP_DEVICE_COMPILE__
@@ -321,46 +325,46 @@ __host__ __device__ __forceinline__ float fast_rsqrt(float x) {
 #endif
 }
 
-// One Jacobi conjugation of the symmetric S in the (P, Q) plane; the
-// rotation is also applied to the quaternion q = (w, x, y, z) of V.
+__host__ __device__ __forceinline__ float fast_sqrt(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return __builtin_amdgcn_sqrtf(x);
+#else
+  return sqrtf(x);
+#endif
+}
+
+__host__ __device__ __forceinline__ float fast_rcp(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return __builtin_amdgcn_rcpf(x);
+#else
+  return 1.0f / x;
+#endif
+}
+
+// One Jacobi rotation of the symmetric S in the (P, Q) plane: t = tan theta
+// of the smaller rotation (|theta| <= pi/4) that zeroes S_PQ,
+// t = 2 S_PQ sgn(d) / (|d| + sqrt(d^2 + 4 S_PQ^2)), d = S_QQ - S_PP (t = 0
+// when S_PQ = d = 0); S <- G^T S G with G e_P = c e_P - s e_Q,
+// G e_Q = s e_P + c e_Q, and V <- V G.
 template <int P, int Q>
-__host__ __device__ __forceinline__ void jacobi_q(float S[3][3], float q[4]) {
+__host__ __device__ __forceinline__ void jacobi_rot(float S[3][3], float V[3][3]) {
   constexpr int R = 3 - P - Q;
-  const float gamma = 5.82842712474619f;  // 3 + 2 sqrt 2
-  const float cstar = 0.9238795325112867f, sstar = 0.3826834323650898f;  // cos, sin pi/8
-  float ch = 2.0f * (S[P][P] - S[Q][Q]), sh = S[P][Q];
-  const bool ok = gamma * sh * sh < ch * ch;
-  const float w = fast_rsqrt(ch * ch + sh * sh);
-  ch = ok ? w * ch : cstar;
-  sh = ok ? w * sh : sstar;
-  const float c = ch * ch - sh * sh, s = 2.0f * ch * sh;  // cos, sin of the full angle
-  // S <- G^T S G with G e_P = c e_P + s e_Q, G e_Q = -s e_P + c e_Q
-  const float spp = S[P][P], sqq = S[Q][Q], spq = S[P][Q];
+  const float spq = S[P][Q];
+  const float d = S[Q][Q] - S[P][P];
+  const float den = fabsf(d) + fast_sqrt(d * d + 4.0f * spq * spq);
+  const float t = den > 0.0f ? (d >= 0.0f ? 2.0f : -2.0f) * spq * fast_rcp(den) : 0.0f;
+  const float c = fast_rsqrt(1.0f + t * t), s = t * c;
+  S[P][P] -= t * spq;
+  S[Q][Q] += t * spq;
+  S[P][Q] = S[Q][P] = 0.0f;
   const float srp = S[R][P], srq = S[R][Q];
-  S[P][P] = c * c * spp + 2.0f * c * s * spq + s * s * sqq;
-  S[Q][Q] = s * s * spp - 2.0f * c * s * spq + c * c * sqq;
-  S[P][Q] = S[Q][P] = (c * c - s * s) * spq + c * s * (sqq - spp);
-  S[R][P] = S[P][R] = c * srp + s * srq;
-  S[R][Q] = S[Q][R] = -s * srp + c * srq;
-  // G is a rotation about axis R by the full angle; its sign is + for the
-  // cyclic planes (0,1), (1,2) and - for (0,2).  q <- q (x) (ch, t e_R).
-  const float t = (P == 0 && Q == 2) ? -sh : sh;
-  const float qw = q[0], qx = q[1], qy = q[2], qz = q[3];
-  if (R == 2) {
-    q[0] = ch * qw - t * qz;
-    q[1] = ch * qx + t * qy;
-    q[2] = ch * qy - t * qx;
-    q[3] = ch * qz + t * qw;
-  } else if (R == 0) {
-    q[0] = ch * qw - t * qx;
-    q[1] = ch * qx + t * qw;
-    q[2] = ch * qy + t * qz;
-    q[3] = ch * qz - t * qy;
-  } else {
-    q[0] = ch * qw - t * qy;
-    q[1] = ch * qx - t * qz;
-    q[2] = ch * qy + t * qw;
-    q[3] = ch * qz + t * qx;
+  S[R][P] = S[P][R] = c * srp - s * srq;
+  S[R][Q] = S[Q][R] = s * srp + c * srq;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const float vp = V[k][P], vq = V[k][Q];
+    V[k][P] = c * vp - s * vq;
+    V[k][Q] = s * vp + c * vq;
   }
 }
 
@@ -417,26 +421,33 @@ __host__ __device__ __forceinline__ void svd3(const float A[3][3], float U[3][3]
     for (int j = 0; j < 3; j++) {
       S[i][j] = A[0][i] * A[0][j] + A[1][i] * A[1][j] + A[2][i] * A[2][j];
       U[i][j] = i == j ? 1.0f : 0.0f;
+      V[i][j] = i == j ? 1.0f : 0.0f;
     }
-  float q[4] = {1.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-  for (int sweep = 0; sweep < 5; sweep++) {
-    jacobi_q<0, 1>(S, q);
-    jacobi_q<1, 2>(S, q);
-    jacobi_q<0, 2>(S, q);
+  for (int sweep = 0; sweep < 3; sweep++) {
+    jacobi_rot<0, 1>(S, V);
+    jacobi_rot<1, 2>(S, V);
+    jacobi_rot<0, 2>(S, V);
   }
   {
-    const float in = fast_rsqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
-    const float w = q[0] * in, x = q[1] * in, y = q[2] * in, z = q[3] * in;
-    V[0][0] = 1.0f - 2.0f * (y * y + z * z);
-    V[0][1] = 2.0f * (x * y - w * z);
-    V[0][2] = 2.0f * (x * z + w * y);
-    V[1][0] = 2.0f * (x * y + w * z);
-    V[1][1] = 1.0f - 2.0f * (x * x + z * z);
-    V[1][2] = 2.0f * (y * z - w * x);
-    V[2][0] = 2.0f * (x * z - w * y);
-    V[2][1] = 2.0f * (y * z + w * x);
-    V[2][2] = 1.0f - 2.0f * (x * x + y * y);
+    // V <- V (3 I - V^T V) / 2: the rounding of the accumulated rotations
+    // leaves V^T V = I + O(1e-7); one Newton-Schulz step squares that
+    float M[3][3], W[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = i; j < 3; j++) {
+        const float g = V[0][i] * V[0][j] + V[1][i] * V[1][j] + V[2][i] * V[2][j];
+        M[i][j] = M[j][i] = (i == j ? 1.5f : 0.0f) - 0.5f * g;
+      }
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) W[i][j] = V[i][0] * M[0][j] + V[i][1] * M[1][j] + V[i][2] * M[2][j];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) V[i][j] = W[i][j];
   }
   float B[3][3];
 #pragma unroll

@@ -108,7 +108,10 @@ def test_adaptive_step_follows_the_reference_loop(oracle, dim, g2p2g):
           " ".join(f"{d / uniform:.3f}" for d in dts), flush=True)
     assert len(dts) == len(ref_dts) == s.total_substeps
     assert dts[1] < 0.95 * uniform  # the CFL limit is active from the second substep
-    np.testing.assert_allclose(dts, ref_dts, rtol=1e-5)
+    # the last substep of a frame takes what is left of it, a difference of
+    # nearly equal float64 sums: its error is bounded against the uniform dt
+    # (a 1e-7 relative difference of the grid maxima moves it ~1e-5 relative)
+    np.testing.assert_allclose(dts, ref_dts, rtol=1e-5, atol=1e-5 * uniform)
     got = s.get_state()
     for f in ("x", "v"):
         e = rel_err(got[f], ref.s[f])

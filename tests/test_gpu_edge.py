@@ -212,6 +212,7 @@ def test_free_fall_stays_with_the_float64_oracle(oracle, dim, mat, res):
     e = rel_err(s.get_state()["v"], run.s["v"])
     o = rel_err(run32.s["v"], run.s["v"])
     print(f"free fall dim={dim} material={mat}: v vs f64 {e:.2e} (f32 oracle {o:.2e})", flush=True)
-    # 3D float SVDs (5 approximate Jacobi sweeps, in the float32 oracle too)
-    # carry their own noise: no further from float64 than twice theirs
+    # 3D float SVDs (the engine's exact-angle Jacobi, the float32 oracle's
+    # approximate sweeps) carry their own noise: no further from float64
+    # than twice the float32 oracle
     assert e <= max(2e-5, 2 * o)

@@ -1,6 +1,6 @@
 """The SVD as the kernels run it on the GPU (mpm_device_svd: the Jacobi
-sweeps with the hardware reciprocal square root, which the host build of
-the same code does not use) against LAPACK (numpy.linalg.svd) and
+rotations with the hardware square root, reciprocal and reciprocal square
+root estimates, which the host build of the same code does not use) against LAPACK (numpy.linalg.svd) and
 scipy.linalg.polar, on near-identity, random, diagonal, rotated, rank-
 deficient, repeated-singular-value and reflected matrices.  Bars as the
 host check of the same code: reconstruction 2e-5 of the matrix's scale,
