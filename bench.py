@@ -129,16 +129,17 @@ def batched_bench(steps, warmup, autotune=True):
                             np.zeros(len(sp["x"]), np.int32))
         probe.autotune(substeps=3)
         chunk, groups = probe.tuning()
+        waves = probe.g2p_waves()
         probe.close()
         del probe
-        b.set_tuning(chunk, groups)
-        tuning = dict(chunk=chunk, groups=groups)
+        b.set_tuning(chunk, groups, waves)
+        tuning = dict(chunk=chunk, groups=groups, g2p_waves=waves)
     solos = []
     for sp in specs:
         s = MPMSolver(res=(sp["res"], ) * 3)
         s.read_restart(len(sp["x"]), sp["x"], sp["v"], sp["m"], np.zeros(len(sp["x"]), np.int32))
         if tuning:
-            s.set_tuning(tuning["chunk"], tuning["groups"])
+            s.set_tuning(tuning["chunk"], tuning["groups"], tuning["g2p_waves"])
         solos.append(s)
     print(f"[bench] batched: {len(specs)} scenes, {total} particles", file=sys.stderr, flush=True)
 
@@ -311,7 +312,7 @@ def variant_bench(x, v, m, steps, warmup, autotune, workload, **solver_kw):
     names, calls, ms = s.kernel_times()
     check(L.mpm_set_timing(s._handle, 0))
     return dict(value=len(x) * steps / el, ms_per_substep=1000 * el / steps,
-                unit="particle-substeps/s", tuning=dict(zip(("chunk", "groups"), s.tuning())),
+                unit="particle-substeps/s", tuning=dict(zip(("chunk", "groups"), s.tuning()), g2p_waves=s.g2p_waves()),
                 bytes_per_particle=s.particle_bytes(),
                 kernels_us={nm: round(1000 * t / max(c, 1), 2) for nm, c, t in zip(names, calls, ms)},
                 workload=workload)
@@ -460,8 +461,10 @@ def plain_leg(config, x, v, m, steps, warmup, autotune, reorder_interval=0, chun
     tuning = None
     if autotune and not chunk:
         cand = s.autotune(substeps=3)
-        tuning = dict(chosen=dict(zip(("chunk", "groups"), s.tuning())),
-                      ms_per_substep={f"{c}/{g}": round(t, 4) for (c, g), t in cand.items()})
+        tuning = dict(chosen=dict(zip(("chunk", "groups"), s.tuning()), g2p_waves=s.g2p_waves()),
+                      ms_per_substep={f"{c}/{g}": round(t, 4) for (c, g), t in cand.items()},
+                      g2p_waves_ms={str(w): round(t, 4)
+                                    for w, t in getattr(s, "last_autotune_g2p_waves", {}).items()})
         print(f"[bench] {config}: autotuned launch shape {tuning['chosen']}", file=sys.stderr,
               flush=True)
 

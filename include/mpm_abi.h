@@ -247,13 +247,20 @@ int mpm_set_g2p2g(mpm_solver* s, int32_t enabled, double allowed_cfl, int32_t cl
 int mpm_set_graphs(mpm_solver* s, int32_t enabled);
 
 /* Launch tuning (no reference counterpart; the reference leaves launch
- * shapes to Taichi).  chunk: particles per transfer work item (16-256);
+ * shapes to Taichi).  chunk: particles per transfer work item (16-1024);
  * groups: workgroups of the persistent transfer kernels (64-16384); 0 keeps
  * a value.  The next substep re-bins from scratch; results change only by
  * float32 summation order.  MPMSolver.autotune() times candidates on the
  * live scene and keeps the fastest. */
 int mpm_set_tuning(mpm_solver* s, int32_t chunk, int32_t groups);
 int mpm_get_tuning(mpm_solver* s, int32_t* chunk, int32_t* groups);
+/* g2p's occupancy (no reference counterpart): waves = 0 lets the compiler
+ * allocate registers freely (5 waves per SIMD), 6 holds g2p to six waves
+ * per SIMD (more grid-tile gathers in flight, which pays in sparse scenes
+ * whose blocks hold few particles).  Results are bitwise the same either
+ * way; MPMSolver.autotune() times both at the chosen launch shape. */
+int mpm_set_g2p_waves(mpm_solver* s, int32_t waves);
+int mpm_get_g2p_waves(mpm_solver* s, int32_t* waves);
 
 /* Device-side snapshot of the simulated state (no reference counterpart;
  * MPMSolver.autotune uses it to put the scene back after timing launch

@@ -70,6 +70,8 @@ SIGNATURES = [
     ("mpm_set_graphs", ctypes.c_int, [_P, ctypes.c_int32]),
     ("mpm_set_tuning", ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),
     ("mpm_get_tuning", ctypes.c_int, [_P, _I32P, _I32P]),
+    ("mpm_set_g2p_waves", ctypes.c_int, [_P, ctypes.c_int32]),
+    ("mpm_get_g2p_waves", ctypes.c_int, [_P, _I32P]),
     ("mpm_snapshot", ctypes.c_int, [_P, ctypes.c_int32]),
     ("mpm_device_svd", ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, _FP, _FP,
                                       _FP, _FP]),

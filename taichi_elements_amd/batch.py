@@ -294,11 +294,16 @@ class MPMSolverBatch:
     def n_particles_total(self):
         return self._total_particles()
 
-    def set_tuning(self, chunk=0, groups=0):
+    def set_tuning(self, chunk=0, groups=0, g2p_waves=None):
         check(_lib.lib().mpm_set_tuning(self._handle, int(chunk), int(groups)))
+        if g2p_waves is not None:
+            check(_lib.lib().mpm_set_g2p_waves(self._handle, int(g2p_waves)))
 
     def tuning(self):
         return MPMSolver.tuning(self)
+
+    def g2p_waves(self):
+        return MPMSolver.g2p_waves(self)
 
     def autotune(self, substeps=4, chunks=None, groups=(1024, 2048, 4096),
                  verbose=False):

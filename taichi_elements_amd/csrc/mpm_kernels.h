@@ -64,6 +64,7 @@ struct SubstepBuffers {
   int key_bits;
   int chunk;
   int persistent_groups;
+  int g2p_waves;  // g2p's occupancy target: 0 = the compiler's choice, 6 = six waves per SIMD
   int parity;  // substep counter parity (selects the digit-histogram copy)
   // neighbour-bounded sort: the previous substep's chunks and block tables,
   // direction codes / new keys per previous sorted position, per-chunk

@@ -847,8 +847,11 @@ __global__ __launch_bounds__(kThreads) void k_grid_op(const Params* __restrict__
 // ---------------------------------------------------------------------------
 // G2P (mpm_solver.py:694-735)
 // ---------------------------------------------------------------------------
-template <int D, bool Q = false>
-__global__ __launch_bounds__(kThreads) void k_g2p(const Params* __restrict__ scenes,
+// W: minimum waves per SIMD the register allocation must allow (1: no
+// constraint, 5 waves at its register count; 6: at most 80 registers, more
+// grid-tile gathers in flight where each block holds few particles).
+template <int D, bool Q = false, int W = 1>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(W))) void k_g2p(const Params* __restrict__ scenes,
                                                   const int* __restrict__ slab,
                                                   float* __restrict__ pf,
                                                   const int* __restrict__ pi, int64_t cap,
@@ -1502,7 +1505,8 @@ void launch_g2p(const Params* P, const int* slab, SubstepBuffers& B, int s_index
   const int chunk_bound = std::min(B.total_blocks, n) + div_up(n, B.chunk);
   {
     TimerScope ts(tm, "k_g2p", st);
-    auto kern = B.quant ? k_g2p<D, true> : k_g2p<D, false>;
+    auto kern = B.g2p_waves >= 6 ? (B.quant ? k_g2p<D, true, 6> : k_g2p<D, false, 6>)
+                                 : (B.quant ? k_g2p<D, true, 1> : k_g2p<D, false, 1>);
     kern<<<clamp_groups(chunk_bound, G), kThreads, 0, st>>>(
         P, slab, B.pf, B.pi, B.cap, B.perm, B.chunks, B.totals, B.grid,
         B.maxv, B.step, B.persistent_groups, B.code, B.nkey, B.dcount, B.count_next, B.err,

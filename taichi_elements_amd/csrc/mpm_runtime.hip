@@ -177,6 +177,7 @@ struct mpm_solver {
   };
   std::vector<GraphEntry> graphs;
   int reorder_interval = 16;
+  int g2p_waves = 0;   // g2p occupancy target (mpm_set_g2p_waves)
   int chunk = kBatch;  // particles per transfer work item (autotuned up to kChunkMax)
   int persistent_groups = 2048;
   int debug = 0;
@@ -481,6 +482,7 @@ void fill_buffers(mpm_solver* s, SubstepBuffers& B) {
   B.key_bits = s->key_bits;
   B.chunk = s->eff_chunk();
   B.persistent_groups = s->persistent_groups;
+  B.g2p_waves = s->g2p_waves;
   B.parity = (int)(s->substep_counter & 1);
 }
 
@@ -554,7 +556,7 @@ std::vector<intptr_t> graph_key(const mpm_solver* s) {
           (intptr_t)s->grid_alt, (intptr_t)s->maxv, (intptr_t)s->tiles, (intptr_t)s->chunks,
           (intptr_t)s->keys_alt, (intptr_t)s->dcount, (intptr_t)s->doff, (intptr_t)s->n,
           (intptr_t)s->cap, (intptr_t)s->debug, (intptr_t)s->chunk,
-          (intptr_t)s->persistent_groups, (intptr_t)s->g2p2g};
+          (intptr_t)s->persistent_groups, (intptr_t)s->g2p2g, (intptr_t)s->g2p_waves};
 }
 
 // Two substeps as one graph launch: captured on first use (the capture
@@ -1933,6 +1935,20 @@ int mpm_snapshot(mpm_solver* s, int32_t op) {
   // particle loads scattered): reorder at the first opportunity
   s->since_reorder = 1 << 30;
   HIP_TRY(hipStreamSynchronize(s->stream));
+  return MPM_OK;
+}
+
+int mpm_set_g2p_waves(mpm_solver* s, int32_t waves) {
+  if (!s) return fail(MPM_EINVAL, "null argument");
+  TRY(no_half_pending(s));
+  if (waves != 0 && waves != 6) return fail(MPM_EINVAL, "g2p waves per SIMD must be 0 or 6");
+  s->g2p_waves = waves;
+  return MPM_OK;
+}
+
+int mpm_get_g2p_waves(mpm_solver* s, int32_t* waves) {
+  if (!s || !waves) return fail(MPM_EINVAL, "null argument");
+  *waves = s->g2p_waves;
   return MPM_OK;
 }
 

@@ -188,6 +188,25 @@ def _autotune(obj, handle, dt, substeps, chunks, groups, verbose, dts):
                 print(f"chunk {ch:4d} groups {gr:5d}: {results[(ch, gr)]:.4f} ms/substep")
     best = min(results, key=results.get)
     check(L.mpm_set_tuning(handle, int(best[0]), int(best[1])))
+    # g2p's occupancy at the chosen shape (results are bitwise the same)
+    waves = {}
+    for w in (0, 6):
+        check(L.mpm_set_g2p_waves(handle, w))
+        check(L.mpm_snapshot(handle, SNAPSHOT_RESTORE))
+        run(4)
+        el = None
+        for _ in range(3):
+            t0 = time.perf_counter()
+            run(substeps)
+            dt_w = time.perf_counter() - t0
+            el = dt_w if el is None else min(el, dt_w)
+        waves[w] = 1000 * el / substeps
+        if verbose:
+            print(f"chunk {best[0]:4d} groups {best[1]:5d} g2p waves {w}: {waves[w]:.4f} ms/substep")
+    w_best = min(waves, key=waves.get)
+    check(L.mpm_set_g2p_waves(handle, w_best))
+    results[best] = min(results[best], waves[w_best])
+    obj.last_autotune_g2p_waves = waves
     check(L.mpm_snapshot(handle, SNAPSHOT_RESTORE))
     check(L.mpm_snapshot(handle, SNAPSHOT_RELEASE))
     return results
@@ -889,8 +908,18 @@ class MPMSolver:
         check(_lib.lib().mpm_get_tuning(self._h(), ch, gr))
         return int(ch.value), int(gr.value)
 
-    def set_tuning(self, chunk=0, groups=0):
+    def set_tuning(self, chunk=0, groups=0, g2p_waves=None):
+        """chunk / groups as tuning() reports them (0 keeps a value);
+        g2p_waves: 0 (the compiler's occupancy) or 6 waves per SIMD."""
         check(_lib.lib().mpm_set_tuning(self._h(), int(chunk), int(groups)))
+        if g2p_waves is not None:
+            check(_lib.lib().mpm_set_g2p_waves(self._h(), int(g2p_waves)))
+
+    def g2p_waves(self):
+        """g2p's occupancy target (0: the compiler's choice, or 6)."""
+        w = _lib.ctypes.c_int32()
+        check(_lib.lib().mpm_get_g2p_waves(self._h(), w))
+        return int(w.value)
 
     def particle_bytes(self):
         """Bytes per particle in this storage mode (device state + colour)."""

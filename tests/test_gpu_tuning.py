@@ -39,6 +39,8 @@ def test_autotune_restores_the_scene_and_keeps_results(use_g2p2g):
     res = s.autotune(substeps=2, chunks=(64, 256), groups=(512, 2048))
     assert len(res) == 4 and all(v > 0 for v in res.values())
     assert s.tuning() == min(res, key=res.get)
+    assert set(s.last_autotune_g2p_waves) == {0, 6}
+    assert s.g2p_waves() == min(s.last_autotune_g2p_waves, key=s.last_autotune_g2p_waves.get)
     after = s.get_state()
     for k in before:
         assert np.array_equal(before[k], after[k]), k
@@ -75,6 +77,11 @@ def test_set_tuning_checks_its_arguments():
         s.set_tuning(groups=10)
     s.set_tuning(chunk=128, groups=1024)
     assert s.tuning() == (128, 1024)
+    for w in (6, 0):
+        s.set_tuning(g2p_waves=w)
+        assert s.g2p_waves() == w and s.tuning() == (128, 1024)
+    with pytest.raises(ValueError):
+        s.set_tuning(g2p_waves=5)
     s.substep(s.default_dt)
     assert s.last_counts()["valid"] == 40000
 
@@ -110,3 +117,16 @@ def test_snapshot_restore_and_its_errors(use_g2p2g):
     assert L.mpm_snapshot(s._h(), SNAPSHOT_RESTORE) == -1
     assert L.mpm_snapshot(s._h(), SNAPSHOT_RELEASE) == 0
     assert L.mpm_snapshot(s._h(), SNAPSHOT_RESTORE) == -1
+
+
+def test_g2p_occupancy_changes_nothing_but_time():
+    """g2p at six waves per SIMD (fewer registers) runs the same arithmetic:
+    ten substeps bitwise equal to the default build's."""
+    a, b = _scene(seed=12), _scene(seed=12)
+    b.set_tuning(g2p_waves=6)
+    for _ in range(10):
+        a.substep(a.default_dt)
+        b.substep(b.default_dt)
+    sa, sb = a.get_state(), b.get_state()
+    for k in sa:
+        assert np.array_equal(sa[k], sb[k]), k
