@@ -119,10 +119,12 @@ def test_p2g_grid_mass_and_momentum(oracle, dim):
     np.testing.assert_allclose(m.sum(dtype=np.float64), mass.sum(), rtol=1e-5)
 
 
-@pytest.mark.parametrize("dim", [2, 3])
-def test_binning_keys_and_permutation_bit_exact(oracle, dim):
+@pytest.mark.parametrize("dim,res", [(2, 48), (3, 48), (3, 512)])
+def test_binning_keys_and_permutation_bit_exact(oracle, dim, res):
+    # res 512 in 3D: a window of 133^3 blocks, whose block scan runs in two
+    # levels (tile sums, one workgroup over the 2300 tile sums, then the
+    # tiles) instead of the single-pass look-back
     rng = np.random.default_rng(300 + dim)
-    res = 48
     n = 20000
     s = _solver(dim, res)
     x = cube_particles(rng, n, [0.05] * dim, [0.9] * dim, dim)
@@ -139,13 +141,14 @@ def test_binning_keys_and_permutation_bit_exact(oracle, dim):
     np.testing.assert_array_equal(keys, ref_keys[ref_perm])
 
 
-@pytest.mark.parametrize("reorder", [False, True])
-def test_neighbour_sort_is_stable_sort_of_previous_order(oracle, reorder):
+@pytest.mark.parametrize("reorder,res", [(False, 32), (True, 32), (False, 512)])
+def test_neighbour_sort_is_stable_sort_of_previous_order(oracle, reorder, res):
     """From the second substep on, the binning is a stable sort of the
     previous substep's order by the new block keys (computed by g2p); with a
-    physical reorder in between, the previous order is the reordered slots."""
+    physical reorder in between, the previous order is the reordered slots.
+    res 512: the two-level block scan under the neighbour sort."""
     rng = np.random.default_rng(310)
-    s = _solver(3, 32, reorder_interval=1 if reorder else -1)
+    s = _solver(3, res, reorder_interval=1 if reorder else -1)
     n = 20000
     x = cube_particles(rng, n, [0.2] * 3, [0.5] * 3, 3)
     v = (3 * rng.uniform(-1, 1, (n, 3))).astype(np.float32)
