@@ -195,6 +195,9 @@ class MPMSolverBatch:
         dts = [float(dt) for dt in dts]
         if len(dts) != len(self.scenes):
             raise ValueError(f"expected {len(self.scenes)} time steps, got {len(dts)}")
+        if any(callable(p) for sc in self.scenes for p in sc.grid_postprocess):
+            raise NotImplementedError("callables in grid_postprocess run with a single "
+                                      "MPMSolver; a batch runs the built-in post-processes only")
         dts32 = np.ascontiguousarray(np.asarray(dts, np.float32))
         L = _lib.lib()
         check(L.mpm_substeps_batch(self._handle, n, _lib.fptr(dts32)))

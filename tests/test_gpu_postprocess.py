@@ -190,3 +190,16 @@ def test_callable_errors():
     s.grid_postprocess.pop()
     s.step(1e-3)  # the solver runs on
     assert np.isfinite(s.get_state()["x"]).all()
+
+
+def test_batch_refuses_callables():
+    from taichi_elements_amd import MPMSolverBatch
+    b = MPMSolverBatch([dict(res=(32, 32, 32)), dict(res=(32, 32, 32))])
+    for sc in b:
+        sc.add_cube((0.3, 0.3, 0.3), (0.2, 0.2, 0.2), sc.material_water)
+    b[1].grid_postprocess.append(lambda t, dt, g: None)
+    with pytest.raises(NotImplementedError):
+        b.substeps(1)
+    b[1].grid_postprocess.pop()
+    b.substeps(1)
+    b.close()
