@@ -742,7 +742,6 @@ class MPMSolver:
         L = _lib.lib()
         h = self._h()
         dev = torch.device("cuda", int(self._device))
-        stream = torch.cuda.current_stream(dev)
         out = np.zeros(max(count, 1), np.float32)
         t = self.t
         for k in range(count):
@@ -755,7 +754,7 @@ class MPMSolver:
                 I = torch.empty((n, self.dim), dtype=torch.int32, device=dev)
                 v = torch.empty((n, self.dim), dtype=torch.float32, device=dev)
                 m = torch.empty((n,), dtype=torch.float32, device=dev)
-                stream.synchronize()  # the gather runs on the solver's own stream
+                torch.cuda.synchronize(dev)  # the gather runs on the solver's own stream
                 check(L.mpm_grid_gather(h, _tptr(I), _tptr(v), _tptr(m), n))
                 nodes = GridNodes(I, v, m, self.dx)
                 for fn in hooks:
@@ -766,7 +765,8 @@ class MPMSolver:
                     raise ValueError(f"grid_postprocess: grid_v.v must stay a float32 ({n}, "
                                      f"{self.dim}) tensor on {dev}")
                 nv = nv.contiguous()
-                stream.synchronize()  # the callables' writes, before the scatter reads them
+                # the callables' work on any stream, before the scatter reads v
+                torch.cuda.synchronize(dev)
                 check(L.mpm_grid_scatter(h, _tptr(nv), n))
                 failed = False
             finally:
