@@ -19,7 +19,8 @@ between grid_normalization_and_gravity and g2p).
     v_p = A x_p + b and C_p = A (mpm_solver.py:694-724);
   * errors: a callable that leaves a wrong-shaped v raises ValueError and
     the solver runs on; other calls on the solver inside a callable raise
-    ValueError.
+    ValueError;
+  * the split-substep C entry points refuse calls out of order.
 """
 import numpy as np
 import pytest
@@ -203,3 +204,34 @@ def test_batch_refuses_callables():
     b[1].grid_postprocess.pop()
     b.substeps(1)
     b.close()
+
+
+def test_split_substep_entry_points_check_their_order():
+    """mpm_substep_begin / mpm_substep_end through the C ABI: a second begin,
+    an end or a node query without a begin, a gather with the wrong node
+    count and other calls on the handle in between are MPM_EINVAL; a
+    begin / gather / scatter / end cycle then runs."""
+    from taichi_elements_amd import _lib
+    L = _lib.lib()
+    s = _scene(3)
+    h = s._h()
+    n = _lib.ctypes.c_int64()
+    assert L.mpm_substep_end(h) == _lib.MPM_EINVAL
+    assert L.mpm_grid_node_count(h, _lib.ctypes.byref(n)) == _lib.MPM_EINVAL
+    dt = s.default_dt
+    assert L.mpm_substep_begin(h, dt) == _lib.MPM_OK
+    assert L.mpm_substep_begin(h, dt) == _lib.MPM_EINVAL
+    assert L.mpm_substeps(h, 1, dt, 0.0) == _lib.MPM_EINVAL
+    assert L.mpm_grid_node_count(h, _lib.ctypes.byref(n)) == _lib.MPM_OK and n.value > 0
+    I = torch.empty((n.value, 3), dtype=torch.int32, device="cuda")
+    v = torch.empty((n.value, 3), dtype=torch.float32, device="cuda")
+    m = torch.empty((n.value, ), dtype=torch.float32, device="cuda")
+    ptr = lambda t: _lib.ctypes.c_void_p(t.data_ptr())
+    torch.cuda.synchronize()
+    assert L.mpm_grid_gather(h, ptr(I), ptr(v), ptr(m), n.value - 1) == _lib.MPM_EINVAL
+    assert L.mpm_grid_gather(h, ptr(I), ptr(v), ptr(m), n.value) == _lib.MPM_OK
+    assert L.mpm_grid_scatter(h, ptr(v), n.value) == _lib.MPM_OK
+    assert L.mpm_substep_end(h) == _lib.MPM_OK
+    assert L.mpm_substep_end(h) == _lib.MPM_EINVAL
+    assert L.mpm_sync(h) == _lib.MPM_OK
+    assert abs(float(m.sum()) - s.n_particles[None] * s.p_mass) <= 1e-5 * s.n_particles[None] * s.p_mass
