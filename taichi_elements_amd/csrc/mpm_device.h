@@ -104,19 +104,20 @@ __device__ __forceinline__ void base_fx(float xa, float inv_dx, int& base, float
   fx = t - (float)base;
 }
 
-// Contributions w_o (q + Ad o, mass) of one particle to the three nodes
-// o = (0..2, oy, oz) of one x line of its stencil (p2g, mpm_solver.py:
-// 575-584, with the affine term folded as q = mass v - Ad fx, Ad = A dx),
-// accumulated into c[0..2].  The tensor-product weight is factored: with
-// wyz = wy[oy] wz[oz], P = wyz (q + Ad (0, oy, oz)) and Q = wyz Ad e_x, node
-// ox receives wx[ox] (P + ox Q) -- a third of the arithmetic of evaluating
-// every node on its own.  oy and oz are compile-time after unrolling.
+// One particle's share of one x line of its stencil, the nodes o = (0..2,
+// oy, oz) (p2g, mpm_solver.py:575-584, with the affine term folded as
+// q = mass v - Ad fx, Ad = A dx): node ox receives wn[ox] (r[ox], mass) with
+// the node weight wn[ox] = wx[ox] (wy[oy] wz[oz]) and r[ox] = q + Ad o,
+// stepped along x as b, b + Ad e_x, b + 2 Ad e_x from b = q + Ad (0, oy, oz).
+// The caller adds these with one fma per channel straight into the node it
+// read from the tile.  oy and oz are compile-time after unrolling.
 template <int D>
-__device__ __forceinline__ void line_contrib(int oy, int oz, const float Ad[3][3],
-                                             const float q[3], float mass, const float w[3][3],
-                                             float4 c[3]) {
+__device__ __forceinline__ void line_terms(int oy, int oz, const float Ad[3][3],
+                                           const float q[3], const float w[3][3],
+                                           float wn[3], float r[3][3]) {
   const float wyz = D == 3 ? w[1][oy] * w[2][oz] : w[1][oy];
-  float r1[3], r2[3], r0[3];
+#pragma unroll
+  for (int ox = 0; ox < 3; ox++) wn[ox] = w[0][ox] * wyz;
 #pragma unroll
   for (int a = 0; a < D; a++) {
     float b = q[a];
@@ -124,20 +125,9 @@ __device__ __forceinline__ void line_contrib(int oy, int oz, const float Ad[3][3
     if (oy == 2) b += 2.0f * Ad[a][1];
     if (D == 3 && oz == 1) b += Ad[a][2];
     if (D == 3 && oz == 2) b += 2.0f * Ad[a][2];
-    const float P = wyz * b, Q = wyz * Ad[a][0];
-    r0[a] = P;
-    r1[a] = P + Q;
-    r2[a] = r1[a] + Q;
-  }
-  const float M = wyz * mass;
-  const float* r[3] = {r0, r1, r2};
-#pragma unroll
-  for (int ox = 0; ox < 3; ox++) {
-    const float wx = w[0][ox];
-    c[ox].x += wx * r[ox][0];
-    c[ox].y += wx * r[ox][1];
-    if (D == 3) c[ox].z += wx * r[ox][2];
-    c[ox].w += wx * M;
+    r[0][a] = b;
+    r[1][a] = b + Ad[a][0];
+    r[2][a] = r[1][a] + Ad[a][0];
   }
 }
 
@@ -262,21 +252,21 @@ __device__ __forceinline__ void cell_scatter_pass(int k0, int ccnt, int cbeg,
   for (int oz = 0; oz < (D == 3 ? 3 : 1); oz++)
 #pragma unroll
     for (int oy = 0; oy < 3; oy++) {
-      float4 line[3];
+      float wn[NB][3], r[NB][3][3];
 #pragma unroll
-      for (int ox = 0; ox < 3; ox++) line[ox] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-#pragma unroll
-      for (int bi = 0; bi < NB; bi++) line_contrib<D>(oy, oz, Ad[bi], q[bi], mass[bi], w[bi], line);
+      for (int bi = 0; bi < NB; bi++) line_terms<D>(oy, oz, Ad[bi], q[bi], w[bi], wn[bi], r[bi]);
 #pragma unroll
       for (int ox = 0; ox < 3; ox++) {
-        const float4 add = line[ox];
         const int node = D == 3 ? (lc[0] + ox) + kT * ((lc[1] + oy) + kT * (lc[2] + oz))
                                 : (lc[0] + ox) + kT * (lc[1] + oy);
         float4 acc = tp[node];
-        acc.x += add.x;
-        acc.y += add.y;
-        acc.z += add.z;
-        acc.w += add.w;
+#pragma unroll
+        for (int bi = 0; bi < NB; bi++) {
+          acc.x = fmaf(wn[bi][ox], r[bi][ox][0], acc.x);
+          acc.y = fmaf(wn[bi][ox], r[bi][ox][1], acc.y);
+          if (D == 3) acc.z = fmaf(wn[bi][ox], r[bi][ox][2], acc.z);
+          acc.w = fmaf(wn[bi][ox], mass[bi], acc.w);
+        }
         tp[node] = acc;
         // Lanes of a wave update distinct nodes in one instruction, but a
         // later node of one lane can be an earlier node of another: keep
