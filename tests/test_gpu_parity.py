@@ -94,6 +94,34 @@ def test_one_substep_from_deformed_state(oracle, dim):
     assert abs(s.all_time_max_velocity - mv) <= TOL * max(1.0, mv)
 
 
+@pytest.mark.parametrize("chunk", [256, 512, 1024])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_full_chunks_of_every_length(oracle, dim, chunk):
+    """Blocks holding ~1100 particles, so that chunks run full: one, two
+    and four batches of 256 particles accumulate into one chunk's node
+    tile.  One substep against the float64 oracle (1e-5 every field) and
+    the float32 oracle (the bars above)."""
+    rng = np.random.default_rng(700 + dim)
+    res = 32
+    n = 30000 if dim == 3 else 10000
+    s = _solver(dim, res)
+    x = cube_particles(rng, n, [0.4] * dim, [0.25] * dim, dim)
+    mat = rng.integers(0, 5, n).astype(np.int32)
+    st = deformed_state(rng, x, mat, dim)
+    _load(s, st)
+    s.set_tuning(chunk=chunk)
+    dt = s.default_dt
+    s.substep(dt)
+    assert n / s.last_counts()["particle_blocks"] >= chunk  # full chunks
+    got = s.get_state()
+    cfg = oracle_cfg_for(oracle, s)
+    ref, *_ = oracle.substep(cfg, st, dt, 0.0, np.float32)
+    _compare(got, ref, label=f"{dim}D chunk {chunk}")
+    st64 = {k: (np.asarray(v, np.float64) if k != "material" else v) for k, v in st.items()}
+    ref64, *_ = oracle.substep(cfg, st64, dt, 0.0, np.float64)
+    _compare(got, ref64, tol=TOL, label=f"{dim}D chunk {chunk} vs f64")
+
+
 @pytest.mark.parametrize("dim", [2, 3])
 def test_p2g_grid_mass_and_momentum(oracle, dim):
     """Raw p2g output (debug mode: grid operator stores mass and momentum)."""
