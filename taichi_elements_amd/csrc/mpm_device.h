@@ -381,12 +381,15 @@ __host__ __device__ __forceinline__ void givens_qr(float B[3][3], float U[3][3])
   float r2 = a * a + b * b;
   float c = 1.0f, s = 0.0f;
   if (r2 > 1e-30f) {
-    // a correctly rounded square root and a correctly rounded division
-    // (within about an ulp of 1 / sqrt and unbiased), not the hardware
-    // estimate: that estimate's last-bit error leaves c^2 + s^2 off 1 in a
-    // biased way, so U and R = U V^T shrink or grow by ~1e-7, which
-    // 2 mu (F - R) F^T turns into stress at F ~ I (2D snow falling freely
-    // drifted 4e-4 from the float64 oracle with it)
+    // a correctly rounded square root, then a correctly rounded division,
+    // not the hardware estimate: that estimate's last-bit error leaves
+    // c^2 + s^2 off 1 in a biased way, so U and R = U V^T shrink or grow by
+    // ~1e-7, which 2 mu (F - R) F^T turns into stress at F ~ I (2D snow
+    // falling freely drifted 4e-4 from the float64 oracle with it).  Nor a
+    // correctly rounded 1 / sqrt (estimate plus a Newton step with an exact
+    // residual; 2D snow: 1.5e-4): with b = 0, sqrtf(fl(a^2)) returns |a|
+    // for nearly every a and a * fl(1 / |a|) rounds to +-1, so c stays at
+    // +-1; a correctly rounded 1 / sqrt(fl(a^2)) does not keep that.
     float ir = 1.0f / sqrtf(r2);
     c = a * ir;
     s = b * ir;
@@ -634,7 +637,8 @@ __device__ __forceinline__ void constitutive(int mat, const Params& P, float mu_
     for (int i = 0; i < D; i++) {
       float ns = sig[i];
       if (mat == kSnow) ns = fminf(fmaxf(sig[i], 1.0f - 2.5e-2f), 1.0f + 4.5e-3f);
-      if (plastic) Jp *= sig[i] / ns;
+      // sig / ns is exactly 1 unless snow's clamp moved sig
+      if (plastic && mat == kSnow) Jp *= sig[i] / ns;
       sig[i] = ns;
       J *= ns;
     }
