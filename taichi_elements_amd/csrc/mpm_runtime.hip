@@ -55,8 +55,10 @@ struct EventTimer : public Timer {
       pool.pop_back();
       return e;
     }
+    // timing only (read after a stream synchronisation): no system-scope
+    // fence, whose cache write-back would be timed with the kernel before it
     hipEvent_t e;
-    hipEventCreate(&e);
+    hipEventCreateWithFlags(&e, hipEventDisableSystemFence);
     return e;
   }
   void begin(const char* name, hipStream_t st) override {
