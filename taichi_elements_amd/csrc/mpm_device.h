@@ -637,8 +637,12 @@ __device__ __forceinline__ void constitutive(int mat, const Params& P, float mu_
     for (int i = 0; i < D; i++) {
       float ns = sig[i];
       if (mat == kSnow) ns = fminf(fmaxf(sig[i], 1.0f - 2.5e-2f), 1.0f + 4.5e-3f);
-      // sig / ns is exactly 1 unless snow's clamp moved sig
-      if (plastic && mat == kSnow) Jp *= sig[i] / ns;
+      if (plastic) {
+        if (mat == kSnow)
+          Jp *= sig[i] / ns;
+        else if (!(fabsf(sig[i]) > 0.0f && fabsf(sig[i]) < INFINITY))
+          Jp = NAN;  // sig / sig: exactly 1 unless sig is 0, infinite or NaN
+      }
       sig[i] = ns;
       J *= ns;
     }

@@ -122,6 +122,33 @@ def test_full_chunks_of_every_length(oracle, dim, chunk):
     _compare(got, ref64, tol=TOL, label=f"{dim}D chunk {chunk} vs f64")
 
 
+def test_singular_F_makes_Jp_nan_as_the_reference(oracle):
+    """Jp *= sig / clamp(sig) (mpm_solver.py:527-536): for elastic and
+    stationary particles the ratio is sig / sig, exactly 1 unless sig is 0,
+    where the reference's 0 / 0 makes Jp NaN.  Particles 0 and 1 (elastic,
+    stationary) get F = 0; the oracle and the HIP path agree on NaN there
+    and within the bars elsewhere."""
+    rng = np.random.default_rng(41)
+    dim, res, n = 3, 32, 4000
+    s = _solver(dim, res)
+    x = cube_particles(rng, n, [0.3] * dim, [0.4] * dim, dim)
+    mat = rng.integers(0, 4, n).astype(np.int32)
+    mat[0], mat[1] = 1, 4
+    st = deformed_state(rng, x, mat, dim)
+    st["F"][:2] = 0.0
+    _load(s, st)
+    dt = s.default_dt
+    s.substep(dt)
+    got = s.get_state()
+    cfg = oracle_cfg_for(oracle, s)
+    ref, *_ = oracle.substep(cfg, st, dt, 0.0, np.float32)
+    assert np.isnan(ref["Jp"][:2]).all() and np.isnan(got["Jp"][:2]).all()
+    assert np.isfinite(got["Jp"][2:]).all()
+    keep = np.arange(n) >= 2
+    _compare({k: v[keep] for k, v in got.items()}, {k: v[keep] for k, v in ref.items()},
+             fields=("x", "v", "F", "Jp"), label="singular F")
+
+
 @pytest.mark.parametrize("dim", [2, 3])
 def test_p2g_grid_mass_and_momentum(oracle, dim):
     """Raw p2g output (debug mode: grid operator stores mass and momentum)."""
